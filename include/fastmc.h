@@ -1,0 +1,205 @@
+/*
+ * fastmc.h -- C-ABI of the MI355X GCMC trial-move energy library
+ * (libfastmc_hip.so).  Plain C types only: no torch, no HIP types.
+ *
+ * The reference (FastMC, Fortran 77/90) has no plugin API; its seam is the
+ * Fortran procedure interface of module mc_moves plus the module globals it
+ * reads and writes.  Each entry point below names the reference procedure it
+ * stands in for.  A Fortran caller binds them with iso_c_binding interfaces;
+ * every pointer is caller-owned and every index that the reference counts
+ * from 1 (guest type, molecule choice) is also passed 1-based here.
+ *
+ * Error convention: every function returns 0 on success or a positive code.
+ * Where the reference would have called error(idnode, code) (error.f) the
+ * same code is returned instead of terminating the process; codes >= 9000
+ * are this library's own (HIP runtime failures, bad arguments).
+ */
+#ifndef FASTMC_H
+#define FASTMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMC_MAX_SPECIES 8  /* molecule types (maxmls), framework + guests */
+#define FMC_MAX_SITES 8    /* sites of one guest molecule (reference cap mxguestsite=50) */
+#define FMC_MAX_TYPES 24   /* unique atom labels (ntpatm) */
+#define FMC_NMOVES 8       /* ins del dis jmp swp tra rot swi, gcmc.f:341-366 */
+
+/* error codes (error.f numbering where the reference has one) */
+#define FMC_OK 0
+#define FMC_ERR_CUTOFF_WIDTH 95      /* rcut exceeds half the cell width, readinputs.f:504 */
+#define FMC_ERR_CELL 410             /* inconsistent cell vectors, readinputs.f:425-461 */
+#define FMC_ERR_ALLOC 1000           /* array allocation, utility_pack.f:1131 / ewald_module.f:83 */
+#define FMC_ERR_NO_EWALD 2311        /* no ewald directive, utility_pack.f:499-501 */
+#define FMC_ERR_INSERT_ITER 2320     /* insert_guests exceeded its attempts, mc_moves.f:125 */
+#define FMC_ERR_ARG 9001             /* invalid argument / size beyond a compiled cap */
+#define FMC_ERR_STATE 9002           /* call out of sequence (e.g. accept without a trial) */
+#define FMC_ERR_HIP 9003             /* HIP runtime failure */
+#define FMC_ERR_IMCON 9004           /* boundary condition other than imcon 1, 2, 3 */
+
+/* precision modes */
+#define FMC_FP64 0   /* every term in FP64 (parity mode) */
+#define FMC_MIXED 1  /* pair and phase arithmetic in FP32; S(k), energies, ln g in FP64 */
+
+/* move kinds as recorded in traces (gcmc.f:821-844) */
+#define FMC_MOVE_INSERT 1
+#define FMC_MOVE_DELETE 2
+#define FMC_MOVE_DISPLACE 3   /* translate + rotate (mc_displace default branch) */
+#define FMC_MOVE_TRANSLATE 4
+#define FMC_MOVE_ROTATE 5
+
+/* One molecule type of the FIELD file (readinputs.f:114-281), in FIELD order. */
+typedef struct fmc_species {
+  int is_guest;               /* 1 for an &guest block, 0 for framework */
+  int natoms;                 /* ATOMS (numatoms) */
+  int nmols;                  /* NUMMOLS (nummols) */
+  const double *site_charge;  /* [natoms] atmchg */
+  const double *site_mass;    /* [natoms] atmwght */
+  const int *site_type;       /* [natoms] ltpsit, 1-based index into the atom-label list */
+  const int *site_frozen;     /* [natoms] lfzsite */
+  const double *template_xyz; /* guests: [3*natoms] COM-centred template (guestx/y/z) */
+  const double *xyz;          /* [3*natoms*nmols] CONFIG coordinates (molxxx/y/z) or NULL */
+} fmc_species;
+
+/* The parsed FIELD / CONFIG / CONTROL content that the trial path needs. */
+typedef struct fmc_system {
+  int imcon;                  /* 1 cubic, 2 orthorhombic, 3 parallelepiped */
+  double cell[9];             /* cell vectors a, b, c (CONFIG rows) */
+  int nspecies;               /* maxmls */
+  const fmc_species *species; /* [nspecies] */
+  int ntypes;                 /* ntpatm */
+  int npairs;                 /* ntpvdw: VDW table rows in FIELD order */
+  const int *pair_i;          /* [npairs] 1-based atom types */
+  const int *pair_j;
+  const int *pair_kind;       /* [npairs] ltpvdw: 2 = lj (only kind on this path) */
+  const double *pair_eps;     /* [npairs] eps already in internal units (x engunit) */
+  const double *pair_sig;     /* [npairs] sigma in A */
+  double rcut;                /* cutoff (A) */
+  double rvdw;                /* vdw cutoff for the table grid (A), defaults to rcut */
+  double delr;                /* neighbour skin (A), guestlistgen */
+  double alpha;               /* Ewald convergence parameter (1/A) */
+  int kmax[3];                /* Ewald kmax1..3 */
+  double engunit;             /* energy unit in internal units (418.4 for kcal) */
+  double epsq;                /* relative dielectric constant (1.0) */
+  double overlap;             /* overlap distance (A, not squared) */
+  double surftol;             /* surface tolerance (statistics only) */
+  double sumchg;              /* total system charge after readfield */
+} fmc_system;
+
+/* Per-walker (state point) settings: temperature, fugacities, RNG seeds, moves. */
+typedef struct fmc_walker {
+  double temperature;                 /* K */
+  double fugacity[FMC_MAX_SPECIES];   /* Pa, per guest type (gstfuga) */
+  float move_cum[FMC_MAX_SPECIES][FMC_NMOVES]; /* REAL(4) cumulative move fractions, gcmc.f:359-366 */
+  double delr[FMC_MAX_SPECIES];       /* initial displacement step per guest (delr) */
+  int seed[4];                        /* duni seeds i, j, k, l (utility_pack.f:2801-2804) */
+  const double *framework_xyz;        /* NULL: the system's framework; else [3*nframework] copy */
+} fmc_walker;
+
+/* One trial, as the reference trial routines report it. */
+typedef struct fmc_trial {
+  double estep;                     /* energy change in FIELD units (mc_moves.f:251-252 etc.) */
+  double delE[FMC_MAX_SPECIES];     /* per-species energy change (delE) */
+  double recip, real, vdw, tail;    /* components in FIELD units, for diagnostics */
+  int overlap;                      /* loverlap */
+  int pad;
+} fmc_trial;
+
+/* One GCMC step of a device-driven chain (the fields a trace replay compares). */
+typedef struct fmc_trace_rec {
+  int step, iguest, move, choice, nmols, overlap, accepted, drew_rande;
+  double estep, rande;
+  double xyz[3 * FMC_MAX_SITES];
+  double energy[FMC_MAX_SPECIES];
+  double delE[FMC_MAX_SPECIES];
+  double delrdisp, rotangle, rota_rotangle, pad;
+} fmc_trace_rec;
+
+/* Per-walker running totals read back after a run. */
+typedef struct fmc_walker_stats {
+  int64_t steps, accepted[6], attempted[6];  /* index by move kind (1..5) */
+  int nmols[FMC_MAX_SPECIES];
+  int pad;
+  double energy[FMC_MAX_SPECIES];            /* cumulative per-species energy (FP64) */
+  double sum_n[FMC_MAX_SPECIES];             /* sum of N over counted steps */
+  double sum_n2[FMC_MAX_SPECIES];
+  double sum_e, sum_e2;                      /* total guest energy moments */
+  int64_t counted;                           /* steps that entered the sums */
+} fmc_walker_stats;
+
+typedef struct fmc_handle fmc_handle;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Build device state for nwalkers independent chains sharing one chemistry.
+ * Replaces the start-up of gcmc.f:229-548 (initscan .. erfcgen, single_point):
+ * k-vector enumeration (ewald.f:151-222), erfc table (erfcgen, ewald.f:1-30),
+ * LJ table (forgen, vdw_terms.f:181-234), tail terms (lrcorrect,
+ * vdw_terms.f:468-684), full S(k) (ewald1, ewald.f:354-661) and intramolecular
+ * exclusions (ewald3, ewald.f:735-790; gcmc.f:2662-2761). */
+int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers,
+               int precision, fmc_handle **out);
+int fmc_destroy(fmc_handle *h);
+
+/* Sizes the handle derived: newld (number of k-vectors), mxegrd, totatm. */
+int fmc_info(const fmc_handle *h, int *newld, int *mxegrd, int *nframework, int *maxmls);
+
+/* Full-system energy of walker w (single_point, gcmc.f:2549-2661):
+ * out[0] electrostatic (ecoul), out[1] reciprocal (ewald1), out[2] real-space
+ * pairs, out[3] vdw incl. tail, out[4] ewald3 exclusions; FIELD units. */
+int fmc_single_point(fmc_handle *h, int w, double out[5]);
+
+/* Re-sum S(k) of walker w in FP64 from its coordinates (ewald1, ewald.f:354-661)
+ * and return the largest |S_incremental - S_resum| found before overwriting. */
+int fmc_full_resum(fmc_handle *h, int w, double *max_drift);
+
+/* ---- one trial at a time (the mc_moves.f seam) ------------------------- */
+
+/* insertion (mc_moves.f:150-276): xyz = [3*natoms] trial coordinates (newx/y/z). */
+int fmc_trial_insert(fmc_handle *h, int w, int iguest, const double *xyz, fmc_trial *out);
+/* deletion (mc_moves.f:375-484): molecule `choice` (1-based) of guest type iguest. */
+int fmc_trial_delete(fmc_handle *h, int w, int iguest, int choice, fmc_trial *out);
+/* displace_guest (gcmc.f:2816-2901) with guest_energy (mc_moves.f:277-374)
+ * before and after: xyz_new = [3*natoms] displaced coordinates. */
+int fmc_trial_displace(fmc_handle *h, int w, int iguest, int choice, const double *xyz_new,
+                       fmc_trial *out);
+/* accept_move / reject_move (mc_moves.f:485-673) for the last trial of walker w. */
+int fmc_accept(fmc_handle *h, int w);
+int fmc_reject(fmc_handle *h, int w);
+
+/* Read guest coordinates of walker w: [3*natoms*nmols] of guest type iguest. */
+int fmc_get_guests(fmc_handle *h, int w, int iguest, double *xyz, int cap_mols, int *nmols);
+/* Read the FP64 structure factor S(k) (total row) of walker w: [2*newld] (re, im). */
+int fmc_get_sk(fmc_handle *h, int w, double *sk);
+/* Per-walker statistics (counts, N and E moments). */
+int fmc_get_stats(fmc_handle *h, int w, fmc_walker_stats *out);
+int fmc_reset_stats(fmc_handle *h);
+
+/* ---- device-driven chains (gcmc.f:704-1045 main loop) ------------------ */
+
+/* Advance every walker by nsteps GCMC steps on the device: move choice,
+ * trial energy, Metropolis test (energy_eval, utility_pack.f:3637-3693) and
+ * accept/reject, with the duni generator (utility_pack.f:2768-2857) per
+ * walker.  If trace is non-NULL, walker 0 writes one record per step
+ * (trace[nsteps]).  count_stats != 0 adds each step to the N/E moments. */
+int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats);
+
+/* Same, captured once into a HIP graph of `sweeps_per_graph` launches and
+ * replayed; nsweeps launches of nsteps_per_sweep steps in total. */
+int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_stats);
+
+/* Time the last fmc_run / fmc_run_graph dispatches with HIP events on the
+ * handle's stream: total milliseconds and launch count. */
+int fmc_last_timing(fmc_handle *h, double *ms, int *launches);
+
+/* Launch-shape control: threads per walker block (64..1024); 0 = autotune. */
+int fmc_set_block(fmc_handle *h, int threads);
+int fmc_get_block(const fmc_handle *h, int *threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTMC_H */

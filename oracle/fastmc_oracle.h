@@ -1,0 +1,38 @@
+/*
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain-C FP64 restatement of the
+ * reference FastMC trial-move energy path, used by tests/, smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  The product
+ * (libfastmc_hip.so) never links, loads or calls it.
+ */
+#ifndef FASTMC_ORACLE_H
+#define FASTMC_ORACLE_H
+
+#include "../include/fastmc.h"
+
+typedef struct orc_state orc_state;
+
+orc_state *orc_create(const fmc_system *sys, const fmc_walker *walker);
+void orc_destroy(orc_state *s);
+
+int orc_info(const orc_state *s, int *newld, int *mxegrd, int *totatm, int *maxmls);
+int orc_single_point(orc_state *s, double out[5]);
+
+int orc_trial_insert(orc_state *s, int iguest, const double *xyz, fmc_trial *out);
+int orc_trial_delete(orc_state *s, int iguest, int choice, fmc_trial *out);
+int orc_trial_displace(orc_state *s, int iguest, int choice, const double *xyz_new, fmc_trial *out);
+int orc_accept(orc_state *s);
+int orc_reject(orc_state *s);
+
+int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
+
+int orc_get_guests(orc_state *s, int iguest, double *xyz, int cap_mols, int *nmols);
+int orc_get_sk(orc_state *s, double *sk);
+int orc_get_table(orc_state *s, int which, int k, double *out, int n);
+int orc_get_kvec(orc_state *s, int *lmn, double *akk);
+int orc_get_energy(orc_state *s, double *energy);
+int orc_full_resum(orc_state *s, double *max_drift);
+
+/* duni (utility_pack.f:2768-2857) draws after seeding with (i, j, k, l). */
+void orc_duni_seq(const int seed[4], int n, double *out);
+
+#endif

@@ -1,0 +1,153 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the C oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  The product package never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, TRACE_DTYPE, make_walker)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for name in ("orc_info", "orc_single_point", "orc_trial_insert", "orc_trial_delete",
+                     "orc_trial_displace", "orc_accept", "orc_reject", "orc_run", "orc_get_guests",
+                     "orc_get_sk", "orc_get_table", "orc_get_kvec", "orc_get_energy",
+                     "orc_full_resum"):
+            getattr(L, name).restype = C.c_int
+        L.orc_duni_seq.restype = None
+        _lib = L
+    return _lib
+
+
+def duni_seq(seed, n):
+    out = np.zeros(n)
+    s = (C.c_int * 4)(*seed)
+    lib().orc_duni_seq(s, C.c_int(n), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+class Oracle:
+    """One reference-faithful FP64 chain on the CPU."""
+
+    def __init__(self, sysm, seed=(25, 89, 100, 150), temperature=None, pressures=None,
+                 framework_xyz=None):
+        self.sysm = sysm
+        self._keep = []
+        self.csys = CSystem(sysm)
+        self.walker = make_walker(sysm, seed, temperature, pressures, framework_xyz, self._keep)
+        self.h = lib().orc_create(C.byref(self.csys.struct), C.byref(self.walker))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        newld, mx, tot, mm = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib().orc_info(C.c_void_p(self.h), C.byref(newld), C.byref(mx), C.byref(tot), C.byref(mm))
+        self.newld, self.mxegrd, self.totatm, self.maxmls = newld.value, mx.value, tot.value, mm.value
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def single_point(self):
+        out = np.zeros(5)
+        lib().orc_single_point(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def run(self, nsteps):
+        tr = np.zeros(nsteps, dtype=TRACE_DTYPE)
+        rc = lib().orc_run(C.c_void_p(self.h), C.c_int(nsteps), tr.ctypes.data_as(C.c_void_p))
+        if rc:
+            raise RuntimeError(f"orc_run rc={rc}")
+        return tr
+
+    def _trial(self, fn, *args):
+        t = FmcTrial()
+        rc = fn(C.c_void_p(self.h), *args, C.byref(t))
+        if rc:
+            raise RuntimeError(f"oracle trial rc={rc}")
+        return t
+
+    def trial_insert(self, iguest, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        return self._trial(lib().orc_trial_insert, C.c_int(iguest), x.ctypes.data_as(C.c_void_p))
+
+    def trial_delete(self, iguest, choice):
+        return self._trial(lib().orc_trial_delete, C.c_int(iguest), C.c_int(choice))
+
+    def trial_displace(self, iguest, choice, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        return self._trial(lib().orc_trial_displace, C.c_int(iguest), C.c_int(choice),
+                           x.ctypes.data_as(C.c_void_p))
+
+    def accept(self):
+        return lib().orc_accept(C.c_void_p(self.h))
+
+    def reject(self):
+        return lib().orc_reject(C.c_void_p(self.h))
+
+    def guests(self, iguest=1, cap=100000):
+        nat = self.sysm.guests[iguest - 1].natoms
+        buf = np.zeros(3 * nat * 4096)
+        n = C.c_int()
+        lib().orc_get_guests(C.c_void_p(self.h), C.c_int(iguest), buf.ctypes.data_as(C.c_void_p),
+                             C.c_int(4096), C.byref(n))
+        return buf[: 3 * nat * n.value].reshape(-1, 3)
+
+    def sk(self):
+        out = np.zeros(2 * self.newld)
+        lib().orc_get_sk(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
+        return out.reshape(-1, 2)
+
+    def energy(self):
+        out = np.zeros(self.maxmls)
+        lib().orc_get_energy(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def table(self, which, k=0, n=None):
+        n = n or self.mxegrd
+        out = np.zeros(n)
+        rc = lib().orc_get_table(C.c_void_p(self.h), C.c_int(which), C.c_int(k),
+                                 out.ctypes.data_as(C.c_void_p), C.c_int(n))
+        if rc:
+            raise RuntimeError("bad table")
+        return out
+
+    def kvecs(self):
+        lmn = np.zeros(3 * self.newld, dtype=np.int32)
+        akk = np.zeros(self.newld)
+        lib().orc_get_kvec(C.c_void_p(self.h), lmn.ctypes.data_as(C.c_void_p),
+                           akk.ctypes.data_as(C.c_void_p))
+        return lmn.reshape(-1, 3), akk
+
+    def full_resum(self):
+        d = C.c_double()
+        lib().orc_full_resum(C.c_void_p(self.h), C.byref(d))
+        return d.value
