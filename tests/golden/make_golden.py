@@ -1,0 +1,234 @@
+"""Regenerate the golden fixtures from the compiled reference (run in the
+build container only; /root/reference is read-only and never travels).
+
+    make -C oracle ref                 # builds oracle/_ref/ref_driver.x from the reference sources
+    python tests/golden/make_golden.py
+
+For each case the inputs are written to a scratch directory, the reference
+trace recorder (oracle/ref_driver.f90 over the reference modules) runs there,
+and its TRACE / OUTPUT are reduced to a gzipped JSON fixture:
+
+* the duni seeds the reference printed (utility_pack.f:2814-2815);
+* sizes and derived parameters (newld, mxegrd, alpha, kmax, volume ...);
+* sampled erfc / LJ table entries and sampled S(k) after single_point;
+* the single-point energies (gcmc.f:2549-2661);
+* per step: guest type, move, molecule choice, N before the move, overlap,
+  accept, estep, rande, the trial coordinates (every step), per-species
+  energy() and delE(), the adaptive step sizes;
+* every 100 steps a checksum of the total S(k) and the guest counts.
+
+The CALF-20 case uses the reference's own test inputs (test/FIELD,
+test/CONFIG), which are stored as fixture data next to this script.
+"""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from fastmc_amd import inputs as I  # noqa: E402
+
+DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver.x")
+SCRATCH = os.path.join(REPO, "_work", "golden_runs")
+
+CALF_CONTROL = """GCMC Run
+temperature  293.000000
+steps    100000
+equilibration    0
+cutoff          12.500000 angstrom
+delr            1.0 angstrom
+ewald precision  1d-6
+numguests 1000
+history 0
+&guest 1
+  pressure (bar) 1.0
+  move insert 0.2
+  move delete 0.2
+  move displace 0.2
+  move jump 0.0
+  move swap 0.0
+  move translate 0.2
+  move rotate 0.2
+  move switch 0.0
+&end
+&guest 2
+  pressure (bar) 4.0
+  move insert 0.2
+  move delete 0.2
+  move displace 0.2
+  move jump 0.0
+  move swap 0.0
+  move translate 0.2
+  move rotate 0.2
+  move switch 0.0
+&end
+finish
+"""
+
+# (name, writer, nsteps)
+CASES = [
+    ("cfg0_p0.1", lambda d: I.config0(d, pressure_bar=0.1), 3000),
+    ("cfg0_p10", lambda d: I.config0(d, pressure_bar=10.0), 3000),
+    ("cfg0_moves5", lambda d: I.config0(d, pressure_bar=3.0, moves={
+        "insert": 0.2, "delete": 0.2, "displace": 0.2, "jump": 0.0, "swap": 0.0,
+        "translate": 0.2, "rotate": 0.2, "switch": 0.0}), 3000),
+    ("calf20_co2n2", None, 1500),
+]
+
+
+def write_calf20(d: str) -> None:
+    os.makedirs(d, exist_ok=True)
+    for f in ("FIELD", "CONFIG"):
+        shutil.copy(os.path.join(HERE, "calf20", f), os.path.join(d, f))
+    with open(os.path.join(d, "CONTROL"), "w") as fh:
+        fh.write(CALF_CONTROL)
+
+
+def ensure_calf20_inputs() -> None:
+    dst = os.path.join(HERE, "calf20")
+    if os.path.exists(os.path.join(dst, "CONFIG")):
+        return
+    os.makedirs(dst, exist_ok=True)
+    for f in ("FIELD", "CONFIG"):
+        shutil.copy(os.path.join("/root/reference/test", f), os.path.join(dst, f))
+
+
+def fhash(d: str) -> dict:
+    out = {}
+    for f in ("FIELD", "CONFIG", "CONTROL"):
+        with open(os.path.join(d, f), "rb") as fh:
+            out[f] = hashlib.sha256(fh.read()).hexdigest()
+    return out
+
+
+def num(x: str) -> float:
+    return float(x.replace("D", "E"))
+
+
+def parse(d: str) -> dict:
+    with open(os.path.join(d, "TRACE"), errors="replace") as fh:
+        lines = fh.read().split("\n")
+    with open(os.path.join(d, "OUTPUT"), errors="replace") as fh:
+        out = fh.read()
+    m = re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out)
+    seeds = [int(g) for g in m.groups()]
+    sp = {}
+    for key in ("Electrostatic energy", "recip energy", "vdw energy"):
+        mm = re.search(r"DRIVER " + key + r"\s+(\S+)", out)
+        sp[key] = num(mm.group(1))
+    fx = {"seeds": seeds, "single_point": sp, "erc": [], "vvv": [], "pairs": [], "lstvdw": [],
+          "sk0": [], "species": [], "guest_sites": [], "framework_sample": [], "trace": {},
+          "sk_checks": [], "nmols": []}
+    tr = {k: [] for k in ("step", "iguest", "move", "choice", "nmols", "natms", "overlap",
+                          "accepted", "estep", "rande", "drew_rande", "xyz", "energy", "delE",
+                          "delrdisp", "rotangle", "rota_rotangle")}
+    maxmls = None
+    for ln in lines:
+        f = ln.split()
+        if not f:
+            continue
+        t = f[0]
+        if t == "I":
+            v = list(map(int, f[1:]))
+            fx["sizes"] = dict(zip(["imcon", "kmax1", "kmax2", "kmax3", "newld", "mxegrd", "maxmls",
+                                    "ntpguest", "ntpatm", "maxvdw"], v))
+            maxmls = v[6]
+        elif t == "J":
+            v = list(map(int, f[1:]))
+            fx["sizes"].update(dict(zip(["totatm", "ntpvdw", "ntpfram"], v)))
+        elif t == "R":
+            v = list(map(num, f[1:]))
+            fx["params"] = dict(zip(["volm", "alpha", "rcut", "delr", "drewd", "dlrpot", "engunit",
+                                     "sumchg", "temp", "overlap2", "beta", "statvolm"], v))
+        elif t == "C":
+            fx["cell"] = list(map(num, f[1:]))
+        elif t == "M":
+            fx["move_cum"] = list(map(num, f[1:]))
+        elif t == "G":
+            fx["fugacity"] = list(map(num, f[1:]))
+        elif t == "T":
+            fx["erc"].append([int(f[1]), num(f[2])])
+        elif t == "P":
+            fx["pairs"].append([int(f[1]), int(f[2]), num(f[3]), num(f[4]), num(f[5])])
+        elif t == "V":
+            fx["vvv"].append([int(f[1]), int(f[2]), num(f[3])])
+        elif t == "L":
+            fx["lstvdw"].append([int(f[1]), int(f[2])])
+        elif t == "W":
+            fx["species"].append([int(f[1])] + [num(x) for x in f[2:]])
+        elif t == "Z":
+            fx["elrc"] = [num(x) for x in f[1:]]
+        elif t == "K0":
+            fx["sk0"].append([int(f[1])] + [num(x) for x in f[2:]])
+        elif t == "UG":
+            fx["guest_sites"].append([int(f[1]), int(f[2])] + [num(x) for x in f[3:]])
+        elif t == "F":
+            fx["framework_sample"].append([int(f[1]), int(f[2])] + [num(x) for x in f[3:]])
+        elif t == "S":
+            v = f[1:]
+            tr["step"].append(int(v[0]))
+            tr["iguest"].append(int(v[1]))
+            tr["move"].append(int(v[2]))
+            tr["choice"].append(int(v[3]))
+            tr["nmols"].append(int(v[4]))
+            tr["natms"].append(int(v[5]))
+            tr["overlap"].append(int(v[6]))
+            tr["accepted"].append(int(v[7]))
+            tr["estep"].append(num(v[8]))
+            tr["rande"].append(num(v[9]))
+            tr["drew_rande"].append(int(v[10]))
+        elif t == "X":
+            tr["xyz"].append([num(x) for x in f[1:]])
+        elif t == "E":
+            vals = [num(x) for x in f[1:]]
+            tr["energy"].append(vals[:maxmls])
+            tr["delE"].append(vals[maxmls:2 * maxmls])
+        elif t == "A":
+            vals = [num(x) for x in f[1:]]
+            tr["delrdisp"].append(vals[0])
+            tr["rotangle"].append(vals[1])
+            tr["rota_rotangle"].append(vals[2])
+        elif t == "K":
+            fx["sk_checks"].append([int(f[1])] + [num(x) for x in f[2:]])
+        elif t == "N":
+            fx["nmols"].append([int(x) for x in f[1:]])
+    fx["trace"] = tr
+    return fx
+
+
+def main() -> None:
+    if not os.path.exists(DRIVER):
+        subprocess.check_call(["make", "-C", os.path.join(REPO, "oracle"), "ref"])
+    ensure_calf20_inputs()
+    os.makedirs(SCRATCH, exist_ok=True)
+    for name, writer, nsteps in CASES:
+        d = os.path.join(SCRATCH, name)
+        shutil.rmtree(d, ignore_errors=True)
+        if writer is None:
+            write_calf20(d)
+        else:
+            writer(d)
+        subprocess.check_call([DRIVER, str(nsteps), "100"], cwd=d, stdout=subprocess.DEVNULL)
+        fx = parse(d)
+        fx["case"] = name
+        fx["input_sha256"] = fhash(d)
+        if writer is None:
+            fx["control"] = CALF_CONTROL
+        path = os.path.join(HERE, f"{name}.json.gz")
+        with gzip.open(path, "wt") as fh:
+            json.dump(fx, fh, separators=(",", ":"))
+        print(name, "steps", len(fx["trace"]["step"]), "bytes", os.path.getsize(path),
+              "accepted", sum(fx["trace"]["accepted"]), "final N", fx["nmols"][-1] if fx["nmols"] else None)
+
+
+if __name__ == "__main__":
+    main()
