@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""GCMC trial moves/s on one MI355X (batched mixed FP32/FP64 vs plain FP64).
+
+Workload (batched isotherm): 64 independent state points in one persistent
+launch -- T in {273, 293, 323} K (22/21/21), P log-spaced 1e-3..10 bar --
+each on its own seed-jittered copy of the 424-atom cubic framework, CO2 guest,
+rcut 12.5 A, Ewald precision 1e-6 (871 k-vectors), mixed precision,
+hipGraph-captured sweeps.  One bench step = one sweep = `--moves` trial moves
+per state point.  Synthetic inputs, deterministic generator.
+
+Printed as one JSON line (rank 0, N=1):
+  value          -- attempted trial moves per second, whole job (all walkers)
+  roofline       -- dominant kernel (k_run) against HBM: algorithmic bytes
+                    B = 17 (N_fw + N_guest_sites) + 32 n_k per trial
+  cpu_baseline   -- the reference's serial build on one state point of the
+                    same workload, this box's host, one core
+  fp64_single    -- plain FP64, one state point, same device kernel
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+METRIC = "GCMC trial moves/s, 1 MI355X: batched FP32/FP64 vs plain FP64, ΔU≤1e-6 rel"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed sweeps")
+    ap.add_argument("--warmup", type=int, default=30, help="untimed sweeps (loading the isotherm)")
+    ap.add_argument("--moves", type=int, default=1000, help="trial moves per state point per sweep")
+    ap.add_argument("--walkers", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def device_sync():
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def make_batch(nwalk: int, root: str):
+    from fastmc_amd import inputs as I
+    from fastmc_amd.system import read_system
+    dirs = I.config2(root, n=nwalk)
+    systems = [read_system(d) for d in dirs]
+    return dirs, systems
+
+
+def algorithmic_bytes(nfw: int, n_guest_sites: float, nk: int) -> float:
+    return 17.0 * (nfw + n_guest_sites) + 32.0 * nk
+
+
+def reference_cpu(state_dir: str, seconds: float):
+    """Time the reference serial build (oracle/_ref/gcmc_ser.x) on one state point."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "gcmc_ser.x")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="fmc_cpu_")
+    try:
+        for f in ("FIELD", "CONFIG", "CONTROL"):
+            shutil.copy(os.path.join(state_dir, f), os.path.join(work, f))
+        steps = 4000
+        res = None
+        for _ in range(4):  # grow the sample until it takes about `seconds`
+            ctl = open(os.path.join(work, "CONTROL")).read()
+            ctl = re.sub(r"(?m)^steps .*$", f"steps    {steps}", ctl)
+            ctl = re.sub(r"(?m)^equilibration .*$", "equilibration    0", ctl)
+            open(os.path.join(work, "CONTROL"), "w").write(ctl)
+            for p in os.listdir(work):
+                if p.startswith("branch"):
+                    shutil.rmtree(os.path.join(work, p), ignore_errors=True)
+            t0 = time.time()
+            r = subprocess.run([exe], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=max(60, 6 * seconds))
+            wall = time.time() - t0
+            out = open(os.path.join(work, "OUTPUT")).read()
+            m = re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps\s*:\s*([0-9.Ee+-]+)", out)
+            if r.returncode != 0 or not m:
+                return None
+            n, t = int(m.group(1)), float(m.group(2))
+            res = (n, t, wall)
+            if t >= 0.6 * seconds:
+                break
+            steps = int(steps * min(8.0, max(1.5, seconds / max(t, 1e-3))))
+        n, t, wall = res
+        return {"value": n / t, "unit": "trial moves/s", "cores": 1, "kind": "reference",
+                "sample": f"reference serial build (amdflang -O3) on one state point of the batch "
+                          f"(T=293 K, P=0.1 bar), {n} GCMC steps from an empty "
+                          f"framework, {t:.2f} s loop time"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def oracle_cpu(sysm, seconds: float):
+    from oracle.oracle import Oracle
+    o = Oracle(sysm, seed=(31, 41, 59, 26))
+    n, t = 0, 0.0
+    step = 500
+    while t < seconds and n < 2_000_000:
+        t0 = time.time()
+        o.run(step)
+        t += time.time() - t0
+        n += step
+    return {"value": n / t, "unit": "trial moves/s", "cores": 1, "kind": "port",
+            "sample": f"C oracle, one state point, {n} steps"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
+              "(state points shard across GPUs with no per-trial communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    from fastmc_amd.system import make_walker
+
+    tmp = tempfile.mkdtemp(prefix="fmc_bench_")
+    try:
+        dirs, systems = make_batch(args.walkers, tmp)
+        base = systems[0]
+        keep = []
+        walkers = []
+        for i, s in enumerate(systems):
+            walkers.append(make_walker(base, seed=(1 + (7 * i) % 170, 1 + (13 * i) % 170, 1 + (29 * i) % 170,
+                                                   (37 * i) % 168),
+                                       temperature=s.temperature, pressures=s.pressures,
+                                       framework_xyz=s.framework_xyz(), keep=keep))
+        h = Handle(base, walkers=walkers, precision=FMC_MIXED)
+        h.set_block(0)  # autotune threads per walker on a snapshot
+        block = h.get_block()
+        # warm-up: fills each state point to its loading, then captures the graph
+        for _ in range(max(1, args.warmup // 10)):
+            h.run_graph(10, args.moves)
+        h.run_graph(args.steps, args.moves)   # capture + one untimed replay of the timed graph
+        h.reset_stats()
+        device_sync()
+        t0 = time.perf_counter()
+        h.run_graph(args.steps, args.moves, count_stats=True)
+        device_sync()
+        wall = time.perf_counter() - t0
+        gpu_ms, launches = h.last_timing()
+        moves = args.walkers * args.moves * args.steps
+        value = moves / wall
+        # guest sites present on average during the timed sweeps
+        nsites = base.guests[0].natoms
+        tot_n, nmax, nmin = 0.0, 0, 10 ** 9
+        for w in range(args.walkers):
+            st = h.stats(w)
+            tot_n += st.sum_n[0] / max(st.counted, 1)
+            nmax = max(nmax, st.nmols[0])
+            nmin = min(nmin, st.nmols[0])
+        n_avg = tot_n / args.walkers
+        B = algorithmic_bytes(h.nframework, n_avg * nsites, h.newld)
+        launch_s = gpu_ms / 1e3 / max(launches, 1)
+        achieved = B * args.walkers * args.moves / launch_s
+        traffic = None
+        pmc = os.path.join(ROOT, "profiles", "k_run_batched_isotherm_counters.json")
+        if os.path.exists(pmc):
+            try:
+                traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+            except Exception:
+                traffic = None
+        import math
+        mid_i = min(range(len(systems)), key=lambda i: (abs(systems[i].temperature - 293.0),
+                                                        abs(math.log10(systems[i].pressures[0] / 1e5) + 1.0)))
+        fp64 = None
+        if not args.no_fp64:
+            mid = systems[mid_i]  # T = 293 K, P = 0.1 bar: the config-0 state point
+            hs = Handle(base, walkers=[make_walker(base, seed=(11, 23, 37, 41), temperature=mid.temperature,
+                                                   pressures=mid.pressures, framework_xyz=mid.framework_xyz(),
+                                                   keep=keep)], precision=FMC_FP64)
+            hs.set_block(0)
+            hs.run_graph(5, args.moves)
+            hs.run_graph(5, args.moves)
+            t1 = time.perf_counter()
+            hs.run_graph(5, args.moves)
+            device_sync()
+            w1 = time.perf_counter() - t1
+            fp64 = {"value": 5 * args.moves / w1, "unit": "trial moves/s",
+                    "config": f"plain FP64, one state point (T={mid.temperature:g} K, "
+                              f"P={mid.pressures[0] / 1e5:.4g} bar), same device kernel, 1 walker",
+                    "threads_per_walker": hs.get_block()}
+            hs.close()
+        cpu = None
+        if not args.no_cpu:
+            cpu = reference_cpu(dirs[mid_i], args.cpu_seconds)
+            if cpu is None:
+                cpu = oracle_cpu(systems[mid_i], args.cpu_seconds)
+        h.close()
+        line = {
+            "metric": METRIC, "value": value, "unit": "trial moves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32+fp64", "data": "synthetic",
+            "config": {"workload": "batched isotherm: 64 state points (T 273/293/323 K x P 1e-3..10 bar), "
+                                   "424-atom jittered cubic frameworks, CO2, rcut 12.5 A, 871 k-vectors, "
+                                   "mixed precision, hipGraph sweeps",
+                       "walkers": args.walkers, "moves_per_walker_per_step": args.moves,
+                       "threads_per_walker": block, "mean_guests": round(n_avg, 2),
+                       "guests_min_max": [nmin, nmax]},
+            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK, "traffic": traffic,
+                         "kernel": "k_run<2,float>", "bytes_per_trial": B,
+                         "avg_launch_ms": launch_s * 1e3},
+            "cpu_baseline": cpu,
+            "fp64_single": fp64,
+            "batched_vs_fp64_single": (value / fp64["value"]) if fp64 else None,
+        }
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

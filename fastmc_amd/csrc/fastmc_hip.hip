@@ -1,0 +1,1051 @@
+// libfastmc_hip.so: C-ABI of the MI355X GCMC trial-move path (include/fastmc.h).
+//
+// Host side: builds the read-only tables once per handle (the erfc and LJ
+// grids, the k-vector list, the tail constants), allocates the per-walker
+// device state, and launches the kernels of fmc_kernels.hpp on one HIP stream
+// per handle.  No computation of the trial path happens on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "fmc_kernels.hpp"
+
+using namespace fmc;
+
+namespace {
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      fprintf(stderr, "fastmc_hip: %s failed: %s\n", #x, hipGetErrorString(e_));      \
+      return FMC_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+struct GraphKey {
+  int nsweeps, nsteps, count;
+  bool operator<(const GraphKey &o) const {
+    if (nsweeps != o.nsweeps) return nsweeps < o.nsweeps;
+    if (nsteps != o.nsteps) return nsteps < o.nsteps;
+    return count < o.count;
+  }
+};
+
+}  // namespace
+
+struct fmc_handle {
+  int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 256;
+  DevParams hp;                  // host copy
+  DevParams *dp = nullptr;       // device copy
+  DevWalkers dw{};
+  std::vector<void *> allocs;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  std::vector<WState> ws_init;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  // pending seam trial (host shadow)
+  int pend_kind[1] = {0};
+  std::vector<int> pend_state;   // per walker: 0 none, 1 valid trial, 2 overlapped trial
+  fmc_trace_rec *dtrace = nullptr;
+  int dtrace_cap = 0;
+  double *dscratch = nullptr;    // small device scratch for reductions (allocated at create)
+  int nspecies_total = 0;
+  // host mirrors for single_point
+  std::vector<double> fw_q;
+  double selfq2_all_fw = 0.0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(fmc_handle *h, T **p, size_t n) {
+  void *q = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return FMC_ERR_ALLOC;
+  h->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+// dcell, utility_pack.f:1904-1960 (0-based in and out)
+void dcell(const double *a, double *b) {
+  b[0] = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  b[1] = std::sqrt(a[3] * a[3] + a[4] * a[4] + a[5] * a[5]);
+  b[2] = std::sqrt(a[6] * a[6] + a[7] * a[7] + a[8] * a[8]);
+  double axb1 = a[1] * a[5] - a[2] * a[4], axb2 = a[2] * a[3] - a[0] * a[5], axb3 = a[0] * a[4] - a[1] * a[3];
+  double bxc1 = a[4] * a[8] - a[5] * a[7], bxc2 = a[5] * a[6] - a[3] * a[8], bxc3 = a[3] * a[7] - a[4] * a[6];
+  double cxa1 = a[7] * a[2] - a[1] * a[8], cxa2 = a[0] * a[8] - a[2] * a[6], cxa3 = a[1] * a[6] - a[0] * a[7];
+  b[9] = std::fabs(a[0] * bxc1 + a[1] * bxc2 + a[2] * bxc3);
+  b[6] = b[9] / std::sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+  b[7] = b[9] / std::sqrt(cxa1 * cxa1 + cxa2 * cxa2 + cxa3 * cxa3);
+  b[8] = b[9] / std::sqrt(axb1 * axb1 + axb2 * axb2 + axb3 * axb3);
+}
+
+// invert, utility_pack.f:1962-2004 (0-based)
+void invert(const double *a, double *b) {
+  b[0] = a[4] * a[8] - a[5] * a[7];
+  b[1] = a[2] * a[7] - a[1] * a[8];
+  b[2] = a[1] * a[5] - a[2] * a[4];
+  b[3] = a[5] * a[6] - a[3] * a[8];
+  b[4] = a[0] * a[8] - a[2] * a[6];
+  b[5] = a[2] * a[3] - a[0] * a[5];
+  b[6] = a[3] * a[7] - a[4] * a[6];
+  b[7] = a[1] * a[6] - a[0] * a[7];
+  b[8] = a[0] * a[4] - a[1] * a[3];
+  double d = a[0] * b[0] + a[3] * b[1] + a[6] * b[2];
+  double r = std::fabs(d) > 0.0 ? 1.0 / d : 0.0;
+  for (int i = 0; i < 9; ++i) b[i] *= r;
+}
+
+void host_min_image(const DevParams &P, double &dx, double &dy, double &dz) {
+  if (P.imcon == 1) {
+    dx -= P.cell[0] * std::round(P.inv_cell[0] * dx);
+    dy -= P.cell[0] * std::round(P.inv_cell[0] * dy);
+    dz -= P.cell[0] * std::round(P.inv_cell[0] * dz);
+  } else if (P.imcon == 2) {
+    dx -= P.cell[0] * std::round(P.inv_cell[0] * dx);
+    dy -= P.cell[4] * std::round(P.inv_cell[1] * dy);
+    dz -= P.cell[8] * std::round(P.inv_cell[2] * dz);
+  } else {
+    const double *r = P.rcell, *c = P.cell;
+    double sx = r[0] * dx + r[3] * dy + r[6] * dz, sy = r[1] * dx + r[4] * dy + r[7] * dz, sz = r[2] * dx + r[5] * dy + r[8] * dz;
+    sx -= std::round(sx);
+    sy -= std::round(sy);
+    sz -= std::round(sz);
+    dx = c[0] * sx + c[3] * sy + c[6] * sz;
+    dy = c[1] * sx + c[4] * sy + c[7] * sz;
+    dz = c[2] * sx + c[5] * sy + c[8] * sz;
+  }
+}
+
+// ewald3 (ewald.f:735-790) of one guest template: the intramolecular
+// exclusion energy subtracted once per guest (gcmc.f:2691-2716).
+double ewald3_template(const DevParams &P, int mol, int g, int natms) {
+  const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741;
+  const double a4 = -1.453152027, a5 = 1.061405429, pp = 0.3275911;
+  const double rr3 = 0.333333333333, r10 = 0.1, r42 = 0.02380952381, r216 = 4.62962962963e-3;
+  double tot = 0.0, alpha = P.alpha;
+  for (int i = 0; i < natms - 1; ++i) {
+    double chgea = P.site_q[mol][i] / P.epsq * R4PIE0, e = 0.0;
+    for (int j = i + 1; j < natms; ++j) {
+      double dx = P.tmpl[g][i][0] - P.tmpl[g][j][0], dy = P.tmpl[g][i][1] - P.tmpl[g][j][1],
+             dz = P.tmpl[g][i][2] - P.tmpl[g][j][2];
+      host_min_image(P, dx, dy, dz);
+      double chgprd = chgea * P.site_q[mol][j];
+      double rrr = std::sqrt(dx * dx + dy * dy + dz * dz), alpr = rrr * alpha, alpr2 = alpr * alpr, erfr;
+      if (alpr < 1e-2) {
+        erfr = 2.0 * chgprd * (alpha / SQRPI) * (1.0 + alpr2 * (-rr3 + alpr2 * (r10 + alpr2 * (-r42 + alpr2 * r216))));
+      } else {
+        double tt = 1.0 / (1.0 + pp * alpha * rrr);
+        double ex = std::exp(-(alpha * rrr) * (alpha * rrr));
+        erfr = (1.0 - tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ex) * chgprd / rrr;
+      }
+      e += erfr;
+    }
+    tot += e;
+  }
+  return tot;
+}
+
+template <int NS, typename T>
+void launch_run(fmc_handle *h, int nsteps, int count) {
+  hipLaunchKernelGGL((k_run<NS, T>), dim3(h->nwalkers), dim3(h->block), 0, h->stream, h->dp, h->dw, nsteps, count);
+}
+
+void run_dispatch(fmc_handle *h, int nsteps, int count) {
+  bool mixed = h->precision == FMC_MIXED;
+  switch (h->ns) {
+    case 2: mixed ? launch_run<2, float>(h, nsteps, count) : launch_run<2, double>(h, nsteps, count); break;
+    case 3: mixed ? launch_run<3, float>(h, nsteps, count) : launch_run<3, double>(h, nsteps, count); break;
+    default: mixed ? launch_run<4, float>(h, nsteps, count) : launch_run<4, double>(h, nsteps, count); break;
+  }
+}
+
+template <int NS, typename T>
+void launch_trial(fmc_handle *h, int w) {
+  hipLaunchKernelGGL((k_trial<NS, T>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w);
+}
+
+void trial_dispatch(fmc_handle *h, int w) {
+  bool mixed = h->precision == FMC_MIXED;
+  switch (h->ns) {
+    case 2: mixed ? launch_trial<2, float>(h, w) : launch_trial<2, double>(h, w); break;
+    case 3: mixed ? launch_trial<3, float>(h, w) : launch_trial<3, double>(h, w); break;
+    default: mixed ? launch_trial<4, float>(h, w) : launch_trial<4, double>(h, w); break;
+  }
+}
+
+void accept_dispatch(fmc_handle *h, int w) {
+  switch (h->ns) {
+    case 2: hipLaunchKernelGGL((k_accept<2>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
+    case 3: hipLaunchKernelGGL((k_accept<3>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
+    default: hipLaunchKernelGGL((k_accept<4>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
+  }
+}
+
+void resum_dispatch(fmc_handle *h, double *drift) {
+  dim3 grid((h->hp.nk + 255) / 256, h->nwalkers);
+  switch (h->ns) {
+    case 2: hipLaunchKernelGGL((k_resum<2>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
+    case 3: hipLaunchKernelGGL((k_resum<3>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
+    default: hipLaunchKernelGGL((k_resum<4>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
+  }
+}
+
+int walker_natoms(fmc_handle *h, int w, std::vector<WState> &ws) {
+  int n = h->hp.nfw;
+  for (int g = 1; g <= h->hp.ntpguest; ++g) {
+    int mol = h->hp.locguest[g];
+    n += ws[w].nmols[mol] * h->hp.natoms[mol];
+  }
+  return n;
+}
+
+// single_point for every walker: species-resolved recip, real and vdw sums.
+// out[w][0..4] as fmc_single_point; also refreshes ws[w].energy (gcmc.f:554-582).
+int single_points(fmc_handle *h, std::vector<WState> &ws, std::vector<double> &out, bool set_energy) {
+  const DevParams &P = h->hp;
+  const int NW = h->nwalkers, NC = KMAXC + 1;
+  std::vector<double> rec((size_t)NW * NC), pairs((size_t)NW * 2 * NC, 0.0);
+  double *d = h->dscratch;
+  HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * (size_t)NW * 3 * NC, h->stream));
+  switch (h->ns) {
+    case 2: hipLaunchKernelGGL((k_recip_sums<2>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
+    case 3: hipLaunchKernelGGL((k_recip_sums<3>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
+    default: hipLaunchKernelGGL((k_recip_sums<4>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
+  }
+  int nmax = 0;
+  for (int w = 0; w < NW; ++w) nmax = std::max(nmax, walker_natoms(h, w, ws));
+  // k_pair_sums walks each walker's own atom count; pass the largest and let
+  // the kernel skip past the walker's end through the atom lookup
+  for (int w = 0; w < NW; ++w) {
+    int n = walker_natoms(h, w, ws);
+    dim3 grid((n + 255) / 256, 1);
+    DevWalkers dw1 = h->dw;
+    // shift the per-walker views so blockIdx.y == 0 addresses walker w
+    dw1.ws = h->dw.ws + w;
+    dw1.gx = h->dw.gx + (size_t)w * P.gatoms_total;
+    dw1.gy = h->dw.gy + (size_t)w * P.gatoms_total;
+    dw1.gz = h->dw.gz + (size_t)w * P.gatoms_total;
+    if (P.fw_per_walker) {
+      dw1.fwx = h->dw.fwx + (size_t)w * P.nfw;
+      dw1.fwy = h->dw.fwy + (size_t)w * P.nfw;
+      dw1.fwz = h->dw.fwz + (size_t)w * P.nfw;
+    }
+    double *o = d + (size_t)NW * NC + (size_t)w * 2 * NC;
+    switch (h->ns) {
+      case 2: hipLaunchKernelGGL((k_pair_sums<2>), grid, dim3(256), 0, h->stream, h->dp, dw1, n, o); break;
+      case 3: hipLaunchKernelGGL((k_pair_sums<3>), grid, dim3(256), 0, h->stream, h->dp, dw1, n, o); break;
+      default: hipLaunchKernelGGL((k_pair_sums<4>), grid, dim3(256), 0, h->stream, h->dp, dw1, n, o); break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rec.data(), d, sizeof(double) * (size_t)NW * NC, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(pairs.data(), d + (size_t)NW * NC, sizeof(double) * (size_t)NW * 2 * NC,
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  out.assign((size_t)NW * 5, 0.0);
+  const int NS = h->ns;
+  const double C = 2.0 * P.rvolm * R4PIE0 / P.epsq;
+  for (int w = 0; w < NW; ++w) {
+    WState &s = ws[w];
+    // ewald1 finalisation, ewald.f:626-643
+    double q2 = h->selfq2_all_fw;
+    for (int g = 1; g <= P.ntpguest; ++g) q2 += s.nmols[P.locguest[g]] * P.selfq2[g];
+    double engsicold = -R4PIE0 / P.epsq * P.alpha * q2 / SQRPI;
+    double qfix = -(0.5 * PI * R4PIE0 / P.epsq) * ((s.sumchg / P.alpha) * (s.sumchg / P.alpha) / P.volm);
+    double ewald1sum = C * rec[(size_t)w * NC] + engsicold + qfix;
+    double e1[KMAXC + 1] = {0};
+    for (int i = 1; i <= NS; ++i) {
+      double fkk = 1.0;
+      for (int j = 1; j <= i; ++j) {
+        if (i == j) fkk = 0.5;
+        int ik = loc2(i, j);
+        double sic = -R4PIE0 / P.epsq * P.alpha * s.engsic[ik] / SQRPI;
+        sic = sic - ((fkk * PI * R4PIE0 / P.epsq) * s.chgsum_mol[i] * s.chgsum_mol[j] / (P.alpha * P.alpha * P.volm));
+        e1[ik] = C * rec[(size_t)w * NC + ik] + sic;
+      }
+    }
+    double e2sum = 0.0, evsum = 0.0, e3 = 0.0;
+    const double *pe2 = pairs.data() + (size_t)w * 2 * NC, *pev = pe2 + NC;
+    for (int t = 1; t <= KMAXC; ++t) {
+      e2sum += pe2[t];
+      evsum += pev[t];
+    }
+    for (int g = 1; g <= P.ntpguest; ++g) e3 += s.nmols[P.locguest[g]] * P.ewald3en[P.locguest[g]];
+    out[(size_t)w * 5 + 0] = (e2sum + ewald1sum - e3) / P.engunit;
+    out[(size_t)w * 5 + 1] = ewald1sum / P.engunit;
+    out[(size_t)w * 5 + 2] = e2sum / P.engunit;
+    out[(size_t)w * 5 + 3] = (evsum + s.elrc) / P.engunit;
+    out[(size_t)w * 5 + 4] = e3 / P.engunit;
+    if (set_energy) {
+      for (int g = 1; g <= P.ntpguest; ++g) {
+        int mol = P.locguest[g];
+        double sp = 0.0;
+        for (int j = 1; j <= NS; ++j) {
+          int ik = loc2(mol, j);
+          sp += (e1[ik] + pe2[ik] + s.elrc_mol[ik] + pev[ik]) / P.engunit;
+        }
+        if (s.nmols[mol] > 0) sp -= s.nmols[mol] * P.ewald3en[mol] / P.engunit;
+        s.energy[mol] = sp;
+      }
+    }
+  }
+  return 0;
+}
+
+// lrcorrect counts and tail (vdw_terms.f:468-684) for one walker's state.
+void lrcorrect_counts(const DevParams &P, const fmc_system *sys, const std::vector<int> &fwtype,
+                      const std::vector<int> &fwspec, const std::vector<int> &fwfrz, WState &s) {
+  memset(s.numtyp, 0, sizeof s.numtyp);
+  memset(s.numfrz, 0, sizeof s.numfrz);
+  memset(s.numtyp_mol, 0, sizeof s.numtyp_mol);
+  memset(s.numfrz_mol, 0, sizeof s.numfrz_mol);
+  for (size_t a = 0; a < fwtype.size(); ++a) {
+    int ka = fwtype[a], mol = fwspec[a];
+    s.numtyp[ka] += 1;
+    s.numtyp_mol[mol][ka] += 1;
+    if (fwfrz[a]) {
+      s.numfrz[ka] += 1;
+      s.numfrz_mol[mol][ka] += 1;
+    }
+  }
+  for (int g = 1; g <= P.ntpguest; ++g) {
+    int mol = P.locguest[g];
+    for (int m = 0; m < s.nmols[mol]; ++m)
+      for (int i = 0; i < P.natoms[mol]; ++i) {
+        int ka = P.site_t[mol][i];
+        s.numtyp[ka] += 1;
+        s.numtyp_mol[mol][ka] += 1;
+        if (P.site_f[mol][i]) {
+          s.numfrz[ka] += 1;
+          s.numfrz_mol[mol][ka] += 1;
+        }
+      }
+  }
+  const double twopi = 2.0 * PI, volm = P.volm;
+  s.elrc = 0.0;
+  for (int t = 0; t <= KMAXC; ++t) s.elrc_mol[t] = 0.0;
+  int ivdw = 0;
+  for (int i = 1; i <= P.ntypes; ++i)
+    for (int j = 1; j <= i; ++j) {
+      ++ivdw;
+      double eadd = P.steadd[ivdw];
+      double denprd = twopi * ((double)s.numtyp[i] * (double)s.numtyp[j] - (double)s.numfrz[i] * (double)s.numfrz[j]) / (volm * volm);
+      s.elrc += volm * denprd * eadd;
+      double e0[KMAXC + 1] = {0};
+      for (int it = 1; it <= P.maxmls; ++it)
+        for (int jt = 1; jt <= P.maxmls; ++jt) {
+          int kt = loc2(it, jt);
+          e0[kt] += twopi * ((double)s.numtyp_mol[it][i] * (double)s.numtyp_mol[jt][j] -
+                             (double)s.numfrz_mol[it][i] * (double)s.numfrz_mol[jt][j]) / (volm * volm);
+        }
+      for (int t = 1; t <= KMAXC; ++t) s.elrc_mol[t] += volm * eadd * e0[t];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, int precision, fmc_handle **out) {
+  if (!sys || !walkers || !out || nwalkers < 1) return FMC_ERR_ARG;
+  if (sys->imcon < 1 || sys->imcon > 3) return FMC_ERR_IMCON;
+  if (sys->nspecies < 2 || sys->nspecies > KMAXS || sys->ntypes > KTYPE || sys->ntypes < 1) return FMC_ERR_ARG;
+  if (precision != FMC_FP64 && precision != FMC_MIXED) return FMC_ERR_ARG;
+  fmc_handle *h = new fmc_handle();
+  h->nwalkers = nwalkers;
+  h->precision = precision;
+  h->ns = sys->nspecies;
+  DevParams &P = h->hp;
+  memset(&P, 0, sizeof P);
+  P.imcon = sys->imcon;
+  P.maxmls = sys->nspecies;
+  P.ntypes = sys->ntypes;
+  for (int i = 0; i < 9; ++i) P.cell[i] = sys->cell[i];
+  invert(P.cell, P.rcell);
+  P.inv_cell[0] = 1.0 / P.cell[0];
+  P.inv_cell[1] = 1.0 / P.cell[4];
+  P.inv_cell[2] = 1.0 / P.cell[8];
+  double celprp[10];
+  dcell(P.cell, celprp);
+  P.volm = celprp[9];
+  P.statvolm = P.volm * 1e-30;
+  double width = std::min(std::min(celprp[6], celprp[7]), celprp[8]) / 2.0;
+  if (sys->rcut > width) {
+    delete h;
+    return FMC_ERR_CUTOFF_WIDTH;
+  }
+  P.rcut2 = sys->rcut * sys->rcut;
+  P.rclim2 = (sys->rcut + sys->delr) * (sys->rcut + sys->delr);
+  P.overlap2 = sys->overlap * sys->overlap;
+  P.alpha = sys->alpha;
+  P.epsq = sys->epsq > 0 ? sys->epsq : 1.0;
+  P.engunit = sys->engunit;
+  P.rvolm = 2.0 * PI / P.volm;
+  P.kmax1 = sys->kmax[0];
+  P.kmax2 = sys->kmax[1];
+  P.kmax3 = sys->kmax[2];
+  if (P.kmax1 >= KCAP || P.kmax2 >= KCAP || P.kmax3 >= KCAP || P.kmax1 < 1 || P.kmax2 < 1 || P.kmax3 < 1) {
+    delete h;
+    return FMC_ERR_ARG;
+  }
+  // ---- tables: mxegrd (utility_pack.f:996), erfcgen (ewald.f:1-30), forgen (vdw_terms.f:181-234)
+  double rvdw = sys->rvdw > 0 ? sys->rvdw : sys->rcut;
+  int mxegrd = std::max(1000, (int)(rvdw / 0.01 + 0.5) + 4);
+  P.mxegrd = mxegrd;
+  P.drewd = sys->rcut / (double)(mxegrd - 4);
+  P.rdrewd = 1.0 / P.drewd;
+  P.dlrpot = rvdw / (double)(mxegrd - 4);
+  P.rdlrpot = 1.0 / P.dlrpot;
+  std::vector<double> erc(mxegrd + 3, 0.0);
+  {
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741;
+    const double a4 = -1.453152027, a5 = 1.061405429, pp = 0.3275911;
+    for (int i = 1; i <= mxegrd; ++i) {
+      double rrr = (double)i * P.drewd, tt = 1.0 / (1.0 + pp * P.alpha * rrr), ar = P.alpha * rrr;
+      double exp1 = std::exp(-(ar * ar));
+      erc[i] = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * exp1 / rrr;
+    }
+  }
+  const int npairs = sys->npairs;
+  const int ntri = (P.ntypes * (P.ntypes + 1)) / 2;
+  const int maxvdw = std::max(npairs, ntri);
+  std::vector<int> lstvdw(maxvdw + 2, 0), ltpvdw(maxvdw + 2, -1);
+  std::vector<double> peps(maxvdw + 2, 0.0), psig(maxvdw + 2, 0.0);
+  for (int p = 1; p <= npairs; ++p) {
+    int i = sys->pair_i[p - 1], j = sys->pair_j[p - 1];
+    if (i < 1 || j < 1 || i > P.ntypes || j > P.ntypes) {
+      delete h;
+      return FMC_ERR_ARG;
+    }
+    lstvdw[loc2(i, j)] = p;
+    ltpvdw[p] = sys->pair_kind[p - 1];
+    peps[p] = sys->pair_eps[p - 1];
+    psig[p] = sys->pair_sig[p - 1];
+  }
+  if (npairs < ntri)
+    for (int i = 1; i <= ntri; ++i)
+      if (lstvdw[i] == 0) lstvdw[i] = npairs + 1;
+  P.vstride = mxegrd + 3;
+  std::vector<double> vvv((size_t)(npairs + 1) * P.vstride, 0.0);
+  for (int p = 1; p <= npairs; ++p) {
+    if (ltpvdw[p] != 2) continue;  // only the lj form is on this path (vdw_terms.f:222-234)
+    for (int i = 1; i <= mxegrd; ++i) {
+      double rrr = (double)i * P.dlrpot, x = psig[p] / rrr, x2 = x * x, sr6 = x2 * x2 * x2;
+      vvv[(size_t)(p - 1) * P.vstride + i] = 4.0 * peps[p] * sr6 * (sr6 - 1.0);
+    }
+  }
+  for (int ti = 0; ti <= KTYPE; ++ti)
+    for (int tj = 0; tj <= KTYPE; ++tj) P.pairidx[ti][tj] = -1;
+  for (int ti = 1; ti <= P.ntypes; ++ti)
+    for (int tj = 1; tj <= P.ntypes; ++tj) {
+      int k = lstvdw[loc2(ti, tj)];
+      // srfrce skip rule, vdw_terms.f:431
+      if (k >= 1 && k <= npairs && ltpvdw[k] < 100 && std::fabs(vvv[(size_t)(k - 1) * P.vstride + 1]) > 1e-10)
+        P.pairidx[ti][tj] = (int8_t)(k - 1);
+    }
+  // tail constants, lrcorrect eadd (vdw_terms.f:581-645)
+  P.ntri = ntri;
+  {
+    int ivdw = 0;
+    double rc = sys->rcut, rc3 = rc * rc * rc, rc9 = rc3 * rc3 * rc3;
+    for (int i = 1; i <= P.ntypes; ++i)
+      for (int j = 1; j <= i; ++j) {
+        ++ivdw;
+        int k = lstvdw[ivdw];
+        double eadd = 0.0;
+        if (k >= 1 && k <= npairs && ltpvdw[k] == 2) {
+          double s2 = psig[k] * psig[k], s6 = s2 * s2 * s2;
+          eadd = 4.0 * peps[k] * (s6 * s6 / (9.0 * rc9) - s6 / (3.0 * rc3));
+        }
+        if (i != j) eadd *= 2.0;
+        P.steadd[ivdw] = eadd;
+      }
+  }
+  // ---- k-vectors, ewald.f:87-94,151-222 (half space, 1.05 rcpcut sphere)
+  std::vector<int> klmn;
+  std::vector<double> akk;
+  {
+    double rb[10];
+    dcell(P.rcell, rb);
+    double twopi = 2.0 * PI;
+    double rcpcut = std::min(std::min((double)P.kmax1 * rb[6], (double)P.kmax2 * rb[7]), (double)P.kmax3 * rb[8]);
+    rcpcut = rcpcut * 1.05 * twopi;
+    double rcpct2 = rcpcut * rcpcut, ralph = -0.25 / (P.alpha * P.alpha);
+    const double *rc = P.rcell;
+    int mmin = 0, nmin = 1;
+    for (int ll = 0; ll <= P.kmax1; ++ll) {
+      double tmp = twopi * (double)ll;
+      double rkx1 = tmp * rc[0], rky1 = tmp * rc[3], rkz1 = tmp * rc[6];
+      for (int mm = mmin; mm <= P.kmax2; ++mm) {
+        tmp = twopi * (double)mm;
+        double rkx2 = rkx1 + tmp * rc[1], rky2 = rky1 + tmp * rc[4], rkz2 = rkz1 + tmp * rc[7];
+        for (int nn = nmin; nn <= P.kmax3; ++nn) {
+          tmp = twopi * (double)nn;
+          double rkx3 = rkx2 + tmp * rc[2], rky3 = rky2 + tmp * rc[5], rkz3 = rkz2 + tmp * rc[8];
+          double rksq = rkx3 * rkx3 + rky3 * rky3 + rkz3 * rkz3;
+          if (rksq <= rcpct2) {
+            klmn.push_back(ll | ((mm + 128) << 8) | ((nn + 128) << 16));
+            akk.push_back(std::exp(ralph * rksq) * (1.0 / rksq));
+          }
+        }
+        nmin = -P.kmax3;
+      }
+      mmin = -P.kmax2;
+    }
+  }
+  P.nk = (int)klmn.size();
+  // ---- species
+  std::vector<double> fwx, fwy, fwz, fwq;
+  std::vector<int> fwinfo, fwtype, fwspec, fwfrz;
+  int ng = 0;
+  double fwq2 = 0.0;
+  for (int m = 1; m <= P.maxmls; ++m) {
+    const fmc_species &sp = sys->species[m - 1];
+    P.natoms[m] = sp.natoms;
+    if (sp.is_guest) {
+      if (sp.natoms > KSITE || sp.natoms < 1) {
+        delete h;
+        return FMC_ERR_ARG;
+      }
+      ++ng;
+      P.locguest[ng] = m;
+      P.gofspecies[m] = ng;
+      double q2 = 0.0, qs = 0.0;
+      for (int i = 0; i < sp.natoms; ++i) {
+        P.site_q[m][i] = sp.site_charge[i];
+        P.site_m[m][i] = sp.site_mass[i];
+        P.site_chgea[m][i] = sp.site_charge[i] / P.epsq * R4PIE0;
+        P.site_t[m][i] = sp.site_type[i];
+        P.site_f[m][i] = sp.site_frozen[i];
+        for (int d = 0; d < 3; ++d) P.tmpl[ng][i][d] = sp.template_xyz[3 * i + d];
+        q2 += sp.site_charge[i] * sp.site_charge[i];
+        qs += sp.site_charge[i];
+        P.numtyp_g[ng][sp.site_type[i]] += 1;
+        if (sp.site_frozen[i]) P.numfrz_g[ng][sp.site_type[i]] += 1;
+      }
+      P.selfq2[ng] = q2;
+      P.chgtmp[ng] = qs;
+    } else {
+      for (int im = 0; im < sp.nmols; ++im)
+        for (int i = 0; i < sp.natoms; ++i) {
+          size_t a = (size_t)im * sp.natoms + i;
+          fwx.push_back(sp.xyz ? sp.xyz[3 * a] : 0.0);
+          fwy.push_back(sp.xyz ? sp.xyz[3 * a + 1] : 0.0);
+          fwz.push_back(sp.xyz ? sp.xyz[3 * a + 2] : 0.0);
+          fwq.push_back(sp.site_charge[i]);
+          fwq2 += sp.site_charge[i] * sp.site_charge[i];
+          fwinfo.push_back(sp.site_type[i] | (m << 8) | ((sp.site_frozen[i] ? 1 : 0) << 16));
+          fwtype.push_back(sp.site_type[i]);
+          fwspec.push_back(m);
+          fwfrz.push_back(sp.site_frozen[i] ? 1 : 0);
+        }
+    }
+  }
+  P.ntpguest = ng;
+  if (ng < 1) {
+    delete h;
+    return FMC_ERR_ARG;
+  }
+  h->selfq2_all_fw = fwq2;
+  P.nfw = (int)fwx.size();
+  // guest capacities: enough for a dense liquid in the cell
+  int slot = 0;
+  for (int g = 1; g <= ng; ++g) {
+    int mol = P.locguest[g];
+    int cap = (int)std::ceil(P.volm / 30.0);
+    cap = std::max(64, std::min(cap, 8192));
+    cap = std::max(cap, sys->species[mol - 1].nmols + 64);
+    if (const char *e = getenv("FMC_GUEST_CAP")) cap = std::max(sys->species[mol - 1].nmols + 1, atoi(e));
+    P.gcap[g] = cap;
+    P.gbase[g] = slot;
+    slot += cap * P.natoms[mol];
+  }
+  P.gatoms_total = slot;
+  // ewald3 exclusions of each guest template
+  for (int g = 1; g <= ng; ++g) P.ewald3en[P.locguest[g]] = ewald3_template(P, P.locguest[g], g, P.natoms[P.locguest[g]]);
+  // ---- device buffers
+  HIPCHK(hipStreamCreate(&h->stream));
+  HIPCHK(hipEventCreate(&h->ev0));
+  HIPCHK(hipEventCreate(&h->ev1));
+  int rc = 0;
+  double *d_erc, *d_vvv, *d_akk, *d_fwx, *d_fwy, *d_fwz, *d_fwq;
+  float *d_erc32, *d_vvv32;
+  int *d_klmn, *d_fwinfo;
+  rc |= dalloc(h, &d_erc, erc.size());
+  rc |= dalloc(h, &d_vvv, vvv.size());
+  rc |= dalloc(h, &d_erc32, erc.size());
+  rc |= dalloc(h, &d_vvv32, vvv.size());
+  rc |= dalloc(h, &d_klmn, klmn.size());
+  rc |= dalloc(h, &d_akk, akk.size());
+  rc |= dalloc(h, &d_fwx, fwx.size());
+  rc |= dalloc(h, &d_fwy, fwy.size());
+  rc |= dalloc(h, &d_fwz, fwz.size());
+  rc |= dalloc(h, &d_fwq, fwq.size());
+  rc |= dalloc(h, &d_fwinfo, fwinfo.size());
+  rc |= dalloc(h, &h->dp, 1);
+  rc |= dalloc(h, &h->dw.ws, (size_t)nwalkers);
+  rc |= dalloc(h, &h->dw.gx, (size_t)nwalkers * P.gatoms_total);
+  rc |= dalloc(h, &h->dw.gy, (size_t)nwalkers * P.gatoms_total);
+  rc |= dalloc(h, &h->dw.gz, (size_t)nwalkers * P.gatoms_total);
+  rc |= dalloc(h, &h->dw.sk, (size_t)nwalkers * (P.maxmls + 1) * P.nk);
+  rc |= dalloc(h, &h->dw.sscr, (size_t)nwalkers * 2 * P.nk);
+  rc |= dalloc(h, (char **)&h->dw.trial_out, 4096);
+  rc |= dalloc(h, &h->dscratch, (size_t)nwalkers * 3 * (KMAXC + 1) + 64);
+  bool per_walker_fw = false;
+  for (int w = 0; w < nwalkers; ++w)
+    if (walkers[w].framework_xyz) per_walker_fw = true;
+  P.fw_per_walker = per_walker_fw ? 1 : 0;
+  if (per_walker_fw) {
+    rc |= dalloc(h, &h->dw.fwx, (size_t)nwalkers * P.nfw);
+    rc |= dalloc(h, &h->dw.fwy, (size_t)nwalkers * P.nfw);
+    rc |= dalloc(h, &h->dw.fwz, (size_t)nwalkers * P.nfw);
+  }
+  if (rc) {
+    fmc_destroy(h);
+    return FMC_ERR_ALLOC;
+  }
+  std::vector<float> erc32(erc.begin(), erc.end()), vvv32(vvv.begin(), vvv.end());
+  HIPCHK(hipMemcpy(d_erc, erc.data(), erc.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_vvv, vvv.data(), vvv.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_erc32, erc32.data(), erc32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_vvv32, vvv32.data(), vvv32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_klmn, klmn.data(), klmn.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_akk, akk.data(), akk.size() * 8, hipMemcpyHostToDevice));
+  if (P.nfw) {
+    HIPCHK(hipMemcpy(d_fwx, fwx.data(), fwx.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_fwy, fwy.data(), fwy.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_fwz, fwz.data(), fwz.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_fwq, fwq.data(), fwq.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_fwinfo, fwinfo.data(), fwinfo.size() * 4, hipMemcpyHostToDevice));
+  }
+  P.erc = d_erc;
+  P.vvv = d_vvv;
+  P.erc32 = d_erc32;
+  P.vvv32 = d_vvv32;
+  P.klmn = d_klmn;
+  P.akk = d_akk;
+  P.fwx = d_fwx;
+  P.fwy = d_fwy;
+  P.fwz = d_fwz;
+  P.fwq = d_fwq;
+  P.fwinfo = d_fwinfo;
+  HIPCHK(hipMemcpy(h->dp, &P, sizeof P, hipMemcpyHostToDevice));
+  if (per_walker_fw) {
+    for (int w = 0; w < nwalkers; ++w) {
+      std::vector<double> x(P.nfw), y(P.nfw), z(P.nfw);
+      for (int a = 0; a < P.nfw; ++a) {
+        if (walkers[w].framework_xyz) {
+          x[a] = walkers[w].framework_xyz[3 * a];
+          y[a] = walkers[w].framework_xyz[3 * a + 1];
+          z[a] = walkers[w].framework_xyz[3 * a + 2];
+        } else {
+          x[a] = fwx[a];
+          y[a] = fwy[a];
+          z[a] = fwz[a];
+        }
+      }
+      HIPCHK(hipMemcpy(h->dw.fwx + (size_t)w * P.nfw, x.data(), 8 * (size_t)P.nfw, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->dw.fwy + (size_t)w * P.nfw, y.data(), 8 * (size_t)P.nfw, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->dw.fwz + (size_t)w * P.nfw, z.data(), 8 * (size_t)P.nfw, hipMemcpyHostToDevice));
+    }
+  }
+  // ---- walkers
+  std::vector<WState> ws(nwalkers);
+  std::vector<double> gx((size_t)P.gatoms_total), gy((size_t)P.gatoms_total), gz((size_t)P.gatoms_total);
+  for (int w = 0; w < nwalkers; ++w) {
+    WState &s = ws[w];
+    memset(&s, 0, sizeof s);
+    const fmc_walker &wk = walkers[w];
+    // duni seeding, utility_pack.f:2816-2836
+    {
+      int i = wk.seed[0], j = wk.seed[1], k = wk.seed[2], l = wk.seed[3];
+      for (int ii = 1; ii <= 97; ++ii) {
+        float sacc = 0.0f, t = 0.5f;
+        for (int jj = 1; jj <= 24; ++jj) {
+          int m = ((i * j) % 179 * k) % 179;
+          i = j;
+          j = k;
+          k = m;
+          l = (53 * l + 1) % 169;
+          if ((l * m) % 64 >= 32) sacc = sacc + t;
+          t = 0.5f * t;
+        }
+        s.u[ii] = sacc;
+      }
+      s.c = 362436.0f / 16777216.0f;
+      s.cd = 7654321.0f / 16777216.0f;
+      s.cm = 16777213.0f / 16777216.0f;
+      s.ir = 97;
+      s.jr = 33;
+    }
+    s.temp = wk.temperature;
+    s.beta = 1.0 / (KBOLTZ * wk.temperature);
+    for (int g = 1; g <= ng; ++g) {
+      s.fug[g] = wk.fugacity[g - 1];
+      for (int mv = 0; mv < FMC_NMOVES; ++mv) s.mcf[g][mv] = wk.move_cum[g - 1][mv];
+      s.delr[g] = wk.delr[g - 1];
+      s.delrdisp[g] = wk.delr[g - 1];
+      s.tran_delr[g] = wk.delr[g - 1];
+    }
+    s.rotangle = PI / 3.0;
+    s.rota_rotangle = s.rotangle;
+    s.sumchg = sys->sumchg;
+    for (int m = 1; m <= P.maxmls; ++m) {
+      const fmc_species &sp = sys->species[m - 1];
+      s.nmols[m] = sp.is_guest ? sp.nmols : 0;
+      double cs = 0.0, c2 = 0.0;
+      for (int im = 0; im < sp.nmols; ++im)
+        for (int i = 0; i < sp.natoms; ++i) {
+          cs += sp.site_charge[i];
+          c2 += sp.site_charge[i] * sp.site_charge[i];
+        }
+      s.chgsum_mol[m] = cs;
+      s.engsic[loc2(m, m)] = c2;
+    }
+    lrcorrect_counts(P, sys, fwtype, fwspec, fwfrz, s);
+    std::fill(gx.begin(), gx.end(), 0.0);
+    std::fill(gy.begin(), gy.end(), 0.0);
+    std::fill(gz.begin(), gz.end(), 0.0);
+    for (int g = 1; g <= ng; ++g) {
+      int mol = P.locguest[g];
+      const fmc_species &sp = sys->species[mol - 1];
+      for (int a = 0; a < sp.nmols * sp.natoms && sp.xyz; ++a) {
+        gx[P.gbase[g] + a] = sp.xyz[3 * a];
+        gy[P.gbase[g] + a] = sp.xyz[3 * a + 1];
+        gz[P.gbase[g] + a] = sp.xyz[3 * a + 2];
+      }
+    }
+    HIPCHK(hipMemcpy(h->dw.gx + (size_t)w * P.gatoms_total, gx.data(), 8 * gx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->dw.gy + (size_t)w * P.gatoms_total, gy.data(), 8 * gy.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->dw.gz + (size_t)w * P.gatoms_total, gz.data(), 8 * gz.size(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * (size_t)nwalkers, hipMemcpyHostToDevice));
+  // full S(k) then the single-point energies (gcmc.f:542-582)
+  resum_dispatch(h, nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<double> sp;
+  if (int e = single_points(h, ws, sp, true)) {
+    fmc_destroy(h);
+    return e;
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * (size_t)nwalkers, hipMemcpyHostToDevice));
+  h->pend_state.assign(nwalkers, 0);
+  *out = h;
+  return FMC_OK;
+}
+
+int fmc_destroy(fmc_handle *h) {
+  if (!h) return FMC_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->dtrace) hipFree(h->dtrace);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return FMC_OK;
+}
+
+int fmc_info(const fmc_handle *h, int *newld, int *mxegrd, int *nframework, int *maxmls) {
+  if (!h) return FMC_ERR_ARG;
+  if (newld) *newld = h->hp.nk;
+  if (mxegrd) *mxegrd = h->hp.mxegrd;
+  if (nframework) *nframework = h->hp.nfw;
+  if (maxmls) *maxmls = h->hp.maxmls;
+  return FMC_OK;
+}
+
+int fmc_single_point(fmc_handle *h, int w, double out[5]) {
+  if (!h || w < 0 || w >= h->nwalkers || !out) return FMC_ERR_ARG;
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  std::vector<double> sp;
+  if (int e = single_points(h, ws, sp, false)) return e;
+  for (int i = 0; i < 5; ++i) out[i] = sp[(size_t)w * 5 + i];
+  return FMC_OK;
+}
+
+int fmc_full_resum(fmc_handle *h, int w, double *max_drift) {
+  if (!h || w < -1 || w >= h->nwalkers) return FMC_ERR_ARG;
+  double *d = h->dscratch;
+  HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * h->nwalkers, h->stream));
+  resum_dispatch(h, d);
+  HIPCHK(hipGetLastError());
+  std::vector<double> dr(h->nwalkers);
+  HIPCHK(hipMemcpyAsync(dr.data(), d, sizeof(double) * h->nwalkers, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (max_drift) {
+    double m = 0.0;
+    for (int i = 0; i < h->nwalkers; ++i)
+      if (w < 0 || i == w) m = std::max(m, dr[i]);
+    *max_drift = m;
+  }
+  return FMC_OK;
+}
+
+static int seam_trial(fmc_handle *h, int w, int kind, int iguest, int choice, const double *xyz, fmc_trial *out) {
+  if (!h || w < 0 || w >= h->nwalkers || !out) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  if (iguest < 1 || iguest > P.ntpguest) return FMC_ERR_ARG;
+  int mol = P.locguest[iguest], natms = P.natoms[mol];
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  if (kind != FMC_MOVE_INSERT && (choice < 1 || choice > s.nmols[mol])) return FMC_ERR_ARG;
+  if (kind == FMC_MOVE_INSERT && s.nmols[mol] >= P.gcap[iguest]) return FMC_ERR_ARG;
+  s.pend_kind = kind;
+  s.pend_iguest = iguest;
+  s.pend_choice = choice;
+  s.pend_valid = 0;
+  for (int i = 0; i < 3 * KSITE; ++i) s.pend_xyz[i] = 0.0;
+  if (xyz)
+    for (int i = 0; i < 3 * natms; ++i) s.pend_xyz[i] = xyz[i];
+  HIPCHK(hipMemcpy(h->dw.ws + w, &s, sizeof s, hipMemcpyHostToDevice));
+  trial_dispatch(h, w);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, h->dw.trial_out, sizeof(fmc_trial), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->pend_state[w] = out->overlap && kind != FMC_MOVE_DELETE ? 2 : 1;
+  return FMC_OK;
+}
+
+int fmc_trial_insert(fmc_handle *h, int w, int iguest, const double *xyz, fmc_trial *out) {
+  if (!xyz) return FMC_ERR_ARG;
+  return seam_trial(h, w, FMC_MOVE_INSERT, iguest, 0, xyz, out);
+}
+
+int fmc_trial_delete(fmc_handle *h, int w, int iguest, int choice, fmc_trial *out) {
+  return seam_trial(h, w, FMC_MOVE_DELETE, iguest, choice, nullptr, out);
+}
+
+int fmc_trial_displace(fmc_handle *h, int w, int iguest, int choice, const double *xyz_new, fmc_trial *out) {
+  if (!xyz_new) return FMC_ERR_ARG;
+  return seam_trial(h, w, FMC_MOVE_DISPLACE, iguest, choice, xyz_new, out);
+}
+
+int fmc_accept(fmc_handle *h, int w) {
+  if (!h || w < 0 || w >= h->nwalkers) return FMC_ERR_ARG;
+  if (h->pend_state[w] != 1) return FMC_ERR_STATE;
+  accept_dispatch(h, w);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->pend_state[w] = 0;
+  return FMC_OK;
+}
+
+int fmc_reject(fmc_handle *h, int w) {
+  if (!h || w < 0 || w >= h->nwalkers) return FMC_ERR_ARG;
+  if (h->pend_state[w] == 0) return FMC_ERR_STATE;
+  h->pend_state[w] = 0;  // nothing on the device changed during the trial
+  return FMC_OK;
+}
+
+int fmc_get_guests(fmc_handle *h, int w, int iguest, double *xyz, int cap_mols, int *nmols) {
+  if (!h || w < 0 || w >= h->nwalkers || !xyz || !nmols) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  if (iguest < 1 || iguest > P.ntpguest) return FMC_ERR_ARG;
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  int mol = P.locguest[iguest], na = P.natoms[mol], nm = s.nmols[mol];
+  *nmols = nm;
+  int n = std::min(nm, cap_mols) * na;
+  std::vector<double> x(n), y(n), z(n);
+  size_t base = (size_t)w * P.gatoms_total + P.gbase[iguest];
+  if (n) {
+    HIPCHK(hipMemcpy(x.data(), h->dw.gx + base, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y.data(), h->dw.gy + base, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(z.data(), h->dw.gz + base, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  }
+  for (int a = 0; a < n; ++a) {
+    xyz[3 * a] = x[a];
+    xyz[3 * a + 1] = y[a];
+    xyz[3 * a + 2] = z[a];
+  }
+  return FMC_OK;
+}
+
+int fmc_get_sk(fmc_handle *h, int w, double *sk) {
+  if (!h || w < 0 || w >= h->nwalkers || !sk) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  HIPCHK(hipMemcpy(sk, h->dw.sk + ((size_t)w * (P.maxmls + 1) + P.maxmls) * P.nk, sizeof(double2) * P.nk,
+                   hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+int fmc_get_stats(fmc_handle *h, int w, fmc_walker_stats *out) {
+  if (!h || w < 0 || w >= h->nwalkers || !out) return FMC_ERR_ARG;
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  memset(out, 0, sizeof *out);
+  out->steps = s.steps;
+  for (int i = 0; i < 6; ++i) {
+    out->accepted[i] = s.acc[i];
+    out->attempted[i] = s.att[i];
+  }
+  const DevParams &P = h->hp;
+  for (int g = 1; g <= P.ntpguest && g <= FMC_MAX_SPECIES; ++g) {
+    out->nmols[g - 1] = s.nmols[P.locguest[g]];
+    out->sum_n[g - 1] = s.sum_n[g];
+    out->sum_n2[g - 1] = s.sum_n2[g];
+  }
+  for (int m = 1; m <= P.maxmls; ++m) out->energy[m - 1] = s.energy[m];
+  out->sum_e = s.sum_e;
+  out->sum_e2 = s.sum_e2;
+  out->counted = s.counted;
+  out->pad = s.overflow;
+  return FMC_OK;
+}
+
+int fmc_reset_stats(fmc_handle *h) {
+  if (!h) return FMC_ERR_ARG;
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  for (auto &s : ws) {
+    s.counted = 0;
+    s.sum_e = s.sum_e2 = 0.0;
+    for (int i = 0; i <= KMAXS; ++i) s.sum_n[i] = s.sum_n2[i] = 0.0;
+    for (int i = 0; i < 6; ++i) s.acc[i] = s.att[i] = 0;
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
+  if (!h || nsteps < 0) return FMC_ERR_ARG;
+  if (nsteps == 0) return FMC_OK;
+  DevWalkers dw = h->dw;
+  if (trace) {
+    if (h->dtrace_cap < nsteps) {
+      if (h->dtrace) hipFree(h->dtrace);
+      h->dtrace = nullptr;
+      HIPCHK(hipMalloc(&h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps));
+      h->dtrace_cap = nsteps;
+    }
+    dw.trace = h->dtrace;
+  } else {
+    dw.trace = nullptr;
+  }
+  DevWalkers saved = h->dw;
+  h->dw = dw;
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  run_dispatch(h, nsteps, count_stats);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->dw = saved;
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_launches = 1;
+  if (trace)
+    HIPCHK(hipMemcpy(trace, h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps, hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_stats) {
+  if (!h || nsweeps < 1 || nsteps_per_sweep < 1) return FMC_ERR_ARG;
+  GraphKey key{nsweeps, nsteps_per_sweep, count_stats};
+  auto it = h->graphs.find(key);
+  if (it == h->graphs.end()) {
+    hipGraph_t graph;
+    DevWalkers saved = h->dw;
+    h->dw.trace = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < nsweeps; ++i) run_dispatch(h, nsteps_per_sweep, count_stats);
+    HIPCHK(hipStreamEndCapture(h->stream, &graph));
+    h->dw = saved;
+    hipGraphExec_t exec;
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    it = h->graphs.emplace(key, exec).first;
+  }
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  HIPCHK(hipGraphLaunch(it->second, h->stream));
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_launches = nsweeps;
+  return FMC_OK;
+}
+
+int fmc_last_timing(fmc_handle *h, double *ms, int *launches) {
+  if (!h) return FMC_ERR_ARG;
+  if (ms) *ms = h->last_ms;
+  if (launches) *launches = h->last_launches;
+  return FMC_OK;
+}
+
+int fmc_set_block(fmc_handle *h, int threads) {
+  if (!h) return FMC_ERR_ARG;
+  if (threads == 0) {
+    // autotune on a snapshot of the walkers: time a short run per block size
+    // and restore the chains afterwards
+    const DevParams &P = h->hp;
+    size_t nws = sizeof(WState) * (size_t)h->nwalkers;
+    size_t ng = 8 * (size_t)h->nwalkers * P.gatoms_total;
+    size_t nsk = sizeof(double2) * (size_t)h->nwalkers * (P.maxmls + 1) * P.nk;
+    void *bws, *bgx, *bgy, *bgz, *bsk;
+    HIPCHK(hipMalloc(&bws, nws));
+    HIPCHK(hipMalloc(&bgx, ng));
+    HIPCHK(hipMalloc(&bgy, ng));
+    HIPCHK(hipMalloc(&bgz, ng));
+    HIPCHK(hipMalloc(&bsk, nsk));
+    HIPCHK(hipMemcpy(bws, h->dw.ws, nws, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(bgx, h->dw.gx, ng, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(bgy, h->dw.gy, ng, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(bgz, h->dw.gz, ng, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(bsk, h->dw.sk, nsk, hipMemcpyDeviceToDevice));
+    int best = 256;
+    double best_ms = 1e30;
+    const int cand[3] = {64, 128, 256};
+    for (int c : cand) {
+      h->block = c;
+      fmc_run(h, 64, nullptr, 0);
+      fmc_run(h, 256, nullptr, 0);
+      if (h->last_ms < best_ms) {
+        best_ms = h->last_ms;
+        best = c;
+      }
+      HIPCHK(hipMemcpy(h->dw.ws, bws, nws, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(h->dw.gx, bgx, ng, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(h->dw.gy, bgy, ng, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(h->dw.gz, bgz, ng, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(h->dw.sk, bsk, nsk, hipMemcpyDeviceToDevice));
+    }
+    hipFree(bws);
+    hipFree(bgx);
+    hipFree(bgy);
+    hipFree(bgz);
+    hipFree(bsk);
+    h->block = best;
+    for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+    return FMC_OK;
+  }
+  if (threads < 64 || threads > 256 || threads % 64) return FMC_ERR_ARG;
+  h->block = threads;
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return FMC_OK;
+}
+
+int fmc_get_block(const fmc_handle *h, int *threads) {
+  if (!h || !threads) return FMC_ERR_ARG;
+  *threads = h->block;
+  return FMC_OK;
+}
+
+}  // extern "C"

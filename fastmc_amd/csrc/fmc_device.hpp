@@ -1,0 +1,173 @@
+// Device-side data layout and building blocks of the GCMC trial-move path
+// for CDNA4 (gfx950).  One workgroup owns one walker (an independent Markov
+// chain at one state point); its mutable state is staged in LDS for the whole
+// launch, the trial energy is split over the workgroup (atoms for the pair
+// terms, k-vectors for the reciprocal terms) and reduced with wavefront
+// shuffles.  Reference routines restated here are cited per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fastmc.h"
+
+namespace fmc {
+
+constexpr int KMAXS = 4;          // species handled by the kernels (framework + 3 guests)
+constexpr int KMAXC = KMAXS * (KMAXS + 1) / 2;
+constexpr int KSITE = 8;          // sites per guest molecule
+constexpr int KTYPE = FMC_MAX_TYPES;
+constexpr int KCAP = 20;          // kmax + 1 cap for the per-site phase tables
+constexpr int KTRI = KTYPE * (KTYPE + 1) / 2;
+constexpr int KWAVES = 16;        // up to 1024 threads per workgroup
+
+constexpr double R4PIE0 = 138935.4835;      // utility_pack.f:113
+constexpr double KBOLTZ = 1.9872041e-3;     // utility_pack.f:121
+constexpr double BOLTZ = 1.3806503e-23;     // utility_pack.f:124
+constexpr double PI = 3.141592653589793;    // utility_pack.f:95
+constexpr double SQRPI = 1.7724538509055159;
+constexpr double DELRMIN = 0.1;             // utility_pack.f:107
+constexpr double MINANGLE = PI / 18.0;      // utility_pack.f:104
+
+// Read-only system description shared by all walkers of a handle.
+struct DevParams {
+  int imcon, maxmls, ntpguest, ntypes, nk, mxegrd, nfw, fw_per_walker;
+  int kmax1, kmax2, kmax3, ntri;
+  int locguest[KMAXS + 1];        // guest g (1-based) -> species index
+  int gofspecies[KMAXS + 1];      // species -> guest g, 0 for framework
+  int natoms[KMAXS + 1];          // by species
+  int gcap[KMAXS + 1];            // guest g: molecule capacity
+  int gbase[KMAXS + 1];           // guest g: first atom slot in the walker's guest arrays
+  int gatoms_total;               // guest atom slots per walker
+  double cell[9], rcell[9], inv_cell[3];  // inv_cell: 1/cell(1), 1/cell(5), 1/cell(9) (images)
+  double volm, statvolm, rcut2, rclim2, overlap2, drewd, rdrewd, dlrpot, rdlrpot;
+  double alpha, epsq, engunit, rvolm;
+  double site_q[KMAXS + 1][KSITE], site_m[KMAXS + 1][KSITE];
+  double site_chgea[KMAXS + 1][KSITE];          // q / epsq * r4pie0 (ewald.f:686)
+  double tmpl[KMAXS + 1][KSITE][3];            // guest g template (COM-centred)
+  int site_t[KMAXS + 1][KSITE], site_f[KMAXS + 1][KSITE];
+  double ewald3en[KMAXS + 1];
+  double selfq2[KMAXS + 1], chgtmp[KMAXS + 1];  // guest g: sum q^2, sum q
+  double steadd[KTRI + 1];
+  int numtyp_g[KMAXS + 1][KTYPE + 1], numfrz_g[KMAXS + 1][KTYPE + 1];  // numtyp_gstmol
+  int8_t pairidx[KTYPE + 1][KTYPE + 1];       // LJ table column or -1 (srfrce skip rule)
+  // tables
+  const double *erc;      // [mxegrd+1]
+  const double *vvv;      // [npt][mxegrd+1]
+  const float *erc32;
+  const float *vvv32;
+  int vstride;
+  // k-vectors
+  const int *klmn;        // packed (ll | (mm+128)<<8 | (nn+128)<<16)
+  const double *akk;
+  // framework (shared or per walker)
+  const double *fwx, *fwy, *fwz;
+  const double *fwq;
+  const int *fwinfo;      // type | species<<8 | frozen<<16
+};
+
+// Per-walker mutable state (global memory; staged into LDS by each launch).
+struct WState {
+  float u[98];
+  float c, cd, cm;
+  int ir, jr;
+  double temp, beta;
+  double fug[KMAXS + 1];
+  float mcf[KMAXS + 1][FMC_NMOVES];
+  double delr[KMAXS + 1], delrdisp[KMAXS + 1], tran_delr[KMAXS + 1];
+  double rotangle, rota_rotangle;
+  int disp_count[KMAXS + 1], accept_disp[KMAXS + 1];
+  int tran_count[KMAXS + 1], accept_tran[KMAXS + 1];
+  int rota_count[KMAXS + 1], accept_rota[KMAXS + 1];
+  int nmols[KMAXS + 1];                     // by species
+  double energy[KMAXS + 2];
+  double sumchg, chgsum_mol[KMAXS + 1], engsic[KMAXC + 1];
+  double elrc, elrc_mol[KMAXC + 1];
+  int numtyp[KTYPE + 1], numfrz[KTYPE + 1];
+  int numtyp_mol[KMAXS + 2][KTYPE + 1], numfrz_mol[KMAXS + 2][KTYPE + 1];
+  // pending trial of the one-trial-at-a-time seam
+  int pend_kind, pend_iguest, pend_choice, pend_valid;
+  double pend_xyz[3 * KSITE];
+  // statistics
+  long long steps, acc[6], att[6], counted;
+  double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
+  int overflow, pad;
+};
+
+// Per-walker device arrays.
+struct DevWalkers {
+  WState *ws;
+  double *gx, *gy, *gz;      // [w][gatoms_total]
+  double2 *sk;               // [w][maxmls+1][nk]: species rows then the total
+  double2 *sscr;             // [w][2][nk]: s_new, s_old of the current trial
+  double *fwx, *fwy, *fwz;   // per-walker framework copies (if fw_per_walker)
+  fmc_trace_rec *trace;      // walker 0 trace or null
+  fmc_trial *trial_out;      // seam output
+};
+
+__host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872
+  int a = i > j ? i : j, b = i > j ? j : i;
+  return (a * (a - 1)) / 2 + b;
+}
+
+// duni, utility_pack.f:2768-2857 (binary32, exact)
+__device__ inline double duni(WState &w) {
+  float uni = w.u[w.ir] - w.u[w.jr];
+  if (uni < 0.0f) uni = uni + 1.0f;
+  w.u[w.ir] = uni;
+  w.ir -= 1;
+  if (w.ir == 0) w.ir = 97;
+  w.jr -= 1;
+  if (w.jr == 0) w.jr = 97;
+  w.c = w.c - w.cd;
+  if (w.c < 0.0f) w.c = w.c + w.cm;
+  uni = uni - w.c;
+  if (uni < 0.0f) uni = uni + 1.0f;
+  return (double)uni;
+}
+
+// images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half away)
+__device__ inline void min_image(const DevParams &P, double &dx, double &dy, double &dz) {
+  if (P.imcon == 1) {
+    double a = P.inv_cell[0];
+    dx = dx - P.cell[0] * round(a * dx);
+    dy = dy - P.cell[0] * round(a * dy);
+    dz = dz - P.cell[0] * round(a * dz);
+  } else if (P.imcon == 2) {
+    dx = dx - P.cell[0] * round(P.inv_cell[0] * dx);
+    dy = dy - P.cell[4] * round(P.inv_cell[1] * dy);
+    dz = dz - P.cell[8] * round(P.inv_cell[2] * dz);
+  } else {
+    const double *r = P.rcell, *c = P.cell;
+    double sx = r[0] * dx + r[3] * dy + r[6] * dz;
+    double sy = r[1] * dx + r[4] * dy + r[7] * dz;
+    double sz = r[2] * dx + r[5] * dy + r[8] * dz;
+    sx -= round(sx);
+    sy -= round(sy);
+    sz -= round(sz);
+    dx = c[0] * sx + c[3] * sy + c[6] * sz;
+    dy = c[1] * sx + c[4] * sy + c[7] * sz;
+    dz = c[2] * sx + c[5] * sy + c[8] * sz;
+  }
+}
+
+// 3-point interpolation of ewald.f:703-715 / vdw_terms.f:441-453
+template <typename T>
+__device__ inline T interp3(const T *tab, T r, T rinv_step) {
+  T x = r * rinv_step;
+  int l = (int)x;
+  T p = x - (T)l;
+  T v0 = tab[l], v1 = tab[l + 1], v2 = tab[l + 2];
+  T t1 = v0 + (v1 - v0) * p;
+  T t2 = v1 + (v2 - v1) * (p - (T)1);
+  return t1 + (t2 - t1) * p * (T)0.5;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace fmc

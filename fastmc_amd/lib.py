@@ -1,0 +1,177 @@
+"""ctypes binding of libfastmc_hip.so (the C-ABI in include/fastmc.h).
+
+The library is the only compute path: if it is missing or fails to load,
+every entry point raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .system import (CSystem, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, TRACE_DTYPE,
+                     System, make_walker)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
+
+FMC_FP64 = 0
+FMC_MIXED = 1
+
+EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_full_resum",
+           "fmc_trial_insert", "fmc_trial_delete", "fmc_trial_displace", "fmc_accept",
+           "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
+           "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block")
+
+_lib = None
+
+
+class FastMCError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        super().__init__(f"{fn} returned error code {code}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libfastmc_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m fastmc_amd._build` "
+                           "(or __graft_entry__.build()); there is no fallback path")
+    L = C.CDLL(path)
+    for name in EXPORTS:
+        getattr(L, name).restype = C.c_int
+    L.fmc_create.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+    _lib = L
+    return L
+
+
+def _chk(fn, rc):
+    if rc != 0:
+        raise FastMCError(fn, rc)
+
+
+class Handle:
+    """A set of walkers (independent GCMC chains) on the GPU sharing one chemistry."""
+
+    def __init__(self, sysm: System, walkers=None, precision: int = FMC_FP64,
+                 seeds=None, temperatures=None, pressures=None, framework_xyz=None, n: int = 1):
+        L = load_library()
+        self.sysm = sysm
+        self._keep = []
+        self.csys = CSystem(sysm)
+        if walkers is None:
+            walkers = []
+            for i in range(n):
+                sd = seeds[i] if seeds is not None else (25 + i % 150, 89, 100, 150)
+                t = temperatures[i] if temperatures is not None else None
+                p = pressures[i] if pressures is not None else None
+                fx = framework_xyz[i] if framework_xyz is not None else None
+                walkers.append(make_walker(sysm, sd, t, p, fx, self._keep))
+        self.nwalkers = len(walkers)
+        arr = (FmcWalker * self.nwalkers)(*walkers)
+        self._keep.append(arr)
+        h = C.c_void_p()
+        _chk("fmc_create", L.fmc_create(C.byref(self.csys.struct), self.nwalkers, arr, precision,
+                                        C.byref(h)))
+        self.h = h
+        self.L = L
+        nk, mx, nfw, mm = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _chk("fmc_info", L.fmc_info(self.h, C.byref(nk), C.byref(mx), C.byref(nfw), C.byref(mm)))
+        self.newld, self.mxegrd, self.nframework, self.maxmls = nk.value, mx.value, nfw.value, mm.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- whole-system
+    def single_point(self, w: int = 0) -> np.ndarray:
+        out = np.zeros(5)
+        _chk("fmc_single_point", self.L.fmc_single_point(self.h, w, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def full_resum(self, w: int = -1) -> float:
+        d = C.c_double()
+        _chk("fmc_full_resum", self.L.fmc_full_resum(self.h, w, C.byref(d)))
+        return d.value
+
+    # ---- seam
+    def _trial(self, name, *args):
+        t = FmcTrial()
+        _chk(name, getattr(self.L, name)(self.h, *args, C.byref(t)))
+        return t
+
+    def trial_insert(self, iguest, xyz, w=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        return self._trial("fmc_trial_insert", C.c_int(w), C.c_int(iguest), x.ctypes.data_as(C.c_void_p))
+
+    def trial_delete(self, iguest, choice, w=0):
+        return self._trial("fmc_trial_delete", C.c_int(w), C.c_int(iguest), C.c_int(choice))
+
+    def trial_displace(self, iguest, choice, xyz, w=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        return self._trial("fmc_trial_displace", C.c_int(w), C.c_int(iguest), C.c_int(choice),
+                           x.ctypes.data_as(C.c_void_p))
+
+    def accept(self, w=0):
+        _chk("fmc_accept", self.L.fmc_accept(self.h, C.c_int(w)))
+
+    def reject(self, w=0):
+        _chk("fmc_reject", self.L.fmc_reject(self.h, C.c_int(w)))
+
+    # ---- device chains
+    def run(self, nsteps: int, trace: bool = False, count_stats: bool = False):
+        tr = np.zeros(nsteps, dtype=TRACE_DTYPE) if trace else None
+        ptr = tr.ctypes.data_as(C.c_void_p) if trace else None
+        _chk("fmc_run", self.L.fmc_run(self.h, C.c_int(nsteps), ptr, C.c_int(int(count_stats))))
+        return tr
+
+    def run_graph(self, nsweeps: int, nsteps_per_sweep: int, count_stats: bool = False):
+        _chk("fmc_run_graph", self.L.fmc_run_graph(self.h, C.c_int(nsweeps), C.c_int(nsteps_per_sweep),
+                                                   C.c_int(int(count_stats))))
+
+    def last_timing(self):
+        ms, n = C.c_double(), C.c_int()
+        _chk("fmc_last_timing", self.L.fmc_last_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_block(self, threads: int):
+        _chk("fmc_set_block", self.L.fmc_set_block(self.h, C.c_int(threads)))
+
+    def get_block(self) -> int:
+        t = C.c_int()
+        _chk("fmc_get_block", self.L.fmc_get_block(self.h, C.byref(t)))
+        return t.value
+
+    # ---- state readback
+    def guests(self, iguest: int = 1, w: int = 0, cap: int = 1 << 16) -> np.ndarray:
+        nat = self.sysm.guests[iguest - 1].natoms
+        buf = np.zeros(3 * nat * cap)
+        n = C.c_int()
+        _chk("fmc_get_guests", self.L.fmc_get_guests(self.h, C.c_int(w), C.c_int(iguest),
+                                                     buf.ctypes.data_as(C.c_void_p), C.c_int(cap),
+                                                     C.byref(n)))
+        return buf[: 3 * nat * min(n.value, cap)].reshape(-1, 3)
+
+    def sk(self, w: int = 0) -> np.ndarray:
+        out = np.zeros(2 * self.newld)
+        _chk("fmc_get_sk", self.L.fmc_get_sk(self.h, C.c_int(w), out.ctypes.data_as(C.c_void_p)))
+        return out.reshape(-1, 2)
+
+    def stats(self, w: int = 0) -> FmcWalkerStats:
+        s = FmcWalkerStats()
+        _chk("fmc_get_stats", self.L.fmc_get_stats(self.h, C.c_int(w), C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        _chk("fmc_reset_stats", self.L.fmc_reset_stats(self.h))
