@@ -62,6 +62,7 @@ struct fmc_handle {
   // host mirrors for single_point
   std::vector<double> fw_q;
   double selfq2_all_fw = 0.0;
+  std::vector<int> kperm;        // internal k -> reference enumeration index
 };
 
 namespace {
@@ -505,6 +506,30 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
     }
   }
   P.nk = (int)klmn.size();
+  // internal order: by a_k, largest first (the sum over k is order-free; S(k)
+  // readback restores the reference's enumeration order)
+  {
+    std::vector<int> order(P.nk);
+    for (int i = 0; i < P.nk; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return akk[a] > akk[b]; });
+    std::vector<int> k2(P.nk);
+    std::vector<double> a2(P.nk);
+    for (int i = 0; i < P.nk; ++i) {
+      k2[i] = klmn[order[i]];
+      a2[i] = akk[order[i]];
+    }
+    klmn.swap(k2);
+    akk.swap(a2);
+    h->kperm = order;
+    double split = 1e-4;
+    if (const char *e = getenv("FMC_MIXED_AK_SPLIT")) split = atof(e);
+    int n64 = P.nk;
+    if (precision == FMC_MIXED) {
+      n64 = 0;
+      while (n64 < P.nk && akk[n64] >= akk[0] * split) ++n64;
+    }
+    P.nk64 = n64;
+  }
   // ---- species
   std::vector<double> fwx, fwy, fwz, fwq;
   std::vector<int> fwinfo, fwtype, fwspec, fwfrz;
@@ -878,8 +903,13 @@ int fmc_get_guests(fmc_handle *h, int w, int iguest, double *xyz, int cap_mols, 
 int fmc_get_sk(fmc_handle *h, int w, double *sk) {
   if (!h || w < 0 || w >= h->nwalkers || !sk) return FMC_ERR_ARG;
   const DevParams &P = h->hp;
-  HIPCHK(hipMemcpy(sk, h->dw.sk + ((size_t)w * (P.maxmls + 1) + P.maxmls) * P.nk, sizeof(double2) * P.nk,
+  std::vector<double2> buf(P.nk);
+  HIPCHK(hipMemcpy(buf.data(), h->dw.sk + ((size_t)w * (P.maxmls + 1) + P.maxmls) * P.nk, sizeof(double2) * P.nk,
                    hipMemcpyDeviceToHost));
+  for (int i = 0; i < P.nk; ++i) {
+    sk[2 * h->kperm[i]] = buf[i].x;
+    sk[2 * h->kperm[i] + 1] = buf[i].y;
+  }
   return FMC_OK;
 }
 

@@ -33,6 +33,7 @@ constexpr double MINANGLE = PI / 18.0;      // utility_pack.f:104
 struct DevParams {
   int imcon, maxmls, ntpguest, ntypes, nk, mxegrd, nfw, fw_per_walker;
   int kmax1, kmax2, kmax3, ntri;
+  int nk64;                       // k-vectors [0, nk64) use FP64 phase products (list sorted by a_k)
   int locguest[KMAXS + 1];        // guest g (1-based) -> species index
   int gofspecies[KMAXS + 1];      // species -> guest g, 0 for framework
   int natoms[KMAXS + 1];          // by species

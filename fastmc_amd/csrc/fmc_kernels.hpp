@@ -269,28 +269,34 @@ __device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS>
 
 // ----------------------------------------------------------- evaluation --
 template <int NS, typename T>
-__device__ inline void pair_site(const DevParams &P, const double *xs, T chgea, int ts, double xj,
+__device__ inline void pair_site(const DevParams &P, const double *xs, double chgea, int ts, double xj,
                                  double yj, double zj, double qj, int tj, int spj, double *acc_real,
                                  double *acc_vdw, int &ovl) {
+  // distance, cutoff and overlap tests and the real-space Coulomb term stay
+  // in FP64 in both modes (pair Coulomb terms are large and cancel); T only
+  // sets the precision of the LJ table interpolation, whose fractional
+  // position is formed in FP64 first
   double dx = xs[0] - xj, dy = xs[1] - yj, dz = xs[2] - zj;
   min_image(P, dx, dy, dz);
-  T fx = (T)dx, fy = (T)dy, fz = (T)dz;
-  T rsq = fx * fx + fy * fy + fz * fz;
-  if (rsq < (T)P.overlap2) ovl = 1;
-  if (rsq < (T)P.rcut2) {
-    T r = sqrt(rsq);
-    T chgprd = chgea * (T)qj;
+  double rsq = dx * dx + dy * dy + dz * dz;
+  if (rsq < P.overlap2) ovl = 1;
+  if (rsq < P.rcut2) {
+    double r = sqrt(rsq);
+    double chgprd = chgea * qj;
     double e2 = 0.0, ev = 0.0;
-    if (fabs((double)chgprd) > 1e-10) {
-      const T *erc;
-      if constexpr (sizeof(T) == 8) erc = (const T *)P.erc; else erc = (const T *)P.erc32;
-      e2 = (double)(interp3<T>(erc, r, (T)P.rdrewd) * chgprd);
-    }
+    if (fabs(chgprd) > 1e-10) e2 = interp3<double>(P.erc, r, P.rdrewd) * chgprd;
     int k = P.pairidx[ts][tj];
     if (k >= 0) {
+      double x = r * P.rdlrpot;
+      int l = (int)x;
+      T pp = (T)(x - (double)l);
       const T *vv;
       if constexpr (sizeof(T) == 8) vv = (const T *)P.vvv; else vv = (const T *)P.vvv32;
-      ev = (double)interp3<T>(vv + (size_t)k * P.vstride, r, (T)P.rdlrpot);
+      vv += (size_t)k * P.vstride + l;
+      T v0 = vv[0], v1 = vv[1], v2 = vv[2];
+      T t1 = v0 + (v1 - v0) * pp;
+      T t2 = v1 + (v2 - v1) * (pp - (T)1);
+      ev = (double)(t1 + (t2 - t1) * pp * (T)0.5);
     }
 #pragma unroll
     for (int i = 0; i < NS; ++i)
@@ -369,9 +375,9 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
     int inf = P.fwinfo[j], tj = inf & 255, spj = (inf >> 8) & 255;
     for (int s = 0; s < natms; ++s) {
       if (sh.has_new)
-        pair_site<NS, T>(P, sh.xn[s], (T)P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc, acc + NS, ovl_new);
+        pair_site<NS, T>(P, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc, acc + NS, ovl_new);
       if (sh.has_old)
-        pair_site<NS, T>(P, sh.xo[s], (T)P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc + 2 * NS, acc + 3 * NS, ovl_old);
+        pair_site<NS, T>(P, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc + 2 * NS, acc + 3 * NS, ovl_old);
     }
   }
   // ---- pair terms: guest atoms (own molecule excluded)
@@ -387,16 +393,17 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
       int tj = P.site_t[molg][site];
       for (int s = 0; s < natms; ++s) {
         if (sh.has_new)
-          pair_site<NS, T>(P, sh.xn[s], (T)P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc, acc + NS, ovl_new);
+          pair_site<NS, T>(P, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc, acc + NS, ovl_new);
         if (sh.has_old)
-          pair_site<NS, T>(P, sh.xo[s], (T)P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc + 2 * NS, acc + 3 * NS, ovl_old);
+          pair_site<NS, T>(P, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc + 2 * NS, acc + 3 * NS, ovl_old);
       }
     }
   }
   // ---- reciprocal terms: one k-vector per thread
   const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
   double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
-  for (int k = tid; k < P.nk; k += nt) {
+  auto recip_k = [&](int k, auto tag) {
+    using U = decltype(tag);
     int lmn = P.klmn[k];
     int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
     double a = P.akk[k];
@@ -406,10 +413,10 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
     for (int i = 0; i < NS; ++i) Si[i] = skw[(size_t)i * P.nk + k];
     double soc = 0.0, sos = 0.0;
     if (sh.has_old) {
-      T c = (T)0, s_ = (T)0;
+      U c = (U)0, s_ = (U)0;
       for (int s = 0; s < natms; ++s) {
-        T ckc, cks;
-        site_phase<T>(sh.ph[1][s], ll, mm, nn, (T)P.site_q[mol][s], ckc, cks);
+        U ckc, cks;
+        site_phase<U>(sh.ph[1][s], ll, mm, nn, (U)P.site_q[mol][s], ckc, cks);
         c += ckc;
         s_ += cks;
       }
@@ -425,10 +432,10 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
       scr[P.nk + k] = make_double2(soc, sos);
     }
     if (sh.has_new) {
-      T c = (T)0, s_ = (T)0;
+      U c = (U)0, s_ = (U)0;
       for (int s = 0; s < natms; ++s) {
-        T ckc, cks;
-        site_phase<T>(sh.ph[0][s], ll, mm, nn, (T)P.site_q[mol][s], ckc, cks);
+        U ckc, cks;
+        site_phase<U>(sh.ph[0][s], ll, mm, nn, (U)P.site_q[mol][s], ckc, cks);
         c += ckc;
         s_ += cks;
       }
@@ -452,7 +459,11 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
       }
       scr[k] = make_double2(snc, sns);
     }
-  }
+  };
+  // the k list is sorted by a_k: the leading (large-weight) vectors always use
+  // FP64 phase products, the tail uses T
+  for (int k = tid; k < P.nk64; k += nt) recip_k(k, 0.0);
+  for (int k = P.nk64 + tid; k < P.nk; k += nt) recip_k(k, (T)0);
   // ---- workgroup reduction: wavefront butterfly, then across waves
   const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
 #pragma unroll
