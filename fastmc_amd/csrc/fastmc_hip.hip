@@ -41,7 +41,7 @@ struct GraphKey {
 }  // namespace
 
 struct fmc_handle {
-  int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 256;
+  int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
   DevParams hp;                  // host copy
   DevParams *dp = nullptr;       // device copy
   DevWalkers dw{};
@@ -159,7 +159,32 @@ double ewald3_template(const DevParams &P, int mol, int g, int natms) {
 
 template <int NS, typename T>
 void launch_run(fmc_handle *h, int nsteps, int count) {
-  hipLaunchKernelGGL((k_run<NS, T>), dim3(h->nwalkers), dim3(h->block), 0, h->stream, h->dp, h->dw, nsteps, count);
+  hipLaunchKernelGGL((k_run<NS, T>), dim3(h->nwalkers), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, nsteps, count);
+}
+
+size_t smem_size(int ns) {
+  switch (ns) {
+    case 2: return sizeof(Smem<2>);
+    case 3: return sizeof(Smem<3>);
+    default: return sizeof(Smem<4>);
+  }
+}
+
+template <int NS, typename T>
+hipError_t allow_lds(int bytes) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_run<NS, T>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_trial<NS, T>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_accept<NS>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t allow_lds_dispatch(int ns, int mixed, int bytes) {
+  switch (ns) {
+    case 2: return mixed ? allow_lds<2, float>(bytes) : allow_lds<2, double>(bytes);
+    case 3: return mixed ? allow_lds<3, float>(bytes) : allow_lds<3, double>(bytes);
+    default: return mixed ? allow_lds<4, float>(bytes) : allow_lds<4, double>(bytes);
+  }
 }
 
 void run_dispatch(fmc_handle *h, int nsteps, int count) {
@@ -173,7 +198,7 @@ void run_dispatch(fmc_handle *h, int nsteps, int count) {
 
 template <int NS, typename T>
 void launch_trial(fmc_handle *h, int w) {
-  hipLaunchKernelGGL((k_trial<NS, T>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w);
+  hipLaunchKernelGGL((k_trial<NS, T>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w);
 }
 
 void trial_dispatch(fmc_handle *h, int w) {
@@ -187,9 +212,9 @@ void trial_dispatch(fmc_handle *h, int w) {
 
 void accept_dispatch(fmc_handle *h, int w) {
   switch (h->ns) {
-    case 2: hipLaunchKernelGGL((k_accept<2>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
-    case 3: hipLaunchKernelGGL((k_accept<3>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
-    default: hipLaunchKernelGGL((k_accept<4>), dim3(1), dim3(h->block), 0, h->stream, h->dp, h->dw, w); break;
+    case 2: hipLaunchKernelGGL((k_accept<2>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w); break;
+    case 3: hipLaunchKernelGGL((k_accept<3>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w); break;
+    default: hipLaunchKernelGGL((k_accept<4>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w); break;
   }
 }
 
@@ -599,8 +624,88 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   P.gatoms_total = slot;
   // ewald3 exclusions of each guest template
   for (int g = 1; g <= ng; ++g) P.ewald3en[P.locguest[g]] = ewald3_template(P, P.locguest[g], g, P.natoms[P.locguest[g]]);
+  // ---- gstlrcorrect (vdw_terms.f:717-765) as linear coefficients in the guest counts
+  {
+    const double twopi = 2.0 * PI, volm = P.volm;
+    int F[KTYPE + 1] = {0}, Ff[KTYPE + 1] = {0};
+    int Fm[KMAXS + 1][KTYPE + 1] = {{0}}, Ffm[KMAXS + 1][KTYPE + 1] = {{0}};
+    for (size_t a = 0; a < fwtype.size(); ++a) {
+      F[fwtype[a]] += 1;
+      Fm[fwspec[a]][fwtype[a]] += 1;
+      if (fwfrz[a]) {
+        Ff[fwtype[a]] += 1;
+        Ffm[fwspec[a]][fwtype[a]] += 1;
+      }
+    }
+    for (int g = 1; g <= ng; ++g) {
+      int mol = P.locguest[g];
+      for (int lex = 0; lex <= 1; ++lex) {
+        double sg = lex ? -1.0 : 1.0;
+        double c0 = 0.0, c1[KMAXS + 1] = {0}, m0[KMAXS + 1] = {0}, m1[KMAXS + 1] = {0};
+        int ivdw = 0;
+        for (int i = 1; i <= P.ntypes; ++i)
+          for (int j = 1; j <= i; ++j) {
+            ++ivdw;
+            double w0 = twopi / volm * P.steadd[ivdw];
+            double na = P.numtyp_g[g][i], nb = P.numtyp_g[g][j], fa = P.numfrz_g[g][i], fb = P.numfrz_g[g][j];
+            c0 += w0 * (sg * (na * F[j] + nb * F[i] - fa * Ff[j] - fb * Ff[i]) + (na * nb - fa * fb));
+            for (int g2 = 1; g2 <= ng; ++g2)
+              c1[g2] += w0 * sg * (na * P.numtyp_g[g2][j] + nb * P.numtyp_g[g2][i] - fa * P.numfrz_g[g2][j] -
+                                   fb * P.numfrz_g[g2][i]);
+            for (int jt = 1; jt <= P.maxmls; ++jt) {
+              double ssg = (lex && jt == mol) ? -1.0 : 1.0;
+              double self = (jt == mol) ? (na * nb - fa * fb) : 0.0;
+              int g2 = P.gofspecies[jt];
+              if (g2) {
+                m0[jt] += w0 * self;
+                m1[jt] += w0 * ssg * (na * P.numtyp_g[g2][j] + nb * P.numtyp_g[g2][i] - fa * P.numfrz_g[g2][j] -
+                                      fb * P.numfrz_g[g2][i]);
+              } else {
+                m0[jt] += w0 * (ssg * (na * Fm[jt][j] + nb * Fm[jt][i] - fa * Ffm[jt][j] - fb * Ffm[jt][i]) + self);
+              }
+            }
+          }
+        P.tl_c0[g][lex] = c0;
+        for (int g2 = 1; g2 <= ng; ++g2) P.tl_c1[g][lex][g2] = c1[g2];
+        for (int jt = 1; jt <= P.maxmls; ++jt) {
+          P.tm_c0[g][lex][jt] = m0[jt];
+          P.tm_c1[g][lex][jt] = m1[jt];
+        }
+      }
+    }
+  }
+  // ---- LDS plan: Smem, erfc table, then the framework and the k list if they fit
+  {
+    const int budget = 160 * 1024 - 1024;
+    size_t off = (smem_size(h->ns) + 255) & ~(size_t)255;
+    P.off_erc = (int)off;
+    off += (size_t)8 * (mxegrd + 3);
+    off = (off + 15) & ~(size_t)15;
+    size_t fwb = (size_t)36 * P.nfw;
+    P.lds_fw = 0;
+    if (P.nfw > 0 && off + fwb <= (size_t)budget && !getenv("FMC_NO_LDS_FW")) {
+      P.lds_fw = 1;
+      P.off_fw = (int)off;
+      off += fwb;
+      off = (off + 15) & ~(size_t)15;
+    }
+    size_t kb = (size_t)12 * P.nk;
+    P.lds_k = 0;
+    if (off + kb <= (size_t)budget && !getenv("FMC_NO_LDS_K")) {
+      P.lds_k = 1;
+      P.off_k = (int)off;
+      off += kb;
+    }
+    P.lds_erc = 1;
+    P.lds_bytes = (int)((off + 15) & ~(size_t)15);
+    if (off > (size_t)budget) {
+      delete h;
+      return FMC_ERR_ARG;
+    }
+  }
   // ---- device buffers
   HIPCHK(hipStreamCreate(&h->stream));
+  HIPCHK(allow_lds_dispatch(h->ns, precision == FMC_MIXED, P.lds_bytes));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
   int rc = 0;
@@ -708,11 +813,8 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
         }
         s.u[ii] = sacc;
       }
-      s.c = 362436.0f / 16777216.0f;
-      s.cd = 7654321.0f / 16777216.0f;
-      s.cm = 16777213.0f / 16777216.0f;
-      s.ir = 97;
-      s.jr = 33;
+      s.mz_made = 0;
+      s.mz_used = 0;
     }
     s.temp = wk.temperature;
     s.beta = 1.0 / (KBOLTZ * wk.temperature);
@@ -1038,9 +1140,9 @@ int fmc_set_block(fmc_handle *h, int threads) {
     HIPCHK(hipMemcpy(bgy, h->dw.gy, ng, hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(bgz, h->dw.gz, ng, hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(bsk, h->dw.sk, nsk, hipMemcpyDeviceToDevice));
-    int best = 256;
+    int best = 512;
     double best_ms = 1e30;
-    const int cand[3] = {64, 128, 256};
+    const int cand[3] = {128, 256, 512};
     for (int c : cand) {
       h->block = c;
       fmc_run(h, 64, nullptr, 0);
@@ -1065,7 +1167,7 @@ int fmc_set_block(fmc_handle *h, int threads) {
     h->graphs.clear();
     return FMC_OK;
   }
-  if (threads < 64 || threads > 256 || threads % 64) return FMC_ERR_ARG;
+  if (threads < 64 || threads > 512 || threads % 64) return FMC_ERR_ARG;
   h->block = threads;
   for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
   h->graphs.clear();

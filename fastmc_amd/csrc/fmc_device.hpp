@@ -34,6 +34,14 @@ struct DevParams {
   int imcon, maxmls, ntpguest, ntypes, nk, mxegrd, nfw, fw_per_walker;
   int kmax1, kmax2, kmax3, ntri;
   int nk64;                       // k-vectors [0, nk64) use FP64 phase products (list sorted by a_k)
+  // LDS staging plan (bytes from the start of the dynamic LDS block)
+  int lds_fw, lds_erc, lds_k;     // 1 if staged
+  int off_fw, off_erc, off_k, lds_bytes;
+  // gstlrcorrect as a linear function of the guest counts (exact algebra of
+  // vdw_terms.f:717-765): delrc(g, lexisting) = tl_c0 + sum_g' tl_c1[g'] N_g',
+  // delrc_mol(loc2(mol, jt)) = tm_c0[jt] + tm_c1[jt] N_jt
+  double tl_c0[KMAXS + 1][2], tl_c1[KMAXS + 1][2][KMAXS + 1];
+  double tm_c0[KMAXS + 1][2][KMAXS + 1], tm_c1[KMAXS + 1][2][KMAXS + 1];
   int locguest[KMAXS + 1];        // guest g (1-based) -> species index
   int gofspecies[KMAXS + 1];      // species -> guest g, 0 for framework
   int natoms[KMAXS + 1];          // by species
@@ -69,9 +77,11 @@ struct DevParams {
 
 // Per-walker mutable state (global memory; staged into LDS by each launch).
 struct WState {
+  // duni (utility_pack.f:2768-2857) as a lag-97/33 subtractive generator with
+  // a pre-generated output ring: rq[n & 127] holds call n+1's value
   float u[98];
-  float c, cd, cm;
-  int ir, jr;
+  float rq[128];
+  long long mz_made, mz_used;
   double temp, beta;
   double fug[KMAXS + 1];
   float mcf[KMAXS + 1][FMC_NMOVES];
@@ -111,20 +121,36 @@ __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872
   return (a * (a - 1)) / 2 + b;
 }
 
-// duni, utility_pack.f:2768-2857 (binary32, exact)
+// Next duni value from the ring (thread 0).  The ring is refilled by
+// mzt_refill before each step, so at least 33 values are always available.
 __device__ inline double duni(WState &w) {
-  float uni = w.u[w.ir] - w.u[w.jr];
-  if (uni < 0.0f) uni = uni + 1.0f;
-  w.u[w.ir] = uni;
-  w.ir -= 1;
-  if (w.ir == 0) w.ir = 97;
-  w.jr -= 1;
-  if (w.jr == 0) w.jr = 97;
-  w.c = w.c - w.cd;
-  if (w.c < 0.0f) w.c = w.c + w.cm;
-  uni = uni - w.c;
-  if (uni < 0.0f) uni = uni + 1.0f;
-  return (double)uni;
+  float v = w.rq[w.mz_used & 127];
+  w.mz_used += 1;
+  return (double)v;
+}
+
+// duni, utility_pack.f:2768-2857, 33 calls at once on lanes 0..32 of one
+// wave.  Call m (1-based) computes u(ir) - u(jr) with ir = 97 - (m-1) mod 97
+// and jr = ir of call m-33, so the 33 calls of a round read only values made
+// before the round; c after m calls is (362436 - m 7654321) mod 16777213 in
+// units of 2^-24.  All operations are exact in binary32.
+__device__ inline void mzt_refill(WState &w, int lane) {
+  if (lane < 33) {
+    long long m = w.mz_made + 1 + lane;
+    int ir = 97 - (int)((m - 1) % 97);
+    int jr = (int)(((32 - (m - 1)) % 97 + 97) % 97) + 1;
+    float a = w.u[ir], b = w.u[jr];
+    float uni = a - b;
+    if (uni < 0.0f) uni = uni + 1.0f;
+    long long cm = 16777213LL, cd = 7654321LL;
+    long long cc = ((362436LL - (m % cm) * cd) % cm + cm) % cm;
+    float c = (float)cc * (1.0f / 16777216.0f);
+    float out = uni - c;
+    if (out < 0.0f) out = out + 1.0f;
+    __builtin_amdgcn_wave_barrier();
+    w.u[ir] = uni;
+    w.rq[(w.mz_made + lane) & 127] = out;
+  }
 }
 
 // images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half away)
@@ -169,6 +195,26 @@ __device__ inline T wave_sum(T v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+template <int CTRL, int ROW, int BANK>
+__device__ inline double dpp_f64(double x) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW, BANK, true);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW, BANK, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave with DPP row shifts and row broadcasts
+// (gfx9 cross-lane pattern); the total lands in lane 63.
+__device__ inline double wave_sum_dpp(double v) {
+  double s = v + dpp_f64<0x111, 0xf, 0xf>(v);   // row_shr:1
+  s += dpp_f64<0x112, 0xf, 0xf>(v);             // row_shr:2
+  s += dpp_f64<0x113, 0xf, 0xf>(v);             // row_shr:3
+  s += dpp_f64<0x114, 0xf, 0xe>(s);             // row_shr:4, banks 1-3
+  s += dpp_f64<0x118, 0xf, 0xc>(s);             // row_shr:8, banks 2-3
+  s += dpp_f64<0x142, 0xa, 0xf>(s);             // row_bcast:15 into rows 1, 3
+  s += dpp_f64<0x143, 0xc, 0xf>(s);             // row_bcast:31 into rows 2, 3
+  return s;
 }
 
 }  // namespace fmc

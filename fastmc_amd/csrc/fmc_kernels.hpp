@@ -25,45 +25,26 @@ struct Smem {
   double2 ph[2][KSITE][3][KCAP];
   double red[KWAVES][Acc<NS>::NACC];
   double tot[Acc<NS>::NACC];
+  int ovlw[KWAVES][2];
   double estep, rande, delrc;
   double delE[KMAXS + 1], delrc_mol[KMAXC + 1];
   double comp[4];
 };
 
 // ------------------------------------------------------------ tail term --
-// gstlrcorrect, vdw_terms.f:686-776
+// gstlrcorrect, vdw_terms.f:686-776, in its linear form (coefficients built on
+// the host from the same expressions): total delrc and the loc2(mol, jt)
+// entries of delrc_mol.
 template <int NS>
 __device__ double gstlrcorrect(const DevParams &P, const WState &ws, int g, int lexisting,
                                double *delrc_mol) {
-  const double twopi = 2.0 * PI, volm = P.volm;
   int mol = P.locguest[g];
-  double delrc = 0.0;
+  double delrc = P.tl_c0[g][lexisting];
+  for (int gg = 1; gg <= P.ntpguest; ++gg) delrc += P.tl_c1[g][lexisting][gg] * (double)ws.nmols[P.locguest[gg]];
   for (int t = 0; t <= KMAXC; ++t) delrc_mol[t] = 0.0;
-  int ivdw = 0;
-  for (int i = 1; i <= P.ntypes; ++i)
-    for (int j = 1; j <= i; ++j) {
-      ++ivdw;
-      double eadd = P.steadd[ivdw];
-      double natyp = P.numtyp_g[g][i], nbtyp = P.numtyp_g[g][j];
-      double nafrz = P.numfrz_g[g][i], nbfrz = P.numfrz_g[g][j];
-      double denprd = (natyp * (double)ws.numtyp[j] + nbtyp * (double)ws.numtyp[i] -
-                       nafrz * (double)ws.numfrz[j] - nbfrz * (double)ws.numfrz[i]);
-      if (lexisting) denprd = -1.0 * denprd;
-      denprd = denprd + (natyp * nbtyp - nafrz * nbfrz);
-      delrc = delrc + volm * twopi * (denprd) / (volm * volm) * eadd;
-      for (int jt = 1; jt <= NS; ++jt) {
-        int kt = loc2(mol, jt);
-        double nctyp = ws.numtyp_mol[jt][j], ndtyp = ws.numtyp_mol[jt][i];
-        double ncfrz = ws.numfrz_mol[jt][j], ndfrz = ws.numfrz_mol[jt][i];
-        double sic = (natyp * nctyp + nbtyp * ndtyp - nafrz * ncfrz - nbfrz * ndfrz);
-        double self = 0.0;
-        if (jt == mol) {
-          self = (natyp * nbtyp - nafrz * nbfrz);
-          if (lexisting) sic = -1.0 * sic;
-        }
-        delrc_mol[kt] += volm * eadd * (twopi * (sic + self) / (volm * volm));
-      }
-    }
+#pragma unroll
+  for (int jt = 1; jt <= NS; ++jt)
+    delrc_mol[loc2(mol, jt)] = P.tm_c0[g][lexisting][jt] + P.tm_c1[g][lexisting][jt] * (double)ws.nmols[jt];
   return delrc;
 }
 
@@ -269,7 +250,7 @@ __device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS>
 
 // ----------------------------------------------------------- evaluation --
 template <int NS, typename T>
-__device__ inline void pair_site(const DevParams &P, const double *xs, double chgea, int ts, double xj,
+__device__ inline void pair_site(const DevParams &P, const double *erc, const double *xs, double chgea, int ts, double xj,
                                  double yj, double zj, double qj, int tj, int spj, double *acc_real,
                                  double *acc_vdw, int &ovl) {
   // distance, cutoff and overlap tests and the real-space Coulomb term stay
@@ -284,7 +265,7 @@ __device__ inline void pair_site(const DevParams &P, const double *xs, double ch
     double r = sqrt(rsq);
     double chgprd = chgea * qj;
     double e2 = 0.0, ev = 0.0;
-    if (fabs(chgprd) > 1e-10) e2 = interp3<double>(P.erc, r, P.rdrewd) * chgprd;
+    if (fabs(chgprd) > 1e-10) e2 = interp3<double>(erc, r, P.rdrewd) * chgprd;
     int k = P.pairidx[ts][tj];
     if (k >= 0) {
       double x = r * P.rdlrpot;
@@ -336,8 +317,38 @@ __device__ inline void site_phase(const double2 (*ph)[KCAP], int ll, int mm, int
 // with ewald2 (ewald.f:663-733), srfrce (vdw_terms.f:382-466) and ewald1_guest
 // (ewald.f:32-352).  Every thread of the workgroup takes part; the reduced
 // sums land in sh.tot.
+// Copies the read-only per-launch data into LDS: the erfc table always, the
+// walker's framework and the k-vector list when the plan made room for them.
+__device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double *erc = (double *)(lds + P.off_erc);
+  for (int i = tid; i < P.mxegrd + 3; i += nt) erc[i] = P.erc[i];
+  if (P.lds_fw) {
+    const double *gx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
+    const double *gy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
+    const double *gz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
+    double *fx = (double *)(lds + P.off_fw), *fy = fx + P.nfw, *fz = fy + P.nfw, *fq = fz + P.nfw;
+    int *fi = (int *)(fq + P.nfw);
+    for (int i = tid; i < P.nfw; i += nt) {
+      fx[i] = gx[i];
+      fy[i] = gy[i];
+      fz[i] = gz[i];
+      fq[i] = P.fwq[i];
+      fi[i] = P.fwinfo[i];
+    }
+  }
+  if (P.lds_k) {
+    double *ak = (double *)(lds + P.off_k);
+    int *kl = (int *)(ak + P.nk);
+    for (int i = tid; i < P.nk; i += nt) {
+      ak[i] = P.akk[i];
+      kl[i] = P.klmn[i];
+    }
+  }
+}
+
 template <int NS, typename T>
-__device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
+__device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds) {
   using A = Acc<NS>;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int natms = sh.natms, mol = sh.mol, g = sh.g;
@@ -365,20 +376,31 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
 #pragma unroll
   for (int i = 0; i < A::NACC; ++i) acc[i] = 0.0;
   int ovl_new = 0, ovl_old = 0;
+  const double *erc = (const double *)(lds + P.off_erc);
   __syncthreads();
-  // ---- pair terms: framework atoms
-  const double *fx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
-  const double *fy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
-  const double *fz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
-  for (int j = tid; j < P.nfw; j += nt) {
-    double xj = fx[j], yj = fy[j], zj = fz[j], qj = P.fwq[j];
-    int inf = P.fwinfo[j], tj = inf & 255, spj = (inf >> 8) & 255;
-    for (int s = 0; s < natms; ++s) {
-      if (sh.has_new)
-        pair_site<NS, T>(P, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc, acc + NS, ovl_new);
-      if (sh.has_old)
-        pair_site<NS, T>(P, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj, acc + 2 * NS, acc + 3 * NS, ovl_old);
+  // ---- pair terms: framework atoms (LDS when staged)
+  auto fw_loop = [&](const double *fx, const double *fy, const double *fz, const double *fq, const int *fi) {
+    for (int j = tid; j < P.nfw; j += nt) {
+      double xj = fx[j], yj = fy[j], zj = fz[j], qj = fq[j];
+      int inf = fi[j], tj = inf & 255, spj = (inf >> 8) & 255;
+      for (int s = 0; s < natms; ++s) {
+        if (sh.has_new)
+          pair_site<NS, T>(P, erc, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj,
+                           acc, acc + NS, ovl_new);
+        if (sh.has_old)
+          pair_site<NS, T>(P, erc, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj,
+                           acc + 2 * NS, acc + 3 * NS, ovl_old);
+      }
     }
+  };
+  if (P.lds_fw) {
+    const double *fx = (const double *)(lds + P.off_fw), *fy = fx + P.nfw, *fz = fy + P.nfw, *fq = fz + P.nfw;
+    fw_loop(fx, fy, fz, fq, (const int *)(fq + P.nfw));
+  } else {
+    const double *fx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
+    const double *fy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
+    const double *fz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
+    fw_loop(fx, fy, fz, P.fwq, P.fwinfo);
   }
   // ---- pair terms: guest atoms (own molecule excluded)
   for (int gg = 1; gg <= P.ntpguest; ++gg) {
@@ -393,20 +415,20 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
       int tj = P.site_t[molg][site];
       for (int s = 0; s < natms; ++s) {
         if (sh.has_new)
-          pair_site<NS, T>(P, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc, acc + NS, ovl_new);
+          pair_site<NS, T>(P, erc, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg,
+                           acc, acc + NS, ovl_new);
         if (sh.has_old)
-          pair_site<NS, T>(P, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg, acc + 2 * NS, acc + 3 * NS, ovl_old);
+          pair_site<NS, T>(P, erc, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg,
+                           acc + 2 * NS, acc + 3 * NS, ovl_old);
       }
     }
   }
   // ---- reciprocal terms: one k-vector per thread
   const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
   double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
-  auto recip_k = [&](int k, auto tag) {
+  auto recip_k = [&](int k, int lmn, double a, auto tag) {
     using U = decltype(tag);
-    int lmn = P.klmn[k];
     int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
-    double a = P.akk[k];
     double2 St = skw[(size_t)NS * P.nk + k];
     double2 Si[NS];
 #pragma unroll
@@ -462,29 +484,45 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
   };
   // the k list is sorted by a_k: the leading (large-weight) vectors always use
   // FP64 phase products, the tail uses T
-  for (int k = tid; k < P.nk64; k += nt) recip_k(k, 0.0);
-  for (int k = P.nk64 + tid; k < P.nk; k += nt) recip_k(k, (T)0);
-  // ---- workgroup reduction: wavefront butterfly, then across waves
+  auto k_loops = [&](const int *kl, const double *ak) {
+    for (int k = tid; k < P.nk64; k += nt) recip_k(k, kl[k], ak[k], 0.0);
+    for (int k = P.nk64 + tid; k < P.nk; k += nt) recip_k(k, kl[k], ak[k], (T)0);
+  };
+  if (P.lds_k) {
+    const double *ak = (const double *)(lds + P.off_k);
+    k_loops((const int *)(ak + P.nk), ak);
+  } else {
+    k_loops(P.klmn, P.akk);
+  }
+  // ---- workgroup reduction: DPP within each wave, then across waves; only
+  // the accumulators of the configurations present are reduced
   const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
 #pragma unroll
   for (int i = 0; i < A::NACC; ++i) {
-    double v = wave_sum(acc[i]);
-    if (lane == 0) sh.red[wave][i] = v;
+    bool is_new = (i < 2 * NS) || (i == 4 * NS) || (i >= 4 * NS + 2 && i < 5 * NS + 2);
+    if (is_new ? sh.has_new : sh.has_old) {
+      double v = wave_sum_dpp(acc[i]);
+      if (lane == 63) sh.red[wave][i] = v;
+    }
   }
   unsigned long long bn = __ballot(ovl_new != 0), bo = __ballot(ovl_old != 0);
-  if (tid == 0) {
-    sh.ovl_new = 0;
-    sh.ovl_old = 0;
+  if (lane == 0) {
+    sh.ovlw[wave][0] = bn != 0ULL;
+    sh.ovlw[wave][1] = bo != 0ULL;
   }
   __syncthreads();
-  if (lane == 0) {
-    if (bn) atomicOr(&sh.ovl_new, 1);
-    if (bo) atomicOr(&sh.ovl_old, 1);
-  }
   if (tid < A::NACC) {
     double v = 0.0;
     for (int wv = 0; wv < nwaves; ++wv) v += sh.red[wv][tid];
     sh.tot[tid] = v;
+  } else if (tid == A::NACC) {
+    int a = 0, b = 0;
+    for (int wv = 0; wv < nwaves; ++wv) {
+      a |= sh.ovlw[wv][0];
+      b |= sh.ovlw[wv][1];
+    }
+    sh.ovl_new = a;
+    sh.ovl_old = b;
   }
   __syncthreads();
 }
@@ -767,16 +805,25 @@ __device__ void record(const DevParams &P, const DevWalkers &W, Smem<NS> &sh, lo
 // ------------------------------------------------------------- kernels --
 // Device-resident chains: one workgroup per walker, nsteps GCMC steps.
 template <int NS, typename T>
-__global__ void __launch_bounds__(256) k_run(const DevParams *__restrict__ Pp, DevWalkers W, int nsteps,
-                                              int count_stats) {
-  __shared__ Smem<NS> sh;
+__global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, DevWalkers W, int nsteps,
+                                             int count_stats) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
   const int w = blockIdx.x;
   stage_in<NS>(W, w, sh);
+  stage_tables(P, W, w, lds);
+  __syncthreads();
   for (int step = 0; step < nsteps; ++step) {
-    if (threadIdx.x == 0) propose<NS>(P, W, w, sh);
+    if (threadIdx.x < 64) {
+      if (sh.ws.mz_made - sh.ws.mz_used < 33) {
+        mzt_refill(sh.ws, threadIdx.x);
+        if (threadIdx.x == 0) sh.ws.mz_made += 33;
+      }
+      if (threadIdx.x == 0) propose<NS>(P, W, w, sh);
+    }
     __syncthreads();
-    evaluate<NS, T>(P, W, w, sh);
+    evaluate<NS, T>(P, W, w, sh, lds);
     if (threadIdx.x == 0) {
       decide<NS>(P, sh, 1);
       WState &ws = sh.ws;
@@ -814,10 +861,12 @@ __global__ void __launch_bounds__(256) k_run(const DevParams *__restrict__ Pp, D
 // One trial of the mc_moves.f seam: the proposal comes from the pending
 // fields of walker w (written by the host); results go to W.trial_out.
 template <int NS, typename T>
-__global__ void __launch_bounds__(256) k_trial(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
-  __shared__ Smem<NS> sh;
+__global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
   stage_in<NS>(W, w, sh);
+  stage_tables(P, W, w, lds);
   if (threadIdx.x == 0) {
     WState &ws = sh.ws;
     int g = ws.pend_iguest, mol = P.locguest[g], natms = P.natoms[mol];
@@ -839,7 +888,7 @@ __global__ void __launch_bounds__(256) k_trial(const DevParams *__restrict__ Pp,
     }
   }
   __syncthreads();
-  evaluate<NS, T>(P, W, w, sh);
+  evaluate<NS, T>(P, W, w, sh, lds);
   if (threadIdx.x == 0) {
     decide<NS>(P, sh, 0);
     fmc_trial *o = W.trial_out;
@@ -863,8 +912,9 @@ __global__ void __launch_bounds__(256) k_trial(const DevParams *__restrict__ Pp,
 }
 
 template <int NS>
-__global__ void __launch_bounds__(256) k_accept(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
-  __shared__ Smem<NS> sh;
+__global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
   stage_in<NS>(W, w, sh);
   if (threadIdx.x == 0) {
