@@ -42,6 +42,7 @@ struct GraphKey {
 
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
+  int nsite = 0;                 // 3: every guest has at most 3 sites (straight-line pair code)
   DevParams hp;                  // host copy
   DevParams *dp = nullptr;       // device copy
   DevWalkers dw{};
@@ -159,7 +160,14 @@ double ewald3_template(const DevParams &P, int mol, int g, int natms) {
 
 template <int NS, typename T>
 void launch_run(fmc_handle *h, int nsteps, int count) {
-  hipLaunchKernelGGL((k_run<NS, T>), dim3(h->nwalkers), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, nsteps, count);
+  dim3 grid(h->nwalkers), block(h->block);
+  size_t lds = h->hp.lds_bytes;
+  if (!h->hp.lds_all)
+    hipLaunchKernelGGL((k_run<NS, T, false, 0>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+  else if (h->nsite == 3)
+    hipLaunchKernelGGL((k_run<NS, T, true, 3>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+  else
+    hipLaunchKernelGGL((k_run<NS, T, true, 0>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
 }
 
 size_t smem_size(int ns) {
@@ -172,11 +180,14 @@ size_t smem_size(int ns) {
 
 template <int NS, typename T>
 hipError_t allow_lds(int bytes) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_run<NS, T>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_trial<NS, T>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_accept<NS>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const void *fns[] = {(const void *)k_run<NS, T, false, 0>, (const void *)k_run<NS, T, true, 3>,
+                       (const void *)k_run<NS, T, true, 0>, (const void *)k_trial<NS, T>,
+                       (const void *)k_accept<NS>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t allow_lds_dispatch(int ns, int mixed, int bytes) {
@@ -609,6 +620,15 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   }
   h->selfq2_all_fw = fwq2;
   P.nfw = (int)fwx.size();
+  // decision constants: the same expressions as the per-trial code of
+  // ewald1_guest (ewald.f:96-105) and mc_insert (gcmc.f:1729)
+  P.dc_C = 2.0 * P.rvolm * R4PIE0 / P.epsq;
+  for (int g = 0; g <= KMAXS; ++g) P.dc_esic[g] = 0.0;
+  for (int g = 1; g <= ng; ++g) P.dc_esic[g] = -R4PIE0 / P.epsq * P.alpha * P.selfq2[g] / SQRPI;
+  h->nsite = 3;
+  for (int g = 1; g <= ng; ++g)
+    if (P.natoms[P.locguest[g]] > 3) h->nsite = 0;
+  if (getenv("FMC_NO_NSITE")) h->nsite = 0;
   // guest capacities: enough for a dense liquid in the cell
   int slot = 0;
   for (int g = 1; g <= ng; ++g) {
@@ -674,34 +694,79 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       }
     }
   }
-  // ---- LDS plan: Smem, erfc table, then the framework and the k list if they fit
+  // ---- LDS plan.  Always: Smem, erfc table, phase tables, constants block.
+  // The all-in-LDS variant also needs the framework, the k list, every guest
+  // slot and, in FP32 mode, the LJ columns of the guest-site types; without
+  // room for all of them (or with FMC_NO_LDS_ALL set) the kernels read those
+  // from global memory.
   {
-    const int budget = 160 * 1024 - 1024;
+    const size_t budget = 160 * 1024 - 1024;
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    P.nsmax = 1;
+    for (int g = 1; g <= ng; ++g) P.nsmax = std::max(P.nsmax, P.natoms[P.locguest[g]]);
+    P.kcp = std::max(P.kmax1, std::max(P.kmax2, P.kmax3)) + 1;
     size_t off = (smem_size(h->ns) + 255) & ~(size_t)255;
     P.off_erc = (int)off;
-    off += (size_t)8 * (mxegrd + 3);
-    off = (off + 15) & ~(size_t)15;
-    size_t fwb = (size_t)36 * P.nfw;
-    P.lds_fw = 0;
-    if (P.nfw > 0 && off + fwb <= (size_t)budget && !getenv("FMC_NO_LDS_FW")) {
-      P.lds_fw = 1;
-      P.off_fw = (int)off;
-      off += fwb;
-      off = (off + 15) & ~(size_t)15;
-    }
-    size_t kb = (size_t)12 * P.nk;
-    P.lds_k = 0;
-    if (off + kb <= (size_t)budget && !getenv("FMC_NO_LDS_K")) {
-      P.lds_k = 1;
-      P.off_k = (int)off;
-      off += kb;
-    }
-    P.lds_erc = 1;
-    P.lds_bytes = (int)((off + 15) & ~(size_t)15);
-    if (off > (size_t)budget) {
+    off = al(off + (size_t)8 * (mxegrd + 3));
+    P.off_ph = (int)off;
+    off = al(off + (size_t)16 * 2 * P.nsmax * 3 * P.kcp);
+    P.off_cst = (int)off;
+    off = al(off + (size_t)CST_BYTES);
+    if (off > budget) {
       delete h;
       return FMC_ERR_ARG;
     }
+    const size_t base_end = off;
+    // LJ columns used by guest sites (FP32 mode only)
+    int ncol = 0;
+    for (int ti = 0; ti <= KTYPE; ++ti)
+      for (int tj = 0; tj <= KTYPE; ++tj) P.lpairidx[ti][tj] = -1;
+    std::vector<int> colmap(npairs + 1, -1);
+    bool cols_ok = true;
+    for (int g = 1; g <= ng; ++g) {
+      int m = P.locguest[g];
+      for (int s2 = 0; s2 < P.natoms[m]; ++s2) {
+        int ti = P.site_t[m][s2];
+        for (int tj = 0; tj <= KTYPE; ++tj) {
+          int c = P.pairidx[ti][tj];
+          if (c < 0) continue;
+          if (colmap[c] < 0) {
+            if (ncol >= 64) {
+              cols_ok = false;
+              continue;
+            }
+            colmap[c] = ncol;
+            P.vcol_src[ncol++] = c;
+          }
+          P.lpairidx[ti][tj] = (int8_t)colmap[c];
+        }
+      }
+    }
+    P.nvcol = ncol;
+    P.lds_all = 0;
+    if (!getenv("FMC_NO_LDS_ALL") && cols_ok) {
+      size_t o = base_end;
+      int ofw = (int)o;
+      o = al(o + (size_t)36 * P.nfw);
+      int ok = (int)o;
+      o = al(o + (size_t)12 * P.nk);
+      int og = (int)o;
+      o = al(o + (size_t)24 * P.gatoms_total);
+      int ov = (int)o;
+      if (precision == FMC_MIXED) o = al(o + (size_t)4 * ncol * P.vstride);
+      if (o <= budget) {
+        P.lds_all = 1;
+        P.off_fw = ofw;
+        P.off_k = ok;
+        P.off_g = og;
+        P.off_vvv = ov;
+        off = o;
+      }
+    }
+    P.lds_bytes = (int)al(off);
+    if (getenv("FMC_VERBOSE"))
+      fprintf(stderr, "fastmc: LDS plan %d bytes, all-in-LDS %d, nsite %d, LJ columns %d, guest slots %d\n",
+              P.lds_bytes, P.lds_all, h->nsite, ncol, P.gatoms_total);
   }
   // ---- device buffers
   HIPCHK(hipStreamCreate(&h->stream));
@@ -815,6 +880,9 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       }
       s.mz_made = 0;
       s.mz_used = 0;
+      s.mz_ir = 97;
+      s.mz_jr = 33;
+      s.mz_c = 362436;
     }
     s.temp = wk.temperature;
     s.beta = 1.0 / (KBOLTZ * wk.temperature);

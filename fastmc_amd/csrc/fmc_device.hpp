@@ -34,9 +34,21 @@ struct DevParams {
   int imcon, maxmls, ntpguest, ntypes, nk, mxegrd, nfw, fw_per_walker;
   int kmax1, kmax2, kmax3, ntri;
   int nk64;                       // k-vectors [0, nk64) use FP64 phase products (list sorted by a_k)
-  // LDS staging plan (bytes from the start of the dynamic LDS block)
-  int lds_fw, lds_erc, lds_k;     // 1 if staged
-  int off_fw, off_erc, off_k, lds_bytes;
+  // LDS plan (bytes from the start of the dynamic LDS block): Smem, erfc
+  // table, per-site phase tables [2][nsmax][3][kcp] double2, constants block;
+  // with lds_all also the framework (x, y, z, q, info), the k list (a_k,
+  // packed l m n), the guest coordinates (x, y, z over all guest slots) and,
+  // in FP32 mode, the LJ columns of the guest-site types
+  int off_erc, off_ph, off_cst, lds_bytes;
+  int nsmax, kcp;
+  int lds_all;
+  int off_fw, off_k, off_g, off_vvv;
+  int nvcol;                      // staged LJ columns (FP32), stride vstride
+  int vcol_src[64];               // staged column -> table column
+  int8_t lpairidx[KTYPE + 1][KTYPE + 1];  // type pair -> staged column or -1
+  // constants of insertion / deletion bookkeeping (ewald.f:96-105,316-345)
+  double dc_C;                    // 2 rvolm r4pie0 / epsq
+  double dc_esic[KMAXS + 1];      // engsictmp of inserting guest g: -r4pie0/epsq alpha sum q^2 / sqrt(pi)
   // gstlrcorrect as a linear function of the guest counts (exact algebra of
   // vdw_terms.f:717-765): delrc(g, lexisting) = tl_c0 + sum_g' tl_c1[g'] N_g',
   // delrc_mol(loc2(mol, jt)) = tm_c0[jt] + tm_c1[jt] N_jt
@@ -82,6 +94,7 @@ struct WState {
   float u[98];
   float rq[128];
   long long mz_made, mz_used;
+  int mz_ir, mz_jr, mz_c, mz_pad;  // ir, jr of the next call and c (units of 2^-24) after mz_made calls
   double temp, beta;
   double fug[KMAXS + 1];
   float mcf[KMAXS + 1][FMC_NMOVES];
@@ -130,20 +143,19 @@ __device__ inline double duni(WState &w) {
 }
 
 // duni, utility_pack.f:2768-2857, 33 calls at once on lanes 0..32 of one
-// wave.  Call m (1-based) computes u(ir) - u(jr) with ir = 97 - (m-1) mod 97
-// and jr = ir of call m-33, so the 33 calls of a round read only values made
-// before the round; c after m calls is (362436 - m 7654321) mod 16777213 in
-// units of 2^-24.  All operations are exact in binary32.
+// wave.  Call m reads u(ir) - u(jr) where ir and jr step down by one per call
+// (wrapping 1 -> 97) and jr equals the ir of call m-33, so the 33 calls of a
+// round read only values made before the round; c steps by -7654321 modulo
+// 16777213 in units of 2^-24.  All operations are exact in binary32.  The
+// caller advances mz_ir, mz_jr, mz_c and mz_made after the round.
 __device__ inline void mzt_refill(WState &w, int lane) {
   if (lane < 33) {
-    long long m = w.mz_made + 1 + lane;
-    int ir = 97 - (int)((m - 1) % 97);
-    int jr = (int)(((32 - (m - 1)) % 97 + 97) % 97) + 1;
+    int ir = ((w.mz_ir - 1 - lane) % 97 + 97) % 97 + 1;
+    int jr = ((w.mz_jr - 1 - lane) % 97 + 97) % 97 + 1;
     float a = w.u[ir], b = w.u[jr];
     float uni = a - b;
     if (uni < 0.0f) uni = uni + 1.0f;
-    long long cm = 16777213LL, cd = 7654321LL;
-    long long cc = ((362436LL - (m % cm) * cd) % cm + cm) % cm;
+    int cc = ((w.mz_c - (lane + 1) * 7654321) % 16777213 + 16777213) % 16777213;
     float c = (float)cc * (1.0f / 16777216.0f);
     float out = uni - c;
     if (out < 0.0f) out = out + 1.0f;
@@ -151,6 +163,13 @@ __device__ inline void mzt_refill(WState &w, int lane) {
     w.u[ir] = uni;
     w.rq[(w.mz_made + lane) & 127] = out;
   }
+}
+
+__device__ inline void mzt_advance(WState &w) {
+  w.mz_ir = ((w.mz_ir - 1 - 33) % 97 + 97) % 97 + 1;
+  w.mz_jr = ((w.mz_jr - 1 - 33) % 97 + 97) % 97 + 1;
+  w.mz_c = ((w.mz_c - 33 * 7654321) % 16777213 + 16777213) % 16777213;
+  w.mz_made += 33;
 }
 
 // images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half away)

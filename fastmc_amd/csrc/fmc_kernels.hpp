@@ -22,14 +22,55 @@ struct Smem {
   int kind, g, mol, natms, choice, nmols, has_new, has_old, excl_lo, excl_hi, nfw;
   int accepted, drew, ovl_new, ovl_old;
   double xn[KSITE][3], xo[KSITE][3];
-  double2 ph[2][KSITE][3][KCAP];
   double red[KWAVES][Acc<NS>::NACC];
   double tot[Acc<NS>::NACC];
   int ovlw[KWAVES][2];
   double estep, rande, delrc;
   double delE[KMAXS + 1], delrc_mol[KMAXC + 1];
   double comp[4];
+  // constants of the current trial, computed while the energy terms run
+  double qfix0, pref;
+  double qfm[KMAXS + 1];
 };
+
+constexpr int KPT = 2;  // k-vectors per thread held in registers for a whole launch
+
+// Register-resident share of the walker's structure factor: slot m of a
+// thread is k = kslot(m).
+template <int NS>
+struct KSlot {
+  double2 S[NS + 1];  // species rows, then the total
+  double2 sn, so;     // s_new, s_old of the current trial
+};
+
+// k of register slot m (reverse thread order: the low threads hold the extra
+// atoms of the pair loop, the high threads the extra k-vectors)
+__device__ inline int kslot(int m) { return (int)(blockDim.x - 1 - threadIdx.x) + m * (int)blockDim.x; }
+
+template <int NS>
+__device__ inline void load_slots(const DevParams &P, const DevWalkers &W, int w, KSlot<NS> *ks) {
+  const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    int k = kslot(m);
+#pragma unroll
+    for (int i = 0; i <= NS; ++i) ks[m].S[i] = k < P.nk ? skw[(size_t)i * P.nk + k] : make_double2(0.0, 0.0);
+    ks[m].sn = ks[m].so = make_double2(0.0, 0.0);
+  }
+}
+
+template <int NS>
+__device__ inline void store_slots(const DevParams &P, const DevWalkers &W, int w, const KSlot<NS> *ks) {
+  double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    int k = kslot(m);
+    if (k < P.nk) {
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = ks[m].S[i];
+    }
+  }
+}
 
 // ------------------------------------------------------------ tail term --
 // gstlrcorrect, vdw_terms.f:686-776, in its linear form (coefficients built on
@@ -93,22 +134,42 @@ __device__ inline void rotate_about(double (*x)[3], int natms, const double *c, 
 __device__ inline void quat_from(double r1, double r2, double r3, double r4, double scale, double *q) {
   double theta = r1 * scale;
   double b1 = 2.0 * r2 - 1.0, b2 = 2.0 * r3 - 1.0, b3 = 2.0 * r4 - 1.0;
-  double st = sin(theta / 2.0);
+  double st, ct;
+  sincos(theta / 2.0, &st, &ct);
   double norm = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
-  q[0] = cos(theta / 2.0);
+  q[0] = ct;
   q[1] = st * (b1 / norm);
   q[2] = st * (b2 / norm);
   q[3] = st * (b3 / norm);
 }
 
-template <int NS>
-__device__ inline void load_molecule(const DevParams &P, const DevWalkers &W, int w, int g, int choice,
+// Guest coordinates of the walker: global arrays, or their LDS copy (kept in
+// step with the global arrays by apply) when LF is set.
+template <bool LF>
+struct GuestXYZ {
+  const double *x, *y, *z;  // slot 0 of guest 1
+  __device__ GuestXYZ(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
+    if constexpr (LF) {
+      x = (const double *)(lds + P.off_g);
+      y = x + P.gatoms_total;
+      z = y + P.gatoms_total;
+    } else {
+      size_t b = (size_t)w * P.gatoms_total;
+      x = W.gx + b;
+      y = W.gy + b;
+      z = W.gz + b;
+    }
+  }
+};
+
+template <bool LF>
+__device__ inline void load_molecule(const DevParams &P, const GuestXYZ<LF> &G, int g, int choice,
                                      double (*x)[3], int natms) {
-  size_t base = (size_t)w * P.gatoms_total + P.gbase[g] + (size_t)(choice - 1) * natms;
+  int base = P.gbase[g] + (choice - 1) * natms;
   for (int i = 0; i < natms; ++i) {
-    x[i][0] = W.gx[base + i];
-    x[i][1] = W.gy[base + i];
-    x[i][2] = W.gz[base + i];
+    x[i][0] = G.x[base + i];
+    x[i][1] = G.y[base + i];
+    x[i][2] = G.z[base + i];
   }
 }
 
@@ -117,8 +178,8 @@ __device__ inline void load_molecule(const DevParams &P, const DevWalkers &W, in
 // trial geometry of mc_insert / mc_delete / mc_displace (gcmc.f:1693-1935)
 // with random_ins, random_disp, random_rot, translate and rotation
 // (utility_pack.f:3161-3563).  Thread 0 only.
-template <int NS>
-__device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
+template <int NS, bool LF>
+__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh) {
   WState &ws = sh.ws;
   int g = 1;
   if (P.ntpguest > 1) g = (int)floor(duni(ws) * P.ntpguest) + 1;
@@ -185,7 +246,7 @@ __device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS>
   } else if (kind == FMC_MOVE_DELETE) {
     int choice = (int)floor(duni(ws) * nmols) + 1;
     sh.choice = choice;
-    load_molecule<NS>(P, W, w, g, choice, sh.xo, natms);
+    load_molecule<LF>(P, G, g, choice, sh.xo, natms);
     sh.has_new = 0;
     sh.has_old = 1;
     sh.excl_lo = (choice - 1) * natms;
@@ -214,7 +275,7 @@ __device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS>
     }
     int choice = (int)floor(duni(ws) * nmols) + 1;
     sh.choice = choice;
-    load_molecule<NS>(P, W, w, g, choice, sh.xo, natms);
+    load_molecule<LF>(P, G, g, choice, sh.xo, natms);
     // translate, utility_pack.f:3161-3218
     for (int i = 0; i < natms; ++i) {
       sh.xn[i][0] = a + sh.xo[i][0];
@@ -249,53 +310,69 @@ __device__ void propose(const DevParams &P, const DevWalkers &W, int w, Smem<NS>
 }
 
 // ----------------------------------------------------------- evaluation --
-template <int NS, typename T>
-__device__ inline void pair_site(const DevParams &P, const double *erc, const double *xs, double chgea, int ts, double xj,
-                                 double yj, double zj, double qj, int tj, int spj, double *acc_real,
-                                 double *acc_vdw, int &ovl) {
-  // distance, cutoff and overlap tests and the real-space Coulomb term stay
-  // in FP64 in both modes (pair Coulomb terms are large and cancel); T only
-  // sets the precision of the LJ table interpolation, whose fractional
-  // position is formed in FP64 first
+// One guest-site / atom pair of insertion / guest_energy / deletion: minimum
+// image, overlap test (surface_check, mc_moves.f:40-42), ewald2 and srfrce
+// terms (ewald.f:689-726, vdw_terms.f:418-460).  Branch-free, so that the
+// pairs of one atom with all trial sites form one straight-line block:
+// table indices are clamped and out-of-range terms selected away.  Geometry
+// and the real-space Coulomb term are FP64 in both modes (pair Coulomb terms
+// are large and cancel); T is the precision of the LJ interpolation, whose
+// fractional grid position is formed in FP64 first.  col < 0: no LJ term;
+// ovl2 < 0: a padding site (never overlaps, chgea 0).
+template <typename T>
+__device__ inline void pair_eval(const DevParams &P, const double *erc, const T *vvv, const double *xs, double chgea,
+                                 int col, double ovl2, double xj, double yj, double zj, double qj, double &e2,
+                                 double &ev, int &ovl) {
   double dx = xs[0] - xj, dy = xs[1] - yj, dz = xs[2] - zj;
   min_image(P, dx, dy, dz);
   double rsq = dx * dx + dy * dy + dz * dz;
-  if (rsq < P.overlap2) ovl = 1;
-  if (rsq < P.rcut2) {
-    double r = sqrt(rsq);
-    double chgprd = chgea * qj;
-    double e2 = 0.0, ev = 0.0;
-    if (fabs(chgprd) > 1e-10) e2 = interp3<double>(erc, r, P.rdrewd) * chgprd;
-    int k = P.pairidx[ts][tj];
-    if (k >= 0) {
-      double x = r * P.rdlrpot;
-      int l = (int)x;
-      T pp = (T)(x - (double)l);
-      const T *vv;
-      if constexpr (sizeof(T) == 8) vv = (const T *)P.vvv; else vv = (const T *)P.vvv32;
-      vv += (size_t)k * P.vstride + l;
-      T v0 = vv[0], v1 = vv[1], v2 = vv[2];
-      T t1 = v0 + (v1 - v0) * pp;
-      T t2 = v1 + (v2 - v1) * (pp - (T)1);
-      ev = (double)(t1 + (t2 - t1) * pp * (T)0.5);
-    }
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-      if (spj == i + 1) {
-        acc_real[i] += e2;
-        acc_vdw[i] += ev;
-      }
-  }
+  ovl |= rsq < ovl2;
+  bool in = rsq < P.rcut2;
+  double r = sqrt(rsq);
+  const int lmax = P.mxegrd;
+  double x = r * P.rdrewd;
+  int l = min((int)x, lmax);
+  double p = x - (double)l;
+  double v0 = erc[l], v1 = erc[l + 1], v2 = erc[l + 2];
+  double t1 = v0 + (v1 - v0) * p;
+  double t2 = v1 + (v2 - v1) * (p - 1.0);
+  double chgprd = chgea * qj;
+  double ec = (t1 + (t2 - t1) * p * 0.5) * chgprd;
+  e2 = (in && fabs(chgprd) > 1e-10) ? ec : 0.0;
+  double xv = r * P.rdlrpot;
+  int lv = min((int)xv, lmax);
+  T pp = (T)(xv - (double)lv);
+  const T *vv = vvv + (size_t)(col < 0 ? 0 : col) * P.vstride + lv;
+  T w0 = vv[0], w1 = vv[1], w2 = vv[2];
+  T u1 = w0 + (w1 - w0) * pp;
+  T u2 = w1 + (w2 - w1) * (pp - (T)1);
+  double el = (double)(u1 + (u2 - u1) * pp * (T)0.5);
+  ev = (in && col >= 0) ? el : 0.0;
 }
 
-// One reciprocal-space phase product for site s of configuration cfg at
-// (ll, mm, nn): the reference's clm/slm then ckc/cks (ewald.f:191-251).
+// LDS block of per-launch constants (offsets from P.off_cst):
+//   ltab [KMAXS+1][KSITE][KTYPE+1] int8: guest g, site s, atom type -> LJ column or -1
+//   lsq  [KMAXS+1][KSITE] double: site charge by species
+//   lst  [KMAXS+1][KSITE] int: site type by species
+constexpr int CST_LTAB = 0;
+constexpr int CST_LSQ = 1008;
+constexpr int CST_LST = CST_LSQ + 8 * (KMAXS + 1) * KSITE;
+constexpr int CST_BYTES = CST_LST + 4 * (KMAXS + 1) * KSITE;
+static_assert((KMAXS + 1) * KSITE * (KTYPE + 1) <= CST_LSQ, "ltab overlaps lsq");
+
+__device__ inline double2 *ph_site(unsigned char *lds, const DevParams &P, int cfg, int s) {
+  return (double2 *)(lds + P.off_ph) + (size_t)((cfg * P.nsmax + s) * 3) * P.kcp;
+}
+
+// One reciprocal-space phase product for one site at (ll, mm, nn): the
+// reference's clm/slm then ckc/cks (ewald.f:191-251).  ph holds the site's
+// three axis rows at stride kcp.
 template <typename T>
-__device__ inline void site_phase(const double2 (*ph)[KCAP], int ll, int mm, int nn, T q, T &ckc, T &cks) {
+__device__ inline void site_phase(const double2 *ph, int kcp, int ll, int mm, int nn, T q, T &ckc, T &cks) {
   int m = mm < 0 ? -mm : mm, n = nn < 0 ? -nn : nn;
-  T lc = (T)ph[0][ll].x, ls = (T)ph[0][ll].y;
-  T mc = (T)ph[1][m].x, ms = (T)ph[1][m].y;
-  T nc = (T)ph[2][n].x, ns = (T)ph[2][n].y;
+  T lc = (T)ph[ll].x, ls = (T)ph[ll].y;
+  T mc = (T)ph[kcp + m].x, ms = (T)ph[kcp + m].y;
+  T nc = (T)ph[2 * kcp + n].x, ns = (T)ph[2 * kcp + n].y;
   T clm, slm;
   if (mm >= 0) {
     clm = lc * mc - ls * ms;
@@ -313,17 +390,38 @@ __device__ inline void site_phase(const double2 (*ph)[KCAP], int ll, int mm, int
   }
 }
 
-// Trial energy terms of insertion / deletion / guest_energy (mc_moves.f:150-484)
-// with ewald2 (ewald.f:663-733), srfrce (vdw_terms.f:382-466) and ewald1_guest
-// (ewald.f:32-352).  Every thread of the workgroup takes part; the reduced
-// sums land in sh.tot.
-// Copies the read-only per-launch data into LDS: the erfc table always, the
-// walker's framework and the k-vector list when the plan made room for them.
+// Copies the read-only per-launch data into LDS: the erfc table and the
+// constants block always; with LF (the plan made room for everything) also
+// the walker's framework, the k list, the LJ columns of the guest sites (in
+// the precision T of the mode, FP32 only) and the guest coordinates.
+template <typename T, bool LF>
 __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
   const int tid = threadIdx.x, nt = blockDim.x;
   double *erc = (double *)(lds + P.off_erc);
   for (int i = tid; i < P.mxegrd + 3; i += nt) erc[i] = P.erc[i];
-  if (P.lds_fw) {
+  {
+    int8_t *lt = (int8_t *)(lds + P.off_cst + CST_LTAB);
+    double *lsq = (double *)(lds + P.off_cst + CST_LSQ);
+    int *lst = (int *)(lds + P.off_cst + CST_LST);
+    const int per = KSITE * (KTYPE + 1);
+    for (int i = tid; i < (KMAXS + 1) * per; i += nt) {
+      int g = i / per, s = (i % per) / (KTYPE + 1), tj = i % (KTYPE + 1);
+      int col = -1;
+      if (g >= 1 && g <= P.ntpguest) {
+        int mol = P.locguest[g];
+        if (s < P.natoms[mol]) {
+          int ts = P.site_t[mol][s];
+          col = (LF && sizeof(T) == 4) ? P.lpairidx[ts][tj] : P.pairidx[ts][tj];
+        }
+      }
+      lt[i] = (int8_t)col;
+    }
+    for (int i = tid; i < (KMAXS + 1) * KSITE; i += nt) {
+      lsq[i] = P.site_q[i / KSITE][i % KSITE];
+      lst[i] = P.site_t[i / KSITE][i % KSITE];
+    }
+  }
+  if constexpr (LF) {
     const double *gx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
     const double *gy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
     const double *gz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
@@ -336,31 +434,88 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
       fq[i] = P.fwq[i];
       fi[i] = P.fwinfo[i];
     }
-  }
-  if (P.lds_k) {
     double *ak = (double *)(lds + P.off_k);
     int *kl = (int *)(ak + P.nk);
     for (int i = tid; i < P.nk; i += nt) {
       ak[i] = P.akk[i];
       kl[i] = P.klmn[i];
     }
+    if constexpr (sizeof(T) == 4) {
+      float *dst = (float *)(lds + P.off_vvv);
+      const int n = P.nvcol * P.vstride;
+      for (int i = tid; i < n; i += nt) {
+        int c = i / P.vstride, r = i - c * P.vstride;
+        dst[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + r];
+      }
+    }
+    double *lx = (double *)(lds + P.off_g), *ly = lx + P.gatoms_total, *lz = ly + P.gatoms_total;
+    const size_t b = (size_t)w * P.gatoms_total;
+    for (int gg = 1; gg <= P.ntpguest; ++gg) {
+      int mol = P.locguest[gg];
+      int n = W.ws[w].nmols[mol] * P.natoms[mol], o = P.gbase[gg];
+      for (int i = tid; i < n; i += nt) {
+        lx[o + i] = W.gx[b + o + i];
+        ly[o + i] = W.gy[b + o + i];
+        lz[o + i] = W.gz[b + o + i];
+      }
+    }
   }
 }
 
-template <int NS, typename T>
-__device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds) {
+// Trial constants that depend only on the proposal: the tail correction
+// (gstlrcorrect), the charged-system terms of ewald1_guest (ewald.f:96-105)
+// and the acceptance prefactor of energy_eval (utility_pack.f:3655-3676).
+// One thread, overlapped with the energy terms.
+template <int NS>
+__device__ void prepare(const DevParams &P, Smem<NS> &sh) {
+  const WState &ws = sh.ws;
+  const int g = sh.g, mol = sh.mol, kind = sh.kind;
+  const double alpha = P.alpha, volm = P.volm;
+  const double chg = P.chgtmp[g];
+  sh.qfix0 = -(0.5 * PI * R4PIE0 / P.epsq) * ((chg * ws.sumchg) / alpha + (chg / alpha) * (chg / alpha)) / volm;
+  for (int i = 1; i <= NS; ++i) {
+    double fkk = (i == mol) ? 0.5 : 1.0;
+    sh.qfm[i] = -(fkk * PI * R4PIE0 / P.epsq) * (ws.chgsum_mol[i] * chg) / (alpha * alpha * volm);
+  }
+  sh.delrc = 0.0;
+  if (kind == FMC_MOVE_INSERT) sh.delrc = gstlrcorrect<NS>(P, ws, g, 0, sh.delrc_mol);
+  else if (kind == FMC_MOVE_DELETE) sh.delrc = gstlrcorrect<NS>(P, ws, g, 1, sh.delrc_mol);
+  const int ingsts = ws.nmols[mol];
+  const double ipress = ws.fug[g];
+  if (kind == FMC_MOVE_INSERT) sh.pref = P.statvolm * ipress / (((double)ingsts + 1.0) * BOLTZ * ws.temp);
+  else if (kind == FMC_MOVE_DELETE) sh.pref = (double)ingsts * BOLTZ * ws.temp / (P.statvolm * ipress);
+  else sh.pref = 1.0;
+}
+
+// Trial energy terms of insertion / deletion / guest_energy (mc_moves.f:150-484)
+// with ewald2 (ewald.f:663-733), srfrce (vdw_terms.f:382-466) and ewald1_guest
+// (ewald.f:32-352).  Every thread of the workgroup takes part: atoms of the
+// pair terms from the bottom thread up, k-vectors from the top down (register
+// slots ks, then any k beyond KPT per thread through global memory); the
+// reduced sums land in sh.tot.  LF: framework, k list, guests (and in FP32
+// the LJ columns) are read from LDS.  NSITE > 0: guests have at most NSITE
+// sites (padding sites masked) and the site loops are straight-line code.
+// seam: also write s_new / s_old of the register slots to the scratch rows.
+template <int NS, typename T, bool LF, int NSITE>
+__device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
+                         KSlot<NS> *ks, bool seam) {
   using A = Acc<NS>;
+  constexpr bool LV = LF && sizeof(T) == 4;
+  constexpr int NSR = NSITE > 0 ? NSITE : KSITE;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int natms = sh.natms, mol = sh.mol, g = sh.g;
+  const int has_new = sh.has_new, has_old = sh.has_old;
+  if (tid == nt - 1) prepare<NS>(P, sh);
   // per-site 1-D phase tables, built by the reference recursion (ewald.f:108-147)
-  for (int t = tid; t < 2 * KSITE * 3; t += nt) {
-    int cfg = t / (KSITE * 3), rem = t % (KSITE * 3), s = rem / 3, ax = rem % 3;
-    if (s >= natms || (cfg == 0 && !sh.has_new) || (cfg == 1 && !sh.has_old)) continue;
+  for (int t = tid; t < 2 * P.nsmax * 3; t += nt) {
+    int cfg = t / (P.nsmax * 3), rem = t % (P.nsmax * 3), s = rem / 3, ax = rem % 3;
+    if (s >= natms || (cfg == 0 && !has_new) || (cfg == 1 && !has_old)) continue;
     const double *x = cfg == 0 ? sh.xn[s] : sh.xo[s];
     double fr = P.rcell[ax] * x[0] + P.rcell[3 + ax] * x[1] + P.rcell[6 + ax] * x[2];
     int kmax = ax == 0 ? P.kmax1 : (ax == 1 ? P.kmax2 : P.kmax3);
-    double c1 = cos(2.0 * PI * fr), s1 = sin(2.0 * PI * fr);
-    double2 *row = sh.ph[cfg][s][ax];
+    double s1, c1;
+    sincospi(2.0 * fr, &s1, &c1);
+    double2 *row = ph_site(lds, P, cfg, s) + ax * P.kcp;
     row[0] = make_double2(1.0, 0.0);
     row[1] = make_double2(c1, s1);
     double c = c1, sn = s1;
@@ -377,68 +532,148 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
   for (int i = 0; i < A::NACC; ++i) acc[i] = 0.0;
   int ovl_new = 0, ovl_old = 0;
   const double *erc = (const double *)(lds + P.off_erc);
-  __syncthreads();
-  // ---- pair terms: framework atoms (LDS when staged)
-  auto fw_loop = [&](const double *fx, const double *fy, const double *fz, const double *fq, const int *fi) {
-    for (int j = tid; j < P.nfw; j += nt) {
-      double xj = fx[j], yj = fy[j], zj = fz[j], qj = fq[j];
-      int inf = fi[j], tj = inf & 255, spj = (inf >> 8) & 255;
-      for (int s = 0; s < natms; ++s) {
-        if (sh.has_new)
-          pair_site<NS, T>(P, erc, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj,
-                           acc, acc + NS, ovl_new);
-        if (sh.has_old)
-          pair_site<NS, T>(P, erc, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, spj,
-                           acc + 2 * NS, acc + 3 * NS, ovl_old);
-      }
-    }
-  };
-  if (P.lds_fw) {
-    const double *fx = (const double *)(lds + P.off_fw), *fy = fx + P.nfw, *fz = fy + P.nfw, *fq = fz + P.nfw;
-    fw_loop(fx, fy, fz, fq, (const int *)(fq + P.nfw));
-  } else {
-    const double *fx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
-    const double *fy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
-    const double *fz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
-    fw_loop(fx, fy, fz, P.fwq, P.fwinfo);
-  }
-  // ---- pair terms: guest atoms (own molecule excluded)
-  for (int gg = 1; gg <= P.ntpguest; ++gg) {
-    int molg = P.locguest[gg], na = P.natoms[molg];
-    int nact = sh.ws.nmols[molg] * na;
-    size_t base = (size_t)w * P.gatoms_total + P.gbase[gg];
-    for (int j = tid; j < nact; j += nt) {
-      if (gg == g && j >= sh.excl_lo && j < sh.excl_hi) continue;
-      int site = j % na;
-      double xj = W.gx[base + j], yj = W.gy[base + j], zj = W.gz[base + j];
-      double qj = P.site_q[molg][site];
-      int tj = P.site_t[molg][site];
-      for (int s = 0; s < natms; ++s) {
-        if (sh.has_new)
-          pair_site<NS, T>(P, erc, sh.xn[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg,
-                           acc, acc + NS, ovl_new);
-        if (sh.has_old)
-          pair_site<NS, T>(P, erc, sh.xo[s], P.site_chgea[mol][s], P.site_t[mol][s], xj, yj, zj, qj, tj, molg,
-                           acc + 2 * NS, acc + 3 * NS, ovl_old);
-      }
-    }
-  }
-  // ---- reciprocal terms: one k-vector per thread
-  const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
-  double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
-  auto recip_k = [&](int k, int lmn, double a, auto tag) {
-    using U = decltype(tag);
-    int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
-    double2 St = skw[(size_t)NS * P.nk + k];
-    double2 Si[NS];
+  const int8_t *lt = (const int8_t *)(lds + P.off_cst + CST_LTAB) + g * KSITE * (KTYPE + 1);
+  const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
+  const int *lst = (const int *)(lds + P.off_cst + CST_LST);
+  const T *vvv;
+  if constexpr (LV) vvv = (const T *)(lds + P.off_vvv);
+  else if constexpr (sizeof(T) == 8) vvv = (const T *)P.vvv;
+  else vvv = (const T *)P.vvv32;
+  // NSITE > 0: trial sites in registers; padding sites (s >= natms) carry
+  // charge 0, no LJ column and no overlap test.  NSITE == 0: read from LDS.
+  constexpr int NSA = NSITE > 0 ? NSITE : 1;
+  double xs[2][NSA][3], cq[NSA], ov2[NSA], qs[NSA];
+  if constexpr (NSITE > 0) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i) Si[i] = skw[(size_t)i * P.nk + k];
+    for (int s = 0; s < NSITE; ++s) {
+      bool real = s < natms;
+      cq[s] = real ? P.site_chgea[mol][s] : 0.0;
+      ov2[s] = real ? P.overlap2 : -1.0;
+      qs[s] = real ? P.site_q[mol][s] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        xs[0][s][d] = real ? sh.xn[s][d] : 0.0;
+        xs[1][s][d] = real ? sh.xo[s][d] : 0.0;
+      }
+    }
+  }
+  // atom index space: framework atoms, then the guests of each kind
+  int nact[KMAXS + 1];
+  int ntot = P.nfw;
+#pragma unroll
+  for (int gg = 1; gg <= KMAXS; ++gg) {
+    nact[gg] = 0;
+    if (gg <= P.ntpguest) {
+      int molg = P.locguest[gg];
+      nact[gg] = sh.ws.nmols[molg] * P.natoms[molg];
+      ntot += nact[gg];
+    }
+  }
+  const GuestXYZ<LF> G(P, W, w, lds);
+  const double *fx, *fy, *fz, *fq;
+  const int *fi;
+  if constexpr (LF) {
+    fx = (const double *)(lds + P.off_fw);
+    fy = fx + P.nfw;
+    fz = fy + P.nfw;
+    fq = fz + P.nfw;
+    fi = (const int *)(fq + P.nfw);
+  } else {
+    fx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
+    fy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
+    fz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
+    fq = P.fwq;
+    fi = P.fwinfo;
+  }
+  __syncthreads();  // phase tables written
+  // ---- pair terms
+  for (int j = tid; j < ntot; j += nt) {
+    double xj, yj, zj, qj;
+    int tj, spj;
+    bool skip = false;
+    if (j < P.nfw) {
+      xj = fx[j];
+      yj = fy[j];
+      zj = fz[j];
+      qj = fq[j];
+      int inf = fi[j];
+      tj = inf & 255;
+      spj = (inf >> 8) & 255;
+    } else {
+      int jj = j - P.nfw, gg = 1;
+      while (gg < P.ntpguest && jj >= nact[gg]) {
+        jj -= nact[gg];
+        ++gg;
+      }
+      int molg = P.locguest[gg], na = P.natoms[molg];
+      int site = jj % na;
+      int a = P.gbase[gg] + jj;
+      xj = G.x[a];
+      yj = G.y[a];
+      zj = G.z[a];
+      qj = lsq[molg * KSITE + site];
+      tj = lst[molg * KSITE + site];
+      spj = molg;
+      skip = gg == g && jj >= sh.excl_lo && jj < sh.excl_hi;
+    }
+    if (skip) continue;
+    int col[NSR];
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) {
+      if (NSITE == 0 && s >= natms) break;
+      col[s] = lt[s * (KTYPE + 1) + tj];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (c == 0 ? !has_new : !has_old) continue;
+      double er = 0.0, evd = 0.0;
+      int ov = 0;
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) {
+        if (NSITE == 0 && s >= natms) break;
+        double e2, ev;
+        if constexpr (NSITE > 0)
+          pair_eval<T>(P, erc, vvv, xs[c][s], cq[s], col[s], ov2[s], xj, yj, zj, qj, e2, ev, ov);
+        else
+          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], P.site_chgea[mol][s], col[s], P.overlap2, xj, yj,
+                       zj, qj, e2, ev, ov);
+        er += e2;
+        evd += ev;
+      }
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        acc[2 * c * NS + i] += (spj == i + 1) ? er : 0.0;
+        acc[(2 * c + 1) * NS + i] += (spj == i + 1) ? evd : 0.0;
+      }
+      if (c == 0) ovl_new |= ov;
+      else ovl_old |= ov;
+    }
+  }
+  // ---- reciprocal terms
+  const int *kl;
+  const double *ak;
+  if constexpr (LF) {
+    ak = (const double *)(lds + P.off_k);
+    kl = (const int *)(ak + P.nk);
+  } else {
+    ak = P.akk;
+    kl = P.klmn;
+  }
+  double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
+  auto recip_k = [&](int k, const double2 *S, double2 &sn_out, double2 &so_out, auto tag) {
+    using U = decltype(tag);
+    const int lmn = kl[k];
+    const double a = ak[k];
+    const int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
+    const double2 St = S[NS];
     double soc = 0.0, sos = 0.0;
-    if (sh.has_old) {
+    if (has_old) {
       U c = (U)0, s_ = (U)0;
-      for (int s = 0; s < natms; ++s) {
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) {
+        if (s >= natms) break;
         U ckc, cks;
-        site_phase<U>(sh.ph[1][s], ll, mm, nn, (U)P.site_q[mol][s], ckc, cks);
+        site_phase<U>(ph_site(lds, P, 1, s), P.kcp, ll, mm, nn, (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
         c += ckc;
         s_ += cks;
       }
@@ -449,21 +684,23 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
 #pragma unroll
       for (int i = 0; i < NS; ++i) {
         double fkk = (i + 1 == mol) ? a : 2.0 * a;
-        acc[5 * NS + 2 + i] += fkk * (soc * Si[i].x + sos * Si[i].y);
+        acc[5 * NS + 2 + i] += fkk * (soc * S[i].x + sos * S[i].y);
       }
-      scr[P.nk + k] = make_double2(soc, sos);
+      so_out = make_double2(soc, sos);
     }
-    if (sh.has_new) {
+    if (has_new) {
       U c = (U)0, s_ = (U)0;
-      for (int s = 0; s < natms; ++s) {
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) {
+        if (s >= natms) break;
         U ckc, cks;
-        site_phase<U>(sh.ph[0][s], ll, mm, nn, (U)P.site_q[mol][s], ckc, cks);
+        site_phase<U>(ph_site(lds, P, 0, s), P.kcp, ll, mm, nn, (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
         c += ckc;
         s_ += cks;
       }
       double snc = (double)c, sns = (double)s_;
       double Stx = St.x, Sty = St.y;
-      if (sh.has_old) {
+      if (has_old) {
         Stx = Stx - soc;
         Sty = Sty - sos;
       }
@@ -472,27 +709,39 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
 #pragma unroll
       for (int i = 0; i < NS; ++i) {
         double fkk = (i + 1 == mol) ? a : 2.0 * a;
-        double sx = Si[i].x, sy = Si[i].y;
-        if (sh.has_old && i + 1 == mol) {
+        double sx = S[i].x, sy = S[i].y;
+        if (has_old && i + 1 == mol) {
           sx = sx - soc;
           sy = sy - sos;
         }
         acc[4 * NS + 2 + i] += fkk * (snc * sx + sns * sy);
       }
-      scr[k] = make_double2(snc, sns);
+      sn_out = make_double2(snc, sns);
     }
   };
-  // the k list is sorted by a_k: the leading (large-weight) vectors always use
-  // FP64 phase products, the tail uses T
-  auto k_loops = [&](const int *kl, const double *ak) {
-    for (int k = tid; k < P.nk64; k += nt) recip_k(k, kl[k], ak[k], 0.0);
-    for (int k = P.nk64 + tid; k < P.nk; k += nt) recip_k(k, kl[k], ak[k], (T)0);
-  };
-  if (P.lds_k) {
-    const double *ak = (const double *)(lds + P.off_k);
-    k_loops((const int *)(ak + P.nk), ak);
-  } else {
-    k_loops(P.klmn, P.akk);
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    const int k = kslot(m);
+    if (k < P.nk) {
+      if (k < P.nk64) recip_k(k, ks[m].S, ks[m].sn, ks[m].so, 0.0);
+      else recip_k(k, ks[m].S, ks[m].sn, ks[m].so, (T)0);
+      if (seam) {
+        scr[k] = ks[m].sn;
+        scr[P.nk + k] = ks[m].so;
+      }
+    }
+  }
+  if (P.nk > KPT * nt) {
+    const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+    for (int k = KPT * nt + tid; k < P.nk; k += nt) {
+      double2 S[NS + 1], sn = make_double2(0.0, 0.0), so = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
+      if (k < P.nk64) recip_k(k, S, sn, so, 0.0);
+      else recip_k(k, S, sn, so, (T)0);
+      scr[k] = sn;
+      scr[P.nk + k] = so;
+    }
   }
   // ---- workgroup reduction: DPP within each wave, then across waves; only
   // the accumulators of the configurations present are reduced
@@ -500,7 +749,7 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
 #pragma unroll
   for (int i = 0; i < A::NACC; ++i) {
     bool is_new = (i < 2 * NS) || (i == 4 * NS) || (i >= 4 * NS + 2 && i < 5 * NS + 2);
-    if (is_new ? sh.has_new : sh.has_old) {
+    if (is_new ? has_new : has_old) {
       double v = wave_sum_dpp(acc[i]);
       if (lane == 63) sh.red[wave][i] = v;
     }
@@ -535,20 +784,15 @@ __device__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS
 // (one-trial-at-a-time seam: the caller decides).
 template <int NS>
 __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
-  using A = Acc<NS>;
   WState &ws = sh.ws;
   const int g = sh.g, mol = sh.mol, kind = sh.kind;
   const double *t = sh.tot;
-  const double C = 2.0 * P.rvolm * R4PIE0 / P.epsq;
-  const double alpha = P.alpha, volm = P.volm, engunit = P.engunit;
-  const double chg = P.chgtmp[g];
-  const double engsic0 = -R4PIE0 / P.epsq * alpha * P.selfq2[g] / SQRPI;
-  const double qfix0 = -(0.5 * PI * R4PIE0 / P.epsq) * ((chg * ws.sumchg) / alpha + (chg / alpha) * (chg / alpha)) / volm;
+  const double C = P.dc_C, engunit = P.engunit;
+  const double engsic0 = P.dc_esic[g], qfix0 = sh.qfix0;
   double e1new[KMAXS + 1], e1old[KMAXS + 1];
   for (int i = 1; i <= NS; ++i) {
-    double fkk = (i == mol) ? 0.5 : 1.0;
     double sic = (i == mol) ? engsic0 : 0.0;
-    double qfm = -(fkk * PI * R4PIE0 / P.epsq) * (ws.chgsum_mol[i] * chg) / (alpha * alpha * volm);
+    double qfm = sh.qfm[i];
     e1new[i] = C * t[4 * NS + 2 + i - 1] + sic + qfm;
     e1old[i] = C * t[5 * NS + 2 + i - 1] + (-sic) + (-qfm);
   }
@@ -563,10 +807,8 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
   double estep = 0.0;
   int overlap = 0;
   for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = 0.0;
-  sh.delrc = 0.0;
   if (kind == FMC_MOVE_INSERT) {
-    double delrc = gstlrcorrect<NS>(P, ws, g, 0, sh.delrc_mol);
-    sh.delrc = delrc;
+    double delrc = sh.delrc;
     overlap = sh.ovl_new;
     if (!overlap) {
       double ewld1eng = C * t[4 * NS] + engsic0 + qfix0;
@@ -582,8 +824,7 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
       sh.comp[3] = delrc / engunit;
     }
   } else if (kind == FMC_MOVE_DELETE) {
-    double delrc = gstlrcorrect<NS>(P, ws, g, 1, sh.delrc_mol);
-    sh.delrc = delrc;
+    double delrc = sh.delrc;
     overlap = sh.ovl_old;
     double ewld1eng = C * t[4 * NS + 1] + (-engsic0) + (-qfix0);
     estep = (-ewld1eng + real_o + vdw_o - ewld3) / engunit;
@@ -619,9 +860,9 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
   sh.drew = 0;
   int accepted = 0;
   if (draw) {
-    double beta = ws.beta;
-    int ingsts = ws.nmols[mol];
-    double ipress = ws.fug[g];
+    // energy_eval, utility_pack.f:3637-3693; sh.pref holds the prefactor of
+    // the insertion / deletion test
+    const double beta = ws.beta;
     if (kind == FMC_MOVE_INSERT) {
       if (!overlap) {
         sh.rande = duni(ws);
@@ -629,7 +870,7 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
         double e = estep, test;
         if (-1 * beta * e > 700.0) test = exp(700.0);
         else if (beta * e > 700.0) test = exp(-700.0);
-        else test = P.statvolm * ipress / (((double)ingsts + 1.0) * BOLTZ * ws.temp) * exp(-1.0 * beta * e);
+        else test = sh.pref * exp(-1.0 * beta * e);
         accepted = sh.rande < test;
       }
     } else if (kind == FMC_MOVE_DELETE) {
@@ -638,8 +879,8 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
       double e = -estep, test;
       if (-1 * beta * e > 700.0) test = exp(700.0);
       else if (beta * e > 700.0) test = exp(-700.0);
-      else if (fabs(ipress) < 1e-7) test = 1.0;
-      else test = (double)ingsts * BOLTZ * ws.temp / (P.statvolm * ipress) * exp(-1.0 * beta * e);
+      else if (fabs(ws.fug[g]) < 1e-7) test = 1.0;
+      else test = sh.pref * exp(-1.0 * beta * e);
       accepted = sh.rande < test;
     } else {
       if (!overlap) {
@@ -667,64 +908,100 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
 // -------------------------------------------------------------- accept --
 // accept_move, mc_moves.f:485-621 (plus the S(k) bookkeeping that
 // mc_displace / displace_guest do around it, gcmc.f:1863-1930,2862-2871).
+// S(k) update of one k-vector for the accepted move.
 template <int NS>
-__device__ void apply(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
+__device__ inline void sk_update(int kind, int mol, double2 &sm, double2 &st, double2 sn, double2 so) {
+  if (kind == FMC_MOVE_INSERT) {
+    sm.x += sn.x; sm.y += sn.y;
+    st.x += sn.x; st.y += sn.y;
+  } else if (kind == FMC_MOVE_DELETE) {
+    sm.x -= so.x; sm.y -= so.y;
+    st.x -= so.x; st.y -= so.y;
+  } else {
+    sm.x = (sm.x - so.x) + sn.x; sm.y = (sm.y - so.y) + sn.y;
+    st.x = (st.x - so.x) + sn.x; st.y = (st.y - so.y) + sn.y;
+  }
+}
+
+// accept_move, mc_moves.f:485-621 (plus the S(k) bookkeeping that
+// mc_displace / displace_guest do around it, gcmc.f:1863-1930,2862-2871).
+// REG: the register slots ks hold S(k) and this trial's s_new / s_old for
+// their k-vectors; every other k goes through global memory.  LF: the LDS
+// copy of the guest coordinates is updated with the global arrays.
+template <int NS, bool LF, bool REG>
+__device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
+                      KSlot<NS> *ks) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int g = sh.g, mol = sh.mol, natms = sh.natms, kind = sh.kind;
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
   const double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
-  for (int k = tid; k < P.nk; k += nt) {
-    double2 sm = skw[(size_t)(mol - 1) * P.nk + k], st = skw[(size_t)NS * P.nk + k];
-    if (kind == FMC_MOVE_INSERT) {
-      double2 sn = scr[k];
-      sm.x += sn.x; sm.y += sn.y;
-      st.x += sn.x; st.y += sn.y;
-    } else if (kind == FMC_MOVE_DELETE) {
-      double2 so = scr[P.nk + k];
-      sm.x -= so.x; sm.y -= so.y;
-      st.x -= so.x; st.y -= so.y;
-    } else {
-      double2 sn = scr[k], so = scr[P.nk + k];
-      sm.x = (sm.x - so.x) + sn.x; sm.y = (sm.y - so.y) + sn.y;
-      st.x = (st.x - so.x) + sn.x; st.y = (st.y - so.y) + sn.y;
+  int kfirst = 0;
+  if constexpr (REG) {
+#pragma unroll
+    for (int m = 0; m < KPT; ++m) {
+      // mol - 1 selects the row at run time; keep the slot in registers
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+        if (i == mol - 1) sk_update<NS>(kind, mol, ks[m].S[i], ks[m].S[NS], ks[m].sn, ks[m].so);
     }
+    kfirst = KPT * nt;
+  }
+  for (int k = kfirst + tid; k < P.nk; k += nt) {
+    double2 sm = skw[(size_t)(mol - 1) * P.nk + k], st = skw[(size_t)NS * P.nk + k];
+    sk_update<NS>(kind, mol, sm, st, scr[k], scr[P.nk + k]);
     skw[(size_t)(mol - 1) * P.nk + k] = sm;
     skw[(size_t)NS * P.nk + k] = st;
   }
   size_t base = (size_t)w * P.gatoms_total + P.gbase[g];
+  double *lx = nullptr, *ly = nullptr, *lz = nullptr;
+  if constexpr (LF) {
+    lx = (double *)(lds + P.off_g) + P.gbase[g];
+    ly = lx + P.gatoms_total;
+    lz = ly + P.gatoms_total;
+  }
   int nm = sh.ws.nmols[mol];
-  if (kind == FMC_MOVE_INSERT) {
+  if (kind != FMC_MOVE_DELETE) {
     if (tid < natms) {
-      size_t a = base + (size_t)nm * natms + tid;
-      W.gx[a] = sh.xn[tid][0];
-      W.gy[a] = sh.xn[tid][1];
-      W.gz[a] = sh.xn[tid][2];
+      int a = (kind == FMC_MOVE_INSERT ? nm : sh.choice - 1) * natms + tid;
+      double x = sh.xn[tid][0], y = sh.xn[tid][1], z = sh.xn[tid][2];
+      W.gx[base + a] = x;
+      W.gy[base + a] = y;
+      W.gz[base + a] = z;
+      if constexpr (LF) {
+        lx[a] = x;
+        ly[a] = y;
+        lz[a] = z;
+      }
     }
-  } else if (kind == FMC_MOVE_DELETE) {
+  } else {
     // shift the molecules after `choice` down by one, keeping their order
     int lo = (sh.choice - 1) * natms, hi = (nm - 1) * natms;
     for (int b0 = lo; b0 < hi; b0 += nt) {
       int j = b0 + tid;
       double x = 0, y = 0, z = 0;
       if (j < hi) {
-        x = W.gx[base + j + natms];
-        y = W.gy[base + j + natms];
-        z = W.gz[base + j + natms];
+        if constexpr (LF) {
+          x = lx[j + natms];
+          y = ly[j + natms];
+          z = lz[j + natms];
+        } else {
+          x = W.gx[base + j + natms];
+          y = W.gy[base + j + natms];
+          z = W.gz[base + j + natms];
+        }
       }
       __syncthreads();
       if (j < hi) {
         W.gx[base + j] = x;
         W.gy[base + j] = y;
         W.gz[base + j] = z;
+        if constexpr (LF) {
+          lx[j] = x;
+          ly[j] = y;
+          lz[j] = z;
+        }
       }
       __syncthreads();
-    }
-  } else {
-    if (tid < natms) {
-      size_t a = base + (size_t)(sh.choice - 1) * natms + tid;
-      W.gx[a] = sh.xn[tid][0];
-      W.gy[a] = sh.xn[tid][1];
-      W.gz[a] = sh.xn[tid][2];
     }
   }
   if (tid == 0) {
@@ -733,7 +1010,7 @@ __device__ void apply(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &
     for (int i = 1; i <= NS; ++i) ws.energy[i] += sh.delE[i];
     if (kind == FMC_MOVE_INSERT || kind == FMC_MOVE_DELETE) {
       double sg = kind == FMC_MOVE_INSERT ? 1.0 : -1.0;
-      double engsic0 = -R4PIE0 / P.epsq * P.alpha * P.selfq2[g] / SQRPI;
+      double engsic0 = P.dc_esic[g];
       double engsictmp = kind == FMC_MOVE_INSERT ? engsic0 : -engsic0;  // ewald.f:324-326
       ws.engsic[kk] += sg * engsictmp;
       ws.chgsum_mol[mol] += sg * P.chgtmp[g];
@@ -803,27 +1080,32 @@ __device__ void record(const DevParams &P, const DevWalkers &W, Smem<NS> &sh, lo
 }
 
 // ------------------------------------------------------------- kernels --
-// Device-resident chains: one workgroup per walker, nsteps GCMC steps.
-template <int NS, typename T>
+// Device-resident chains: one workgroup per walker, nsteps GCMC steps.  LF /
+// NSITE select the all-in-LDS variant and the site count of its straight-line
+// pair code (see evaluate).
+template <int NS, typename T, bool LF, int NSITE>
 __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, DevWalkers W, int nsteps,
                                              int count_stats) {
   extern __shared__ __align__(16) unsigned char lds[];
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
   const int w = blockIdx.x;
+  KSlot<NS> ks[KPT];
+  load_slots<NS>(P, W, w, ks);
   stage_in<NS>(W, w, sh);
-  stage_tables(P, W, w, lds);
+  stage_tables<T, LF>(P, W, w, lds);
+  const GuestXYZ<LF> G(P, W, w, lds);
   __syncthreads();
   for (int step = 0; step < nsteps; ++step) {
     if (threadIdx.x < 64) {
       if (sh.ws.mz_made - sh.ws.mz_used < 33) {
         mzt_refill(sh.ws, threadIdx.x);
-        if (threadIdx.x == 0) sh.ws.mz_made += 33;
+        if (threadIdx.x == 0) mzt_advance(sh.ws);
       }
-      if (threadIdx.x == 0) propose<NS>(P, W, w, sh);
+      if (threadIdx.x == 0) propose<NS, LF>(P, G, sh);
     }
     __syncthreads();
-    evaluate<NS, T>(P, W, w, sh, lds);
+    evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
       decide<NS>(P, sh, 1);
       WState &ws = sh.ws;
@@ -838,7 +1120,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ws.steps += 1;
     }
     __syncthreads();
-    if (sh.accepted) apply<NS>(P, W, w, sh);
+    if (sh.accepted) apply<NS, LF, true>(P, W, w, sh, lds, ks);
     if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
     if (threadIdx.x == 0 && count_stats) {
       WState &ws = sh.ws;
@@ -855,18 +1137,22 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ws.counted += 1;
     }
   }
+  store_slots<NS>(P, W, w, ks);
   stage_out<NS>(W, w, sh);
 }
 
 // One trial of the mc_moves.f seam: the proposal comes from the pending
-// fields of walker w (written by the host); results go to W.trial_out.
+// fields of walker w (written by the host); results go to W.trial_out and
+// s_new / s_old of every k-vector to the walker's scratch rows.
 template <int NS, typename T>
 __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
   extern __shared__ __align__(16) unsigned char lds[];
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
+  KSlot<NS> ks[KPT];
+  load_slots<NS>(P, W, w, ks);
   stage_in<NS>(W, w, sh);
-  stage_tables(P, W, w, lds);
+  stage_tables<T, false>(P, W, w, lds);
   if (threadIdx.x == 0) {
     WState &ws = sh.ws;
     int g = ws.pend_iguest, mol = P.locguest[g], natms = P.natoms[mol];
@@ -882,13 +1168,14 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     for (int i = 0; i < natms; ++i)
       for (int d = 0; d < 3; ++d) sh.xn[i][d] = ws.pend_xyz[3 * i + d];
     if (sh.has_old) {
-      load_molecule<NS>(P, W, w, g, sh.choice, sh.xo, natms);
+      const GuestXYZ<false> G(P, W, w, lds);
+      load_molecule<false>(P, G, g, sh.choice, sh.xo, natms);
       sh.excl_lo = (sh.choice - 1) * natms;
       sh.excl_hi = sh.choice * natms;
     }
   }
   __syncthreads();
-  evaluate<NS, T>(P, W, w, sh, lds);
+  evaluate<NS, T, false, 0>(P, W, w, sh, lds, ks, true);
   if (threadIdx.x == 0) {
     decide<NS>(P, sh, 0);
     fmc_trial *o = W.trial_out;
@@ -936,7 +1223,7 @@ __global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp
     ws.pend_kind = 0;
   }
   __syncthreads();
-  apply<NS>(P, W, w, sh);
+  apply<NS, false, false>(P, W, w, sh, lds, nullptr);
   stage_out<NS>(W, w, sh);
 }
 
