@@ -629,6 +629,10 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   for (int g = 1; g <= ng; ++g)
     if (P.natoms[P.locguest[g]] > 3) h->nsite = 0;
   if (getenv("FMC_NO_NSITE")) h->nsite = 0;
+  // mixed mode re-syncs the FP32-updated S(k) with a full FP64 re-sum after
+  // this many accepted moves; the FP64 mode follows the reference (never)
+  P.resum_every = precision == FMC_MIXED ? 1000 : 0;
+  if (getenv("FMC_RESUM_EVERY")) P.resum_every = std::max(0, atoi(getenv("FMC_RESUM_EVERY")));
   // guest capacities: enough for a dense liquid in the cell
   int slot = 0;
   for (int g = 1; g <= ng; ++g) {
@@ -1104,6 +1108,8 @@ int fmc_get_stats(fmc_handle *h, int w, fmc_walker_stats *out) {
   out->sum_e2 = s.sum_e2;
   out->counted = s.counted;
   out->pad = s.overflow;
+  out->resums = s.nresum;
+  out->resum_drift = s.resum_drift;
   return FMC_OK;
 }
 
@@ -1245,6 +1251,13 @@ int fmc_set_block(fmc_handle *h, int threads) {
 int fmc_get_block(const fmc_handle *h, int *threads) {
   if (!h || !threads) return FMC_ERR_ARG;
   *threads = h->block;
+  return FMC_OK;
+}
+
+int fmc_set_resum(fmc_handle *h, int every) {
+  if (!h || every < 0) return FMC_ERR_ARG;
+  h->hp.resum_every = every;
+  HIPCHK(hipMemcpy(h->dp, &h->hp, sizeof(DevParams), hipMemcpyHostToDevice));
   return FMC_OK;
 }
 

@@ -44,6 +44,7 @@ struct DevParams {
   int lds_all;
   int off_fw, off_k, off_g, off_vvv;
   int nvcol;                      // staged LJ columns (FP32), stride vstride
+  int resum_every;                // full FP64 S(k) re-sum after this many accepted moves (0: never)
   int vcol_src[64];               // staged column -> table column
   int8_t lpairidx[KTYPE + 1][KTYPE + 1];  // type pair -> staged column or -1
   // constants of insertion / deletion bookkeeping (ewald.f:96-105,316-345)
@@ -115,7 +116,9 @@ struct WState {
   // statistics
   long long steps, acc[6], att[6], counted;
   double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
-  int overflow, pad;
+  int overflow, acc_since_resum;
+  long long nresum;               // in-kernel FP64 re-sums done
+  double resum_drift;             // largest |S_total(incremental) - S_total(re-sum)| seen at a re-sum
 };
 
 // Per-walker device arrays.

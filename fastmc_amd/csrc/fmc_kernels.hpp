@@ -1033,6 +1033,113 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
   __syncthreads();
 }
 
+// Full FP64 structure factor of this walker (ewald1, ewald.f:354-661) for
+// the workgroup's k-vectors: register slots and any k beyond them in global
+// memory.  Same arithmetic as k_resum.  Records the largest change of the
+// total row in ws.resum_drift.  Called by every thread.
+template <int NS, bool LF>
+__device__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
+                             KSlot<NS> *ks) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const GuestXYZ<LF> G(P, W, w, lds);
+  const double *fx, *fy, *fz, *fq;
+  const int *fi;
+  const int *kl;
+  if constexpr (LF) {
+    fx = (const double *)(lds + P.off_fw);
+    fy = fx + P.nfw;
+    fz = fy + P.nfw;
+    fq = fz + P.nfw;
+    fi = (const int *)(fq + P.nfw);
+    kl = (const int *)((const double *)(lds + P.off_k) + P.nk);
+  } else {
+    fx = P.fw_per_walker ? W.fwx + (size_t)w * P.nfw : P.fwx;
+    fy = P.fw_per_walker ? W.fwy + (size_t)w * P.nfw : P.fwy;
+    fz = P.fw_per_walker ? W.fwz + (size_t)w * P.nfw : P.fwz;
+    fq = P.fwq;
+    fi = P.fwinfo;
+    kl = P.klmn;
+  }
+  const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
+  const double *r = P.rcell;
+  auto one_k = [&](int k, double2 *S) {
+    int lmn = kl[k];
+    double l = (double)(lmn & 255), m = (double)(((lmn >> 8) & 255) - 128), n = (double)(((lmn >> 16) & 255) - 128);
+    double sr[NS + 1], si[NS + 1];
+#pragma unroll
+    for (int i = 0; i <= NS; ++i) sr[i] = si[i] = 0.0;
+    for (int j = 0; j < P.nfw; ++j) {
+      double x = fx[j], y = fy[j], z = fz[j];
+      double sx = r[0] * x + r[3] * y + r[6] * z, sy = r[1] * x + r[4] * y + r[7] * z,
+             sz = r[2] * x + r[5] * y + r[8] * z;
+      double sn, cs;
+      sincospi(2.0 * (l * sx + m * sy + n * sz), &sn, &cs);
+      double q = fq[j];
+      int sp = (fi[j] >> 8) & 255;
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+        if (sp == i + 1) {
+          sr[i] += q * cs;
+          si[i] += q * sn;
+        }
+      sr[NS] += q * cs;
+      si[NS] += q * sn;
+    }
+    for (int gg = 1; gg <= P.ntpguest; ++gg) {
+      int molg = P.locguest[gg], na = P.natoms[molg], nact = sh.ws.nmols[molg] * na;
+      for (int j = 0; j < nact; ++j) {
+        int a = P.gbase[gg] + j;
+        double x = G.x[a], y = G.y[a], z = G.z[a];
+        double sx = r[0] * x + r[3] * y + r[6] * z, sy = r[1] * x + r[4] * y + r[7] * z,
+               sz = r[2] * x + r[5] * y + r[8] * z;
+        double sn, cs;
+        sincospi(2.0 * (l * sx + m * sy + n * sz), &sn, &cs);
+        double q = lsq[molg * KSITE + j % na];
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+          if (molg == i + 1) {
+            sr[i] += q * cs;
+            si[i] += q * sn;
+          }
+        sr[NS] += q * cs;
+        si[NS] += q * sn;
+      }
+    }
+    double d = fmax(fabs(S[NS].x - sr[NS]), fabs(S[NS].y - si[NS]));
+#pragma unroll
+    for (int i = 0; i <= NS; ++i) S[i] = make_double2(sr[i], si[i]);
+    return d;
+  };
+  double dmax = 0.0;
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    int k = kslot(m);
+    if (k < P.nk) dmax = fmax(dmax, one_k(k, ks[m].S));
+  }
+  double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+  for (int k = KPT * nt + tid; k < P.nk; k += nt) {
+    double2 S[NS + 1];
+#pragma unroll
+    for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
+    dmax = fmax(dmax, one_k(k, S));
+#pragma unroll
+    for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = S[i];
+  }
+  // workgroup maximum: per wave, then across waves through LDS
+  for (int off = 32; off >= 1; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+  const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
+  if (lane == 0) sh.red[wave][0] = dmax;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+    for (int q = 0; q < nwaves; ++q) v = fmax(v, sh.red[q][0]);
+    sh.ws.resum_drift = fmax(sh.ws.resum_drift, v);
+    sh.ws.nresum += 1;
+    sh.ws.acc_since_resum = 0;
+  }
+  __syncthreads();
+}
+
 template <int NS>
 __device__ inline void stage_in(const DevWalkers &W, int w, Smem<NS> &sh) {
   const int *src = (const int *)(W.ws + w);
@@ -1120,7 +1227,16 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ws.steps += 1;
     }
     __syncthreads();
-    if (sh.accepted) apply<NS, LF, true>(P, W, w, sh, lds, ks);
+    if (sh.accepted) {
+      apply<NS, LF, true>(P, W, w, sh, lds, ks);
+      if (P.resum_every > 0) {
+        // apply ends with a barrier, so every thread sees the same count
+        int due = sh.ws.acc_since_resum + 1 >= P.resum_every;
+        __syncthreads();
+        if (threadIdx.x == 0) sh.ws.acc_since_resum += 1;
+        if (due) resum_walker<NS, LF>(P, W, w, sh, lds, ks);
+      }
+    }
     if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
     if (threadIdx.x == 0 && count_stats) {
       WState &ws = sh.ws;

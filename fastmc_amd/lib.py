@@ -22,7 +22,8 @@ FMC_MIXED = 1
 EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_full_resum",
            "fmc_trial_insert", "fmc_trial_delete", "fmc_trial_displace", "fmc_accept",
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
-           "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block")
+           "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
+           "fmc_set_resum")
 
 _lib = None
 
@@ -147,6 +148,9 @@ class Handle:
 
     def set_block(self, threads: int):
         _chk("fmc_set_block", self.L.fmc_set_block(self.h, C.c_int(threads)))
+
+    def set_resum(self, every: int):
+        _chk("fmc_set_resum", self.L.fmc_set_resum(self.h, C.c_int(every)))
 
     def get_block(self) -> int:
         t = C.c_int()

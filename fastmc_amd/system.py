@@ -594,7 +594,8 @@ class FmcWalkerStats(C.Structure):
                 ("nmols", C.c_int * FMC_MAX_SPECIES), ("pad", C.c_int),
                 ("energy", C.c_double * FMC_MAX_SPECIES), ("sum_n", C.c_double * FMC_MAX_SPECIES),
                 ("sum_n2", C.c_double * FMC_MAX_SPECIES), ("sum_e", C.c_double),
-                ("sum_e2", C.c_double), ("counted", C.c_int64)]
+                ("sum_e2", C.c_double), ("counted", C.c_int64), ("resums", C.c_int64),
+                ("resum_drift", C.c_double)]
 
 
 TRACE_DTYPE = np.dtype([("step", "i4"), ("iguest", "i4"), ("move", "i4"), ("choice", "i4"),

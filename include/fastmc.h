@@ -128,6 +128,8 @@ typedef struct fmc_walker_stats {
   double sum_n2[FMC_MAX_SPECIES];
   double sum_e, sum_e2;                      /* total guest energy moments */
   int64_t counted;                           /* steps that entered the sums */
+  int64_t resums;                            /* in-kernel FP64 S(k) re-sums done */
+  double resum_drift;                        /* largest |S_total| correction a re-sum made */
 } fmc_walker_stats;
 
 typedef struct fmc_handle fmc_handle;
@@ -195,9 +197,16 @@ int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_st
  * handle's stream: total milliseconds and launch count. */
 int fmc_last_timing(fmc_handle *h, double *ms, int *launches);
 
-/* Launch-shape control: threads per walker block (64..1024); 0 = autotune. */
+/* Launch-shape control: threads per walker block (64..512, multiple of 64);
+ * 0 = autotune on a snapshot of the walkers. */
 int fmc_set_block(fmc_handle *h, int threads);
 int fmc_get_block(const fmc_handle *h, int *threads);
+
+/* Full FP64 re-sum of each walker's S(k) (ewald1, ewald.f:354-661) inside the
+ * device chain after every `every` accepted moves of that walker; 0 = never.
+ * Defaults: 1000 in FMC_MIXED (re-syncs the FP32-updated phases), 0 in
+ * FMC_FP64 (the reference never re-sums). */
+int fmc_set_resum(fmc_handle *h, int every);
 
 #ifdef __cplusplus
 }

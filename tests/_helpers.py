@@ -21,9 +21,10 @@ WRITERS = {
     "cfg0_moves5": lambda d: I.config0(d, pressure_bar=3.0, moves={
         "insert": 0.2, "delete": 0.2, "displace": 0.2, "jump": 0.0, "swap": 0.0,
         "translate": 0.2, "rotate": 0.2, "switch": 0.0}),
+    "cfg1_p10": lambda d: I.config1(d, pressure_bar=10.0),
 }
 
-CASES = ["cfg0_p0.1", "cfg0_p10", "cfg0_moves5", "calf20_co2n2"]
+CASES = ["cfg0_p0.1", "cfg0_p10", "cfg0_moves5", "calf20_co2n2", "cfg1_p10"]
 
 _cache = {}
 

@@ -2,7 +2,7 @@
 build container only; /root/reference is read-only and never travels).
 
     make -C oracle ref                 # builds oracle/_ref/ref_driver.x from the reference sources
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 For each case the inputs are written to a scratch directory, the reference
 trace recorder (oracle/ref_driver.f90 over the reference modules) runs there,
@@ -82,6 +82,7 @@ CASES = [
         "insert": 0.2, "delete": 0.2, "displace": 0.2, "jump": 0.0, "swap": 0.0,
         "translate": 0.2, "rotate": 0.2, "switch": 0.0}), 3000),
     ("calf20_co2n2", None, 1500),
+    ("cfg1_p10", lambda d: I.config1(d, pressure_bar=10.0), 2000),
 ]
 
 
@@ -210,7 +211,10 @@ def main() -> None:
         subprocess.check_call(["make", "-C", os.path.join(REPO, "oracle"), "ref"])
     ensure_calf20_inputs()
     os.makedirs(SCRATCH, exist_ok=True)
+    only = set(sys.argv[1:])  # optional: regenerate just the named cases
     for name, writer, nsteps in CASES:
+        if only and name not in only:
+            continue
         d = os.path.join(SCRATCH, name)
         shutil.rmtree(d, ignore_errors=True)
         if writer is None:

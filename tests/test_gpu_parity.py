@@ -145,6 +145,36 @@ def test_gpu_sk_resum_and_batched_walkers():
     h.close()
 
 
+def test_gpu_inkernel_fp64_resum():
+    """The periodic in-kernel FP64 re-sum of S(k): in FP64 mode it changes nothing
+    beyond round-off (the reference trace still replays with re-sums every 10
+    accepted moves); in mixed mode it bounds the drift of the FP32-updated S(k)."""
+    Handle, FP64, MIXED = _lib()
+    name = "cfg0_moves5"
+    fx = fixture(name)
+    sysm = case_system(name)
+    say("in-kernel resum, FP64 replay", name)
+    h = Handle(sysm, precision=FP64, seeds=[fx["seeds"]])
+    h.set_resum(10)
+    tr = h.run(len(fx["trace"]["step"]), trace=True)
+    bad, worst = compare_trace(tr, fx)
+    assert not bad, f"first mismatching steps {bad[:5]}"
+    assert worst <= 1e-10
+    st = h.stats(0)
+    assert st.resums == sum(fx["trace"]["accepted"]) // 10
+    assert st.resum_drift < 1e-10
+    h.close()
+    say("in-kernel resum, mixed drift")
+    hm = Handle(sysm, precision=MIXED, seeds=[fx["seeds"]])
+    hm.set_resum(25)
+    hm.run(3000)
+    st = hm.stats(0)
+    assert st.resums >= 10
+    assert 0.0 < st.resum_drift < 1e-4
+    assert hm.full_resum(0) < 1e-4
+    hm.close()
+
+
 def test_gpu_graph_replays_match_eager():
     """hipGraph-captured sweeps give the same chains as eager launches (replayed 3x)."""
     Handle, FP64, _ = _lib()
@@ -165,7 +195,7 @@ def test_gpu_graph_replays_match_eager():
     b.close()
 
 
-@pytest.mark.parametrize("name", ["cfg0_p0.1", "cfg0_p10"])
+@pytest.mark.parametrize("name", ["cfg0_p0.1", "cfg0_p10", "cfg1_p10"])
 def test_gpu_mixed_precision_du_error(name):
     """Mixed mode (FP32 pair/phase arithmetic) against FP64 on the same trials.
     Bound: |dU_mixed - dU_fp64| <= 1e-6 * max(|dU_fp64|, kB T)."""
