@@ -1,0 +1,625 @@
+! Drop-in replacement for FastMC's module mc_moves (mc_moves.f): the same
+! module name, public procedures and dummy argument lists, so gcmc.f,
+! readinputs.f and the CONFIG/FIELD/CONTROL inputs stay as they are.  The
+! trial energies of insertion, deletion and guest_energy are evaluated by
+! libfastmc_hip.so (include/fastmc.h) on the GPU; this module keeps the host
+! bookkeeping that gcmc.f reads (coordinates, counters, energies, tails).
+!
+! Build: compile against the reference's module files in place of
+! mc_moves.f and link libfastmc_hip.so (see oracle/Makefile, target dropin).
+!
+! The device handle is created at the first trial from the module globals the
+! reference's readers filled (readinputs.f, utility_pack.f, vdw_module.f,
+! ewald_module.f) and follows the chain through accept_move / reject_move.
+! Environment:
+!   FMC_SHIM_PRECISION=mixed   FP32 pair/phase arithmetic (default FP64)
+!   FMC_SHIM_TRACE=<file>      one line per trial and per decision
+!
+! Not mirrored to the host: the trial structure factors ckcsnew/ckssnew and
+! the running ckcsum/ckssum (S(k) lives on the device; gcmc.f only backs it
+! up and restores it around displacements, mc_displace gcmc.f:1863-1930),
+! and the surface-atom flags (lsurf: statistics only, always .false.).
+module mc_moves
+  use, intrinsic :: iso_c_binding
+  use readinputs
+  use ewald_module
+  use vdw_module
+  use utility_pack
+  implicit none
+
+  ! ---- bind(C) mirrors of the structs of include/fastmc.h
+  type, bind(C) :: fmc_species_c
+    integer(c_int) :: is_guest, natoms, nmols
+    type(c_ptr) :: site_charge, site_mass, site_type, site_frozen, template_xyz, xyz
+  end type fmc_species_c
+
+  type, bind(C) :: fmc_system_c
+    integer(c_int) :: imcon
+    real(c_double) :: cell(9)
+    integer(c_int) :: nspecies
+    type(c_ptr) :: species
+    integer(c_int) :: ntypes, npairs
+    type(c_ptr) :: pair_i, pair_j, pair_kind, pair_eps, pair_sig
+    real(c_double) :: rcut, rvdw, delr, alpha
+    integer(c_int) :: kmax(3)
+    real(c_double) :: engunit, epsq, overlap, surftol, sumchg
+  end type fmc_system_c
+
+  type, bind(C) :: fmc_walker_c
+    real(c_double) :: temperature
+    real(c_double) :: fugacity(8)
+    real(c_float) :: move_cum(8, 8)
+    real(c_double) :: delr(8)
+    integer(c_int) :: seed(4)
+    type(c_ptr) :: framework_xyz
+  end type fmc_walker_c
+
+  type, bind(C) :: fmc_trial_c
+    real(c_double) :: estep
+    real(c_double) :: dele(8)
+    real(c_double) :: recip, creal, vdw, tail
+    integer(c_int) :: overlap, pad
+  end type fmc_trial_c
+
+  interface
+    integer(c_int) function fmc_create(sys, nwalkers, walkers, precision, out) bind(C, name='fmc_create')
+      import :: c_int, c_ptr, fmc_system_c, fmc_walker_c
+      type(fmc_system_c), intent(in) :: sys
+      integer(c_int), value :: nwalkers
+      type(fmc_walker_c), intent(in) :: walkers
+      integer(c_int), value :: precision
+      type(c_ptr), intent(out) :: out
+    end function fmc_create
+    integer(c_int) function fmc_trial_insert(h, w, iguest, xyz, out) bind(C, name='fmc_trial_insert')
+      import :: c_int, c_ptr, c_double, fmc_trial_c
+      type(c_ptr), value :: h
+      integer(c_int), value :: w, iguest
+      real(c_double), intent(in) :: xyz(*)
+      type(fmc_trial_c), intent(out) :: out
+    end function fmc_trial_insert
+    integer(c_int) function fmc_trial_delete(h, w, iguest, choice, out) bind(C, name='fmc_trial_delete')
+      import :: c_int, c_ptr, fmc_trial_c
+      type(c_ptr), value :: h
+      integer(c_int), value :: w, iguest, choice
+      type(fmc_trial_c), intent(out) :: out
+    end function fmc_trial_delete
+    integer(c_int) function fmc_trial_displace(h, w, iguest, choice, xyz, out) bind(C, name='fmc_trial_displace')
+      import :: c_int, c_ptr, c_double, fmc_trial_c
+      type(c_ptr), value :: h
+      integer(c_int), value :: w, iguest, choice
+      real(c_double), intent(in) :: xyz(*)
+      type(fmc_trial_c), intent(out) :: out
+    end function fmc_trial_displace
+    integer(c_int) function fmc_accept(h, w) bind(C, name='fmc_accept')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: w
+    end function fmc_accept
+    integer(c_int) function fmc_reject(h, w) bind(C, name='fmc_reject')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: w
+    end function fmc_reject
+  end interface
+
+  type(c_ptr), save :: fmc_h = c_null_ptr
+  integer, save :: fmc_pending = 0          ! 1 while a device trial awaits accept/reject
+  integer, save :: fmc_disp_iguest = 0      ! guest_energy(lexisting) seen: molecule to displace
+  integer, save :: fmc_disp_choice = 0
+  integer, save :: fmc_trace_unit = 0
+  logical, save :: fmc_tracing = .false.
+  ! storage the device handle was built from
+  type(fmc_species_c), allocatable, target, save :: fmc_sp(:)
+  real(c_double), allocatable, target, save :: fmc_q(:, :), fmc_m(:, :), fmc_tm(:, :), fmc_xyz(:, :)
+  integer(c_int), allocatable, target, save :: fmc_t(:, :), fmc_f(:, :)
+  integer(c_int), allocatable, target, save :: fmc_pi(:), fmc_pj(:), fmc_pk(:)
+  real(c_double), allocatable, target, save :: fmc_pe(:), fmc_ps(:)
+
+contains
+
+  ! ---------------------------------------------------------------- setup --
+  subroutine fmc_fail(where, rc)
+    character(*), intent(in) :: where
+    integer(c_int), intent(in) :: rc
+    write(*, '(a,a,a,i6)') 'fastmc: ', where, ' returned error code ', rc
+    call error(0, int(rc))
+  end subroutine fmc_fail
+
+  subroutine fmc_shim_init(imcon, maxmls, alpha, kmax1, kmax2, kmax3, rcut, ddelr, epsq, engunit, &
+                           overlap, surftol, sumchg, dlrpot)
+    integer, intent(in) :: imcon, maxmls, kmax1, kmax2, kmax3
+    real(8), intent(in) :: alpha, rcut, ddelr, epsq, engunit, overlap, surftol, sumchg, dlrpot
+    type(fmc_system_c) :: sys
+    type(fmc_walker_c) :: wk
+    integer :: ntpguest, m, g, i, j, k, ntypes, nsite, nat, np, isg(maxmls)
+    integer(c_int) :: rc, prec
+    character(len=256) :: env
+    integer :: envlen, envst
+
+    ntpguest = size(guestx, 1)
+    isg = 0
+    do g = 1, ntpguest
+      isg(locguest(g)) = g
+    enddo
+    ntypes = 0
+    nsite = 1
+    do m = 1, maxmls
+      nsite = max(nsite, numatoms(m) * max(nummols(m), 1))
+      do i = 1, numatoms(m)
+        ntypes = max(ntypes, ltpsit(m, i))
+      enddo
+    enddo
+    allocate(fmc_sp(maxmls))
+    allocate(fmc_q(nsite, maxmls), fmc_m(nsite, maxmls), fmc_t(nsite, maxmls), fmc_f(nsite, maxmls))
+    allocate(fmc_tm(3 * nsite, maxmls), fmc_xyz(3 * nsite, maxmls))
+    fmc_tm = 0.d0
+    fmc_xyz = 0.d0
+    do m = 1, maxmls
+      nat = numatoms(m)
+      do i = 1, nat
+        fmc_q(i, m) = atmchg(m, i)
+        fmc_m(i, m) = atmwght(m, i)
+        fmc_t(i, m) = ltpsit(m, i)
+        fmc_f(i, m) = lfzsite(m, i)
+      enddo
+      g = isg(m)
+      if (g > 0) then
+        do i = 1, nat
+          fmc_tm(3 * i - 2, m) = guestx(g, i)
+          fmc_tm(3 * i - 1, m) = guesty(g, i)
+          fmc_tm(3 * i, m) = guestz(g, i)
+        enddo
+      endif
+      do i = 1, nat * nummols(m)
+        fmc_xyz(3 * i - 2, m) = molxxx(m, i)
+        fmc_xyz(3 * i - 1, m) = molyyy(m, i)
+        fmc_xyz(3 * i, m) = molzzz(m, i)
+      enddo
+      fmc_sp(m)%is_guest = merge(1, 0, g > 0)
+      fmc_sp(m)%natoms = nat
+      fmc_sp(m)%nmols = nummols(m)
+      fmc_sp(m)%site_charge = c_loc(fmc_q(1, m))
+      fmc_sp(m)%site_mass = c_loc(fmc_m(1, m))
+      fmc_sp(m)%site_type = c_loc(fmc_t(1, m))
+      fmc_sp(m)%site_frozen = c_loc(fmc_f(1, m))
+      fmc_sp(m)%template_xyz = c_loc(fmc_tm(1, m))
+      fmc_sp(m)%xyz = c_loc(fmc_xyz(1, m))
+    enddo
+    ! one LJ row per type pair that has a 12-6 entry (lstvdw / ltpvdw / prmvdw, vdw_terms.f:1-179)
+    allocate(fmc_pi(ntypes * (ntypes + 1) / 2), fmc_pj(ntypes * (ntypes + 1) / 2))
+    allocate(fmc_pk(ntypes * (ntypes + 1) / 2))
+    allocate(fmc_pe(ntypes * (ntypes + 1) / 2), fmc_ps(ntypes * (ntypes + 1) / 2))
+    np = 0
+    do i = 1, ntypes
+      do j = 1, i
+        k = lstvdw(loc2(i, j))
+        if (k < 1 .or. k > size(ltpvdw)) cycle
+        if (ltpvdw(k) /= 2) cycle
+        np = np + 1
+        fmc_pi(np) = i
+        fmc_pj(np) = j
+        fmc_pk(np) = 2
+        fmc_pe(np) = prmvdw(k, 1)
+        fmc_ps(np) = prmvdw(k, 2)
+      enddo
+    enddo
+    sys%imcon = imcon
+    sys%cell = cell
+    sys%nspecies = maxmls
+    sys%species = c_loc(fmc_sp(1))
+    sys%ntypes = ntypes
+    sys%npairs = np
+    sys%pair_i = c_loc(fmc_pi(1))
+    sys%pair_j = c_loc(fmc_pj(1))
+    sys%pair_kind = c_loc(fmc_pk(1))
+    sys%pair_eps = c_loc(fmc_pe(1))
+    sys%pair_sig = c_loc(fmc_ps(1))
+    sys%rcut = rcut
+    ! the table grid: dlrpot = rvdw / (mxegrd - 4), vdw_terms.f:202
+    sys%rvdw = dlrpot * dble(mxegrd - 4)
+    sys%delr = ddelr
+    sys%alpha = alpha
+    sys%kmax = (/ kmax1, kmax2, kmax3 /)
+    sys%engunit = engunit
+    sys%epsq = epsq
+    sys%overlap = sqrt(overlap)   ! gcmc.f:325 squared it
+    sys%surftol = surftol
+    sys%sumchg = sumchg
+    wk%temperature = 300.d0       ! the seam never draws: temperature, moves and seeds are unused
+    wk%fugacity = 0.d0
+    wk%move_cum = 0.0
+    wk%delr = ddelr
+    wk%seed = (/ 12, 34, 56, 78 /)
+    wk%framework_xyz = c_null_ptr
+    prec = 0
+    call get_environment_variable('FMC_SHIM_PRECISION', env, envlen, envst)
+    if (envst == 0 .and. envlen > 0) then
+      if (env(1:5) == 'mixed') prec = 1
+    endif
+    rc = fmc_create(sys, 1_c_int, wk, prec, fmc_h)
+    if (rc /= 0) call fmc_fail('fmc_create', rc)
+    call get_environment_variable('FMC_SHIM_TRACE', env, envlen, envst)
+    if (envst == 0 .and. envlen > 0) then
+      open(newunit=fmc_trace_unit, file=env(1:envlen), status='replace', action='write')
+      fmc_tracing = .true.
+    endif
+  end subroutine fmc_shim_init
+
+  subroutine fmc_pack(natms, xyz)
+    integer, intent(in) :: natms
+    real(c_double), intent(out) :: xyz(3 * natms)
+    integer :: i
+    do i = 1, natms
+      xyz(3 * i - 2) = newx(i)
+      xyz(3 * i - 1) = newy(i)
+      xyz(3 * i) = newz(i)
+    enddo
+  end subroutine fmc_pack
+
+  subroutine fmc_log_trial(kind, iguest, choice, natms, xyz, t)
+    integer, intent(in) :: kind, iguest, choice, natms
+    real(c_double), intent(in) :: xyz(3 * natms)
+    type(fmc_trial_c), intent(in) :: t
+    if (.not. fmc_tracing) return
+    write(fmc_trace_unit, '(a,4i8,1x,es24.16e3,i3,*(1x,es24.16e3))') 'T', kind, iguest, choice, natms, &
+      t%estep, t%overlap, xyz
+  end subroutine fmc_log_trial
+
+  ! sum of site charges and the self-interaction term of ewald1_guest
+  ! (ewald.f:96-105, 322-325)
+  subroutine fmc_selfterms(mol, natms, alpha, epsq, lexisting, chgtmp, engsictmp)
+    integer, intent(in) :: mol, natms
+    real(8), intent(in) :: alpha, epsq
+    logical, intent(in) :: lexisting
+    real(8), intent(out) :: chgtmp, engsictmp
+    integer :: n
+    real(8) :: q
+    chgtmp = 0.d0
+    engsictmp = 0.d0
+    do n = 1, natms
+      q = atmchg(mol, n)
+      chgtmp = chgtmp + q
+      engsictmp = engsictmp + q * q
+    enddo
+    engsictmp = -r4pie0 / epsq * alpha * engsictmp / sqrpi
+    if (lexisting) engsictmp = -engsictmp
+  end subroutine fmc_selfterms
+
+  subroutine fmc_zero_terms(maxmls)
+    integer, intent(in) :: maxmls
+    integer :: mxcmls
+    mxcmls = maxmls * (maxmls - 1) / 2 + maxmls
+    delE(1:mxcmls) = 0.d0
+    ewald1en(1:mxcmls) = 0.d0
+    ewald2en(1:mxcmls) = 0.d0
+    vdwen(1:mxcmls) = 0.d0
+    delrc_mol(1:mxcmls) = 0.d0
+  end subroutine fmc_zero_terms
+
+  ! ------------------------------------------------------------- checks --
+  ! Overlap and surface-atom tests of one neighbour pair (rsqdf(j), ilist(j)),
+  ! as used by single_point; semantics of mc_moves.f:10-44.
+  subroutine surface_check(iatm, j, mol, surftol, overlap, loverlap, latmsurf)
+    integer :: iatm, j, mol
+    real(8) :: surftol, overlap
+    logical :: loverlap, latmsurf
+    integer :: ta, tb, jatm, ivdw
+    real(8) :: reach
+    latmsurf = .false.
+    jatm = ilist(j)
+    ta = ltpsit(mol, iatm)
+    tb = ltype(jatm)
+    ivdw = lstvdw(loc2(ta, tb))
+    reach = (surftol + prmvdw(ivdw, 2))**2
+    if (rsqdf(j) < reach .and. lfreezesite(jatm) == 1) latmsurf = .true.
+    if (rsqdf(j) < overlap) loverlap = .true.
+  end subroutine surface_check
+
+  ! Pore-blocking test of site iatm (mc_moves.f:46-82); the reference reads
+  ! the squared distance from xdf/ydf/zdf into an integer, and so does this.
+  subroutine poreblock_check(iatm, imcon, numblocks, loverlap)
+    integer :: iatm, imcon, numblocks
+    logical :: loverlap
+    integer :: b, rsq
+    do b = 1, numblocks
+      bxdf(b) = newx(iatm) - bxxx(b)
+      bydf(b) = newy(iatm) - byyy(b)
+      bzdf(b) = newz(iatm) - bzzz(b)
+    enddo
+    call images(imcon, numblocks, cell, bxdf, bydf, bzdf)
+    do b = 1, numblocks
+      rsq = int(xdf(b)**2 + ydf(b)**2 + zdf(b)**2)
+      if (rsq - bradii(b) <= 0.d0) then
+        loverlap = .true.
+        exit
+      endif
+    enddo
+  end subroutine poreblock_check
+
+  ! -------------------------------------------------------------- trials --
+  ! Initial loading to nguests molecules with random non-overlapping
+  ! insertions, accepted regardless of energy (mc_moves.f:84-149).
+  subroutine insert_guests(idnode, imcon, totatm, ntpguest, ntpfram, iguest, nguests, rcut, ddelr, &
+                           sumchg, surftol, overlap, keyfce, alpha, drewd, volm, newld, kmax1, kmax2, &
+                           kmax3, epsq, dlrpot, ntpatm, maxvdw, engunit, delrc, maxmls, iter, lblock, &
+                           numblocks)
+    integer :: idnode, imcon, totatm, ntpguest, ntpfram, iguest, nguests, keyfce, newld
+    integer :: kmax1, kmax2, kmax3, ntpatm, maxvdw, maxmls, iter, numblocks
+    real(8) :: rcut, ddelr, sumchg, surftol, overlap, alpha, drewd, volm, epsq, dlrpot, engunit, delrc
+    logical :: lblock
+    logical :: loverlap, lnewsurf
+    integer :: mol, natms, maxiter, idum
+    real(8) :: estep, chgtmp, engsictmp
+    mol = locguest(iguest)
+    natms = numatoms(mol)
+    maxiter = 1000 * nguests
+    iter = 0
+    do while (nummols(mol) < nguests)
+      if (iter >= maxiter) call error(idnode, 2320)
+      call random_ins(idnode, natms, iguest, rcut, ddelr)
+      estep = 0.d0
+      engsicorig = engsic
+      call insertion(imcon, iguest, keyfce, alpha, rcut, ddelr, drewd, totatm, volm, kmax1, kmax2, &
+                     kmax3, epsq, dlrpot, ntpatm, maxvdw, engunit, delrc, estep, sumchg, chgtmp, &
+                     engsictmp, maxmls, loverlap, lnewsurf, surftol, overlap, newld, lblock, numblocks)
+      if (.not. loverlap) then
+        call accept_move(iguest, .true., .false., .false., lnewsurf, delrc, totatm, idum, ntpfram, &
+                         ntpguest, maxmls, sumchg, engsictmp, chgtmp, newld)
+      else
+        call reject_move(iguest, 0, .true., .false., .false., .false.)
+        engsic = engsicorig
+      endif
+      delE(:) = 0.d0
+      iter = iter + 1
+    enddo
+  end subroutine insert_guests
+
+  ! Energy of one more guest at newx/newy/newz (mc_moves.f:150-276).
+  subroutine insertion(imcon, iguest, keyfce, alpha, rcut, ddelr, drewd, totatm, volm, kmax1, kmax2, &
+                       kmax3, epsq, dlrpot, ntpatm, maxvdw, engunit, delrc, estep, sumchg, chgtmp, &
+                       engsictmp, maxmls, loverlap, lnewsurf, surftol, overlap, newld, lblock, numblocks)
+    integer :: imcon, iguest, keyfce, totatm, kmax1, kmax2, kmax3, ntpatm, maxvdw, maxmls, newld
+    integer :: numblocks
+    real(8) :: alpha, rcut, ddelr, drewd, volm, epsq, dlrpot, engunit, delrc, estep, sumchg, chgtmp
+    real(8) :: engsictmp, surftol, overlap
+    logical :: loverlap, lnewsurf, lblock
+    type(fmc_trial_c) :: t
+    real(c_double) :: xyz(3 * mxguestsite)
+    integer :: mol, natms, i
+    integer(c_int) :: rc
+    if (lblock) call error(0, 9001)   ! pore blocking is not on the device path
+    if (.not. c_associated(fmc_h)) call fmc_shim_init(imcon, maxmls, alpha, kmax1, kmax2, kmax3, rcut, &
+                                                      ddelr, epsq, engunit, overlap, surftol, sumchg, dlrpot)
+    mol = locguest(iguest)
+    natms = numatoms(mol)
+    lnewsurf = .false.
+    loverlap = .false.
+    call fmc_zero_terms(maxmls)
+    call gstlrcorrect(imcon, iguest, keyfce, ntpatm, maxvdw, delrc, volm, maxmls, .false.)
+    call fmc_pack(natms, xyz)
+    rc = fmc_trial_insert(fmc_h, 0_c_int, int(iguest, c_int), xyz, t)
+    if (rc /= 0) call fmc_fail('fmc_trial_insert', rc)
+    fmc_pending = 1
+    call fmc_log_trial(1, iguest, 0, natms, xyz, t)
+    loverlap = t%overlap /= 0
+    if (loverlap) return
+    call fmc_selfterms(mol, natms, alpha, epsq, .false., chgtmp, engsictmp)
+    estep = estep + t%estep
+    do i = 1, maxmls
+      if (i == mol) cycle
+      ewald1en(loc2(mol, i)) = t%dele(i) * engunit - delrc_mol(loc2(mol, i))
+      delE(i) = delE(i) + t%dele(i)
+    enddo
+    delE(mol) = delE(mol) + estep
+  end subroutine insertion
+
+  ! Energy of guest molecule `choice` in place, with (lexisting) or without
+  ! it in S(k) (mc_moves.f:277-374).  displace_guest (gcmc.f:2816-2901) calls
+  ! it for the molecule in place, then at its displaced position: the first
+  ! call only records the molecule (from ind(), filled by get_guest); the
+  ! second evaluates the whole displacement on the device and returns
+  ! E_new - E_old, so that displace_guest's estep - enginit is the same dU.
+  subroutine guest_energy(imcon, iguest, alpha, rcut, ddelr, drewd, totatm, maxmls, volm, kmax1, kmax2, &
+                          kmax3, epsq, dlrpot, engunit, vdwsum, ewld2sum, ewld1eng, lsurf, newld, &
+                          surftol, sumchg, chgtmp, engsictmp, loverlap, overlap, estep, lexisting, &
+                          lblock, numblocks)
+    integer :: imcon, iguest, totatm, maxmls, kmax1, kmax2, kmax3, newld, numblocks
+    real(8) :: alpha, rcut, ddelr, drewd, volm, epsq, dlrpot, engunit, vdwsum, ewld2sum, ewld1eng
+    real(8) :: surftol, sumchg, chgtmp, engsictmp, overlap, estep
+    logical :: lsurf, loverlap, lexisting, lblock
+    type(fmc_trial_c) :: t
+    real(c_double) :: xyz(3 * mxguestsite)
+    integer :: mol, natms, i, g, atmadd
+    integer(c_int) :: rc
+    if (lblock .and. .not. lexisting) call error(0, 9001)
+    if (.not. c_associated(fmc_h)) call fmc_shim_init(imcon, maxmls, alpha, kmax1, kmax2, kmax3, rcut, &
+                                                      ddelr, epsq, engunit, overlap, surftol, sumchg, dlrpot)
+    mol = locguest(iguest)
+    natms = numatoms(mol)
+    lsurf = .false.
+    loverlap = .false.
+    call fmc_zero_terms(maxmls)
+    estep = 0.d0
+    vdwsum = 0.d0
+    ewld2sum = 0.d0
+    ewld1eng = 0.d0
+    call fmc_selfterms(mol, natms, alpha, epsq, lexisting, chgtmp, engsictmp)
+    if (lexisting) then
+      atmadd = 0
+      do g = 1, iguest - 1
+        atmadd = atmadd + numatoms(locguest(g)) * nummols(locguest(g))
+      enddo
+      fmc_disp_iguest = iguest
+      fmc_disp_choice = (ind(1) - atmadd - 1) / natms + 1
+      return
+    endif
+    if (fmc_disp_iguest /= iguest) call fmc_fail('guest_energy without a molecule in place', 9002_c_int)
+    call fmc_pack(natms, xyz)
+    rc = fmc_trial_displace(fmc_h, 0_c_int, int(iguest, c_int), int(fmc_disp_choice, c_int), xyz, t)
+    if (rc /= 0) call fmc_fail('fmc_trial_displace', rc)
+    fmc_pending = 1
+    fmc_disp_iguest = 0
+    call fmc_log_trial(3, iguest, fmc_disp_choice, natms, xyz, t)
+    loverlap = t%overlap /= 0
+    estep = t%estep
+    ewld1eng = t%recip * engunit
+    ewld2sum = t%creal * engunit
+    vdwsum = t%vdw * engunit
+    do i = 1, maxmls
+      if (i == mol) cycle
+      ewald1en(loc2(mol, i)) = t%dele(i) * engunit
+    enddo
+  end subroutine guest_energy
+
+  ! Energy released by removing guest molecule `choice` (mc_moves.f:375-484).
+  subroutine deletion(imcon, keyfce, iguest, choice, alpha, rcut, ddelr, drewd, maxmls, totatm, volm, &
+                      kmax1, kmax2, kmax3, epsq, dlrpot, ntpatm, maxvdw, engunit, delrc, estep, &
+                      linitsurf, surftol, sumchg, engsictmp, chgtmp, overlap, newld)
+    integer :: imcon, keyfce, iguest, choice, maxmls, totatm, kmax1, kmax2, kmax3, ntpatm, maxvdw, newld
+    real(8) :: alpha, rcut, ddelr, drewd, volm, epsq, dlrpot, engunit, delrc, estep, surftol, sumchg
+    real(8) :: engsictmp, chgtmp, overlap
+    logical :: linitsurf
+    type(fmc_trial_c) :: t
+    real(c_double) :: xyz(3 * mxguestsite)
+    integer :: mol, natms, i, a
+    integer(c_int) :: rc
+    if (.not. c_associated(fmc_h)) call fmc_shim_init(imcon, maxmls, alpha, kmax1, kmax2, kmax3, rcut, &
+                                                      ddelr, epsq, engunit, overlap, surftol, sumchg, dlrpot)
+    mol = locguest(iguest)
+    natms = numatoms(mol)
+    linitsurf = .false.
+    call fmc_zero_terms(maxmls)
+    call gstlrcorrect(imcon, iguest, keyfce, ntpatm, maxvdw, delrc, volm, maxmls, .true.)
+    do i = 1, natms
+      a = (choice - 1) * natms + i
+      xyz(3 * i - 2) = molxxx(mol, a)
+      xyz(3 * i - 1) = molyyy(mol, a)
+      xyz(3 * i) = molzzz(mol, a)
+    enddo
+    rc = fmc_trial_delete(fmc_h, 0_c_int, int(iguest, c_int), int(choice, c_int), t)
+    if (rc /= 0) call fmc_fail('fmc_trial_delete', rc)
+    fmc_pending = 1
+    call fmc_log_trial(2, iguest, choice, natms, xyz, t)
+    call fmc_selfterms(mol, natms, alpha, epsq, .true., chgtmp, engsictmp)
+    estep = estep + t%estep
+    do i = 1, maxmls
+      if (i == mol) cycle
+      ewald1en(loc2(mol, i)) = -t%dele(i) * engunit - delrc_mol(loc2(mol, i))
+      delE(i) = delE(i) + t%dele(i)
+    enddo
+    delE(mol) = delE(mol) - estep
+  end subroutine deletion
+
+  ! ------------------------------------------------------------ decisions --
+  ! Commits the last trial: on the device, and in the host arrays gcmc.f
+  ! reads (semantics of mc_moves.f:485-621; S(k) stays on the device).
+  subroutine accept_move(iguest, insert, delete, displace, lsurf, delrc, totatm, choice, ntpfram, &
+                         ntpguest, maxmls, sumchg, engsictmp, chgtmp, newld)
+    integer :: iguest, totatm, choice, ntpfram, ntpguest, maxmls, newld
+    logical :: insert, delete, displace, lsurf
+    real(8) :: delrc, sumchg, engsictmp, chgtmp
+    integer :: mol, natms, kk, first, last, i, s
+    integer(c_int) :: rc
+    if (fmc_pending /= 0) then
+      rc = fmc_accept(fmc_h, 0_c_int)
+      if (rc /= 0) call fmc_fail('fmc_accept', rc)
+      fmc_pending = 0
+      if (fmc_tracing) write(fmc_trace_unit, '(a)') 'D 1'
+    endif
+    mol = locguest(iguest)
+    natms = numatoms(mol)
+    kk = loc2(mol, mol)
+    energy(1:maxmls) = energy(1:maxmls) + delE(1:maxmls)
+    if (insert) then
+      engsic(kk) = engsic(kk) + engsictmp
+      chgsum_mol(mol) = chgsum_mol(mol) + chgtmp
+      sumchg = sumchg + chgtmp
+      if (lsurf) surfacemols(mol) = surfacemols(mol) + 1
+      first = natms * nummols(mol)
+      do s = 1, natms
+        molxxx(mol, first + s) = newx(s)
+        molyyy(mol, first + s) = newy(s)
+        molzzz(mol, first + s) = newz(s)
+        call fmc_count_site(mol, s, 1)
+      enddo
+      elrc = elrc + delrc
+      elrc_mol(:) = elrc_mol(:) + delrc_mol(:)
+      nummols(mol) = nummols(mol) + 1
+      totatm = totatm + natms
+      call condense(totatm, ntpfram, ntpguest)
+      choice = nummols(mol)
+    elseif (delete) then
+      engsic(kk) = engsic(kk) - engsictmp
+      chgsum_mol(mol) = chgsum_mol(mol) - chgtmp
+      sumchg = sumchg - chgtmp
+      if (lsurf) surfacemols(mol) = surfacemols(mol) - 1
+      do s = 1, natms
+        call fmc_count_site(mol, s, -1)
+      enddo
+      ! molecules after `choice` move down one slot, in order
+      first = (choice - 1) * natms + 1
+      last = natms * (nummols(mol) - 1)
+      do i = first, last
+        molxxx(mol, i) = molxxx(mol, i + natms)
+        molyyy(mol, i) = molyyy(mol, i + natms)
+        molzzz(mol, i) = molzzz(mol, i + natms)
+      enddo
+      elrc = elrc - delrc
+      elrc_mol(:) = elrc_mol(:) - delrc_mol(:)
+      nummols(mol) = nummols(mol) - 1
+      totatm = totatm - natms
+      call condense(totatm, ntpfram, ntpguest)
+    elseif (displace) then
+      first = (choice - 1) * natms
+      do s = 1, natms
+        molxxx(mol, first + s) = newx(s)
+        molyyy(mol, first + s) = newy(s)
+        molzzz(mol, first + s) = newz(s)
+      enddo
+      call condense(totatm, ntpfram, ntpguest)
+    endif
+  end subroutine accept_move
+
+  subroutine fmc_count_site(mol, s, dn)
+    integer, intent(in) :: mol, s, dn
+    integer :: ka
+    ka = ltpsit(mol, s)
+    numtyp(ka) = numtyp(ka) + dn
+    numtyp_mol(mol, ka) = numtyp_mol(mol, ka) + dn
+    if (lfzsite(mol, s) /= 0) then
+      numfrz(ka) = numfrz(ka) + dn
+      numfrz_mol(mol, ka) = numfrz_mol(mol, ka) + dn
+    endif
+  end subroutine fmc_count_site
+
+  ! Discards the last trial (semantics of mc_moves.f:623-673).
+  subroutine reject_move(iguest, jguest, insert, delete, displace, swap)
+    integer :: iguest, jguest
+    logical :: insert, delete, displace, swap
+    integer :: mol, jmol, s
+    integer(c_int) :: rc
+    if (fmc_pending /= 0) then
+      rc = fmc_reject(fmc_h, 0_c_int)
+      if (rc /= 0) call fmc_fail('fmc_reject', rc)
+      fmc_pending = 0
+      if (fmc_tracing) write(fmc_trace_unit, '(a)') 'D 0'
+    endif
+    delE(:) = 0.d0
+    if (insert .or. delete) then
+      engsic = engsicorig
+    elseif (swap) then
+      mol = locguest(iguest)
+      jmol = locguest(jguest)
+      engsic = engsicorig
+      elrc_mol = origelrc_mol
+      do s = 1, numatoms(mol)
+        call fmc_count_site(mol, s, 1)
+      enddo
+      do s = 1, numatoms(jmol)
+        call fmc_count_site(jmol, s, -1)
+      enddo
+    endif
+    if (displace) fmc_disp_iguest = 0
+  end subroutine reject_move
+
+end module mc_moves

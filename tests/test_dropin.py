@@ -1,0 +1,119 @@
+"""Drop-in boundary: the reference's own main program (gcmc.f and the
+reference modules, compiled from the reference sources by `make -C oracle
+dropin`) running over the GPU module mc_moves (fastmc_amd/fortran/
+mc_moves_fmc.f90 -> libfastmc_hip.so).
+
+Every trial the Fortran chain sent to the GPU is logged by the module
+(FMC_SHIM_TRACE) and replayed through the oracle's one-trial seam with the
+same accept/reject decisions: same overlap flags, dU within the FP64 bound
+(mixed: within the mixed bound).  With the duni seeds the run printed, the
+oracle's own chain must make the same sequence of moves and decisions, i.e.
+gcmc.f over the GPU module is the reference chain."""
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from _helpers import KB_KCAL, du_tol
+from fastmc_amd import inputs as I
+from fastmc_amd.system import read_system
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "dropin", "gcmc_fmc.x")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _run(tmp, steps, precision):
+    I.config0(tmp, pressure_bar=10.0, steps=steps, equilibration=0)
+    trace = os.path.join(tmp, "shim.trace")
+    env = dict(os.environ, FMC_SHIM_TRACE=trace, FMC_SHIM_PRECISION=precision)
+    r = subprocess.run([EXE], cwd=tmp, env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-3000:])
+    out = open(os.path.join(tmp, "OUTPUT"), errors="replace").read()
+    m = re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out)
+    seeds = tuple(int(x) for x in m.groups())
+    trials = []
+    for ln in open(trace):
+        f = ln.split()
+        if f[0] == "T":
+            kind, g, choice, natms = (int(x) for x in f[1:5])
+            est = float(f[5].replace("D", "E"))
+            ovl = int(f[6])
+            xyz = np.array([float(x) for x in f[7:7 + 3 * natms]]).reshape(natms, 3)
+            trials.append({"kind": kind, "g": g, "choice": choice, "estep": est, "overlap": ovl,
+                           "xyz": xyz, "accepted": None})
+        elif f[0] == "D":
+            trials[-1]["accepted"] = int(f[1])
+    assert all(t["accepted"] is not None for t in trials)
+    return seeds, trials, read_system(tmp)
+
+
+def _replay(sysm, seeds, trials, tol_scale):
+    o = Oracle(sysm, seed=seeds)
+    o.run(0)
+    T = sysm.temperature
+    worst = 0.0
+    for i, t in enumerate(trials):
+        if t["kind"] == 1:
+            a = o.trial_insert(t["g"], t["xyz"])
+        elif t["kind"] == 2:
+            a = o.trial_delete(t["g"], t["choice"])
+        else:
+            a = o.trial_displace(t["g"], t["choice"], t["xyz"])
+        assert a.overlap == t["overlap"], (i, t["kind"])
+        if not a.overlap or t["kind"] == 2:
+            err = abs(a.estep - t["estep"]) / max(abs(a.estep), KB_KCAL * T)
+            worst = max(worst, err)
+            assert abs(a.estep - t["estep"]) <= du_tol(a.estep, T) * tol_scale, (i, t["kind"], a.estep, t["estep"])
+        if t["accepted"]:
+            o.accept()
+        else:
+            o.reject()
+    return o, worst
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in build absent (make -C oracle dropin)")
+def test_dropin_gcmc_main_program_fp64():
+    tmp = tempfile.mkdtemp(prefix="fmc_dropin_")
+    try:
+        say("drop-in: reference gcmc.f over the GPU mc_moves module, FP64")
+        seeds, trials, sysm = _run(tmp, 1000, "fp64")
+        say("drop-in:", len(trials), "trials, seeds", seeds)
+        assert len(trials) >= 1000
+        o, worst = _replay(sysm, seeds, trials, 1.0)
+        say("drop-in: worst scaled dU difference", worst)
+        # the oracle's own chain from the printed seeds makes the same moves and decisions
+        ref = Oracle(sysm, seed=seeds).run(len(trials))
+        for i, t in enumerate(trials):
+            r = ref[i]
+            assert min(int(r["move"]), 3) == t["kind"], i
+            assert int(r["accepted"]) == t["accepted"], i
+            if t["kind"] != 1:
+                assert int(r["choice"]) == t["choice"], i
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in build absent (make -C oracle dropin)")
+def test_dropin_gcmc_main_program_mixed():
+    tmp = tempfile.mkdtemp(prefix="fmc_dropin_")
+    try:
+        say("drop-in: reference gcmc.f over the GPU mc_moves module, mixed")
+        seeds, trials, sysm = _run(tmp, 600, "mixed")
+        o, worst = _replay(sysm, seeds, trials, 1e4)   # 1e-10 * 1e4 = the mixed bound 1e-6
+        say("drop-in mixed: worst scaled dU difference", worst, "over", len(trials), "trials")
+        assert worst <= 1e-6
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
