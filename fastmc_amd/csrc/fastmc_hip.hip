@@ -64,6 +64,8 @@ struct fmc_handle {
   std::vector<double> fw_q;
   double selfq2_all_fw = 0.0;
   std::vector<int> kperm;        // internal k -> reference enumeration index
+  size_t lds_base_end = 0;       // LDS bytes of the always-staged part of the plan
+  int wl_maxn = 0, wl_minn = 0, wl_nwin = 0;  // Wang-Landau set-up (0: off)
 };
 
 namespace {
@@ -721,6 +723,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       return FMC_ERR_ARG;
     }
     const size_t base_end = off;
+    h->lds_base_end = base_end;
     // LJ columns used by guest sites (FP32 mode only)
     int ncol = 0;
     for (int ti = 0; ti <= KTYPE; ++ti)
@@ -1251,6 +1254,215 @@ int fmc_set_block(fmc_handle *h, int threads) {
 int fmc_get_block(const fmc_handle *h, int *threads) {
   if (!h || !threads) return FMC_ERR_ARG;
   *threads = h->block;
+  return FMC_OK;
+}
+
+int fmc_wl_window(int win, int nwindows, int maxn, int minn, int *minmol, int *maxmol, int *tail) {
+  // divide_jobs, wang_landau.f:1742-1781 (integer arithmetic as written there)
+  if (nwindows < 1 || win < 0 || win >= nwindows || maxn < 1 || minn < 0 || minn > maxn) return FMC_ERR_ARG;
+  int t = nwindows == 1 ? 0 : 3;
+  int nodex = win + 1;
+  long long a = (long long)(maxn + 1) * nodex * nodex + (long long)((maxn + 1) / nwindows) * nodex * nodex;
+  int mx = (int)(a / ((long long)nodex * nodex + nodex + 1)) + t;
+  long long b = (long long)(maxn + 1) * win * win + (long long)((maxn + 1) / nwindows) * win * win;
+  int mn = (int)(b / ((long long)win * win + win + 1)) - t;
+  if (nodex == 1) mn = minn;
+  if (nodex == nwindows) mx = maxn;
+  if (mn < 0) mn = 0;
+  if (mx > maxn) mx = maxn;
+  if (minmol) *minmol = mn;
+  if (maxmol) *maxmol = mx;
+  if (tail) *tail = t;
+  return FMC_OK;
+}
+
+int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
+  if (!h || !p) return FMC_ERR_ARG;
+  if (p->maxn < 1 || p->minn < 0 || p->minn >= p->maxn || p->nwindows < 1 || p->nwindows > h->nwalkers ||
+      p->visittol < 0 || !(p->flatcoeff > 0.0) || !(p->prectol > 0.0) || p->create_molecules < 0)
+    return FMC_ERR_ARG;
+  if (!(p->wlprec > 1.0)) return 2319;  // readinputs.f:739
+  DevParams &P = h->hp;
+  if (p->maxn > P.gcap[1]) return FMC_ERR_ARG;
+  for (int win = 0; win < p->nwindows; ++win) {
+    int mn, mx;
+    if (fmc_wl_window(win, p->nwindows, p->maxn, p->minn, &mn, &mx, nullptr) != FMC_OK || mn > mx)
+      return FMC_ERR_ARG;
+  }
+  // LDS: ln g and visits after the rest of the plan; without room the
+  // all-in-LDS variant gives way
+  const size_t budget = 160 * 1024 - 1024;
+  const int nbins = p->maxn + 2;
+  const size_t need = (size_t)12 * nbins + 32;
+  size_t base = (size_t)P.lds_bytes;
+  if (P.wl_nbins > 0) base = (size_t)P.wl_off;
+  if (base + need > budget && P.lds_all) {
+    P.lds_all = 0;
+    base = h->lds_base_end;
+  }
+  if (base + need > budget) return FMC_ERR_ARG;
+  P.wl_off = (int)((base + 15) & ~(size_t)15);
+  P.lds_bytes = (int)((P.wl_off + (size_t)12 * nbins + 15) & ~(size_t)15);
+  P.wl_nbins = nbins;
+  P.wl_flat = p->flatcoeff;
+  P.wl_visittol = p->visittol;
+  P.wl_prectol = p->prectol;
+  if (!h->dw.wl_lng) {
+    int rc = dalloc(h, &h->dw.wl_lng, (size_t)h->nwalkers * nbins);
+    rc |= dalloc(h, &h->dw.wl_hist, (size_t)h->nwalkers * nbins);
+    if (rc) return FMC_ERR_ALLOC;
+  } else if (nbins > h->wl_maxn + 2) {
+    return FMC_ERR_ARG;  // re-setup may not grow the range
+  }
+  HIPCHK(hipMemset(h->dw.wl_lng, 0, sizeof(double) * (size_t)h->nwalkers * nbins));
+  HIPCHK(hipMemset(h->dw.wl_hist, 0, sizeof(int) * (size_t)h->nwalkers * nbins));
+  HIPCHK(allow_lds_dispatch(h->ns, h->precision == FMC_MIXED, P.lds_bytes));
+  HIPCHK(hipMemcpy(h->dp, &P, sizeof P, hipMemcpyHostToDevice));
+  // guest-1 thermal wavelength, wang_landau.f:121-123 (molmass = sum of site masses, readinputs.f:208,247)
+  const int mol1 = P.locguest[1];
+  double molmass = 0.0;
+  for (int i = 0; i < P.natoms[mol1]; ++i) molmass += P.site_m[mol1][i];
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  for (int w = 0; w < h->nwalkers; ++w)
+    if (ws[w].nmols[mol1] > p->maxn) return FMC_ERR_ARG;  // more guests than the sampled range
+  for (int w = 0; w < h->nwalkers; ++w) {
+    WState &s = ws[w];
+    int win = (int)((long long)w * p->nwindows / h->nwalkers), mn, mx;
+    fmc_wl_window(win, p->nwindows, p->maxn, p->minn, &mn, &mx, nullptr);
+    s.wl_on = 1;
+    s.wl_done = 0;
+    s.wl_error = 0;
+    s.wl_min = mn;
+    s.wl_max = mx;
+    s.wl_preload = std::max(p->create_molecules, mn);
+    s.wl_hmin = 0;
+    s.wl_hmin_cnt = mx - mn + 1;
+    s.wl_hsum = 0;
+    s.wl_sweeps = 0;
+    s.wl_iter = 0;
+    s.wl_f = p->wlprec;
+    s.wl_logf = std::log(p->wlprec);
+    double lambda = HPLANCK / std::sqrt(2.0 * PI * molmass / 1000.0 * BOLTZ * s.temp / AVO) / 1.0e-10;
+    s.wl_lambda3 = lambda * lambda * lambda;
+    // step sizes of the Wang-Landau loop, wang_landau.f:105-110
+    s.rotangle = PI / 3.0;
+    s.rota_rotangle = s.rotangle;
+    for (int g = 1; g <= P.ntpguest; ++g) {
+      s.delrdisp[g] = s.delr[g];
+      s.tran_delr[g] = s.delr[g];
+    }
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
+  h->wl_maxn = p->maxn;
+  h->wl_minn = p->minn;
+  h->wl_nwin = p->nwindows;
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return FMC_OK;
+}
+
+int fmc_wl_get(fmc_handle *h, int w, double *lng, int *visits, fmc_wl_status *st) {
+  if (!h || w < 0 || w >= h->nwalkers || !h->hp.wl_nbins) return FMC_ERR_ARG;
+  const int nb = h->hp.wl_nbins;
+  if (lng)
+    HIPCHK(hipMemcpy(lng, h->dw.wl_lng + (size_t)w * nb, sizeof(double) * (h->wl_maxn + 1), hipMemcpyDeviceToHost));
+  if (visits)
+    HIPCHK(hipMemcpy(visits, h->dw.wl_hist + (size_t)w * nb, sizeof(int) * (h->wl_maxn + 1), hipMemcpyDeviceToHost));
+  if (st) {
+    WState s;
+    HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+    st->minmol = s.wl_min;
+    st->maxmol = s.wl_max;
+    st->done = s.wl_done;
+    st->error = s.wl_error;
+    st->sweeps = s.wl_sweeps;
+    st->steps = s.steps;
+    st->f = s.wl_f;
+    st->lnf = s.wl_logf;
+    st->lambda3 = s.wl_lambda3;
+  }
+  return FMC_OK;
+}
+
+int fmc_wl_stitch(const double *lng_windows, int nwindows, int maxn, int minn, double *lng_out) {
+  // stitch_branches, wang_landau.f:1548-1621, in 0-based N
+  if (!lng_windows || !lng_out || nwindows < 1) return FMC_ERR_ARG;
+  const int nb = maxn + 1;
+  int mn0, mx0, tail;
+  if (fmc_wl_window(0, nwindows, maxn, minn, &mn0, &mx0, &tail) != FMC_OK) return FMC_ERR_ARG;
+  std::vector<double> dos(lng_windows, lng_windows + nb);
+  const double shift = dos[0];
+  for (int i = 0; i < mx0 - mn0; ++i) dos[i] -= shift;  // the reference shifts varchunk bins
+  for (int win = 1; win < nwindows; ++win) {
+    const double *buf = lng_windows + (size_t)win * nb;
+    int mn, mx;
+    fmc_wl_window(win, nwindows, maxn, minn, &mn, &mx, &tail);
+    const double cst = dos[mn + 1] - buf[mn + 1];  // obtain_min_shift: obj(1) - opt(1)
+    int ik = 1;
+    for (int j = mn; j <= mx; ++j) {
+      ++ik;
+      if (ik >= 2 && ik <= 2 * tail - 1) dos[j] = (dos[j] + buf[j] + cst) / 2.0;
+      else dos[j] = buf[j] + cst;
+    }
+  }
+  for (int i = 0; i < nb; ++i) lng_out[i] = dos[i];
+  return FMC_OK;
+}
+
+int fmc_wl_result(fmc_handle *h, double *lng_out) {
+  if (!h || !lng_out || !h->hp.wl_nbins) return FMC_ERR_ARG;
+  const int nb = h->wl_maxn + 1, nwin = h->wl_nwin;
+  std::vector<double> all((size_t)h->nwalkers * h->hp.wl_nbins);
+  HIPCHK(hipMemcpy(all.data(), h->dw.wl_lng, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+  std::vector<double> wins((size_t)nwin * nb, 0.0);
+  std::vector<int> cnt(nwin, 0);
+  for (int w = 0; w < h->nwalkers; ++w) {
+    int win = (int)((long long)w * nwin / h->nwalkers), mn, mx;
+    fmc_wl_window(win, nwin, h->wl_maxn, h->wl_minn, &mn, &mx, nullptr);
+    const double *g = all.data() + (size_t)w * h->hp.wl_nbins;
+    const double ref = g[mn];
+    for (int n = mn; n <= mx; ++n) wins[(size_t)win * nb + n] += g[n] - ref;
+    cnt[win] += 1;
+  }
+  for (int win = 0; win < nwin; ++win)
+    for (int n = 0; n < nb; ++n)
+      if (cnt[win]) wins[(size_t)win * nb + n] /= cnt[win];
+  return fmc_wl_stitch(wins.data(), nwin, h->wl_maxn, h->wl_minn, lng_out);
+}
+
+int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, double lambda3,
+                    const double *p_pa, int np, int units, double *n_out) {
+  // compute_isotherm / grand_canonical_partition, wang_landau.f:1623-1740:
+  // Z = sum_{N=0}^{maxn-minn} exp(ln g(N) + beta mu (N - minn)),
+  // <N> = sum_{N=0}^{maxn-1} N exp(ln g(N) + beta mu N) / Z, as log-sum-exp
+  if (!lng || !p_pa || !n_out || np < 0 || maxn < 1 || minn < 0 || minn > maxn || !(temperature > 0.0) ||
+      !(lambda3 > 0.0) || (units != 0 && units != 1))
+    return FMC_ERR_ARG;
+  const double beta = 1.0 / (KBOLTZ * temperature);
+  std::vector<double> zt(maxn - minn + 1), nt(maxn);
+  for (int ip = 0; ip < np; ++ip) {
+    double bmu;
+    if (units == 0) {
+      double mu = std::log(p_pa[ip] * lambda3 * beta) / beta;
+      bmu = beta * mu;
+    } else {
+      bmu = std::log(p_pa[ip] * lambda3 * 1.0e-30 / (BOLTZ * temperature));
+    }
+    double zmax = -INFINITY, nmax = -INFINITY;
+    for (int i = 0; i <= maxn - minn; ++i) {
+      zt[i] = lng[i] + bmu * (double)(i - minn);
+      zmax = std::max(zmax, zt[i]);
+    }
+    for (int k = 1; k < maxn; ++k) {
+      nt[k] = std::log((double)k) + lng[k] + bmu * (double)k;
+      nmax = std::max(nmax, nt[k]);
+    }
+    double zs = 0.0, ns = 0.0;
+    for (int i = 0; i <= maxn - minn; ++i) zs += std::exp(zt[i] - zmax);
+    for (int k = 1; k < maxn; ++k) ns += std::exp(nt[k] - nmax);
+    n_out[ip] = maxn > 1 ? std::exp(std::log(ns) + nmax - std::log(zs) - zmax) : 0.0;
+  }
   return FMC_OK;
 }
 

@@ -28,6 +28,8 @@ constexpr double PI = 3.141592653589793;    // utility_pack.f:95
 constexpr double SQRPI = 1.7724538509055159;
 constexpr double DELRMIN = 0.1;             // utility_pack.f:107
 constexpr double MINANGLE = PI / 18.0;      // utility_pack.f:104
+constexpr double HPLANCK = 6.62607004e-34;  // utility_pack.f:144
+constexpr double AVO = 6.022140857e23;      // utility_pack.f:127
 
 // Read-only system description shared by all walkers of a handle.
 struct DevParams {
@@ -45,6 +47,12 @@ struct DevParams {
   int off_fw, off_k, off_g, off_vvv;
   int nvcol;                      // staged LJ columns (FP32), stride vstride
   int resum_every;                // full FP64 S(k) re-sum after this many accepted moves (0: never)
+  // Wang-Landau sampling in N of guest 1 (wang_landau.f); wl_nbins = 0: off
+  int wl_nbins;                   // bins per walker: N = 0 .. wl_nbins-1 (maxn + 2)
+  int wl_off;                     // LDS offset of the walker's ln g (double) then visits (int)
+  int wl_visittol;                // wl_visit_tolerance
+  double wl_flat;                 // wl_flat_coeff
+  double wl_prectol;              // stop once ln f < wl_prec_tolerance
   int vcol_src[64];               // staged column -> table column
   int8_t lpairidx[KTYPE + 1][KTYPE + 1];  // type pair -> staged column or -1
   // constants of insertion / deletion bookkeeping (ewald.f:96-105,316-345)
@@ -116,6 +124,13 @@ struct WState {
   // statistics
   long long steps, acc[6], att[6], counted;
   double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
+  // Wang-Landau walker: window [wl_min, wl_max] of N, preload target,
+  // incremental minimum of the window's visit histogram, f and ln f
+  int wl_on, wl_done, wl_min, wl_max;
+  int wl_preload, wl_hmin, wl_hmin_cnt, wl_error;
+  long long wl_hsum, wl_sweeps, wl_iter;
+  double wl_f, wl_logf;
+  double wl_lambda3;              // thermal wavelength cubed of guest 1 at this T (A^3), wang_landau.f:121-123
   int overflow, acc_since_resum;
   long long nresum;               // in-kernel FP64 re-sums done
   double resum_drift;             // largest |S_total(incremental) - S_total(re-sum)| seen at a re-sum
@@ -130,6 +145,8 @@ struct DevWalkers {
   double *fwx, *fwy, *fwz;   // per-walker framework copies (if fw_per_walker)
   fmc_trace_rec *trace;      // walker 0 trace or null
   fmc_trial *trial_out;      // seam output
+  double *wl_lng;            // [w][wl_nbins] ln g(N) (dos_hist)
+  int *wl_hist;              // [w][wl_nbins] visits (visit_hist)
 };
 
 __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872

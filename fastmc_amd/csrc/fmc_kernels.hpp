@@ -20,6 +20,7 @@ template <int NS>
 struct Smem {
   WState ws;
   int kind, g, mol, natms, choice, nmols, has_new, has_old, excl_lo, excl_hi, nfw;
+  int wlmode;  // 0 grand-canonical move, 1 Wang-Landau move, 2 insert_guests preload
   int accepted, drew, ovl_new, ovl_old;
   double xn[KSITE][3], xo[KSITE][3];
   double red[KWAVES][Acc<NS>::NACC];
@@ -173,29 +174,146 @@ __device__ inline void load_molecule(const DevParams &P, const GuestXYZ<LF> &G, 
   }
 }
 
+// ---------------------------------------------------------- Wang-Landau --
+// The walker's ln g(N) and visit histogram (dos_hist / visit_hist of
+// wang_landau_module.f), staged in LDS for the launch; bin b holds N = b.
+struct WLView {
+  double *lng;
+  int *hist;
+  int nbins;
+};
+
+__device__ inline WLView wl_view(const DevParams &P, unsigned char *lds) {
+  double *l = (double *)(lds + P.wl_off);
+  return WLView{l, (int *)(l + P.wl_nbins), P.wl_nbins};
+}
+
+// minimum of the window's visits and how many bins hold it
+__device__ inline void wl_rescan(WState &ws, const WLView &v) {
+  int m = 0x7fffffff, c = 0;
+  for (int b = ws.wl_min; b <= ws.wl_max; ++b) {
+    int h = v.hist[b];
+    if (h < m) {
+      m = h;
+      c = 1;
+    } else if (h == m) {
+      ++c;
+    }
+  }
+  ws.wl_hmin = m;
+  ws.wl_hmin_cnt = c;
+}
+
+// One visit of N = b: visit_hist += 1, dos_hist += ln f (wang_landau.f:148-150,
+// 189-191, 668-677, 757-766), keeping the window minimum incrementally.
+__device__ inline void wl_visit(WState &ws, const WLView &v, int b) {
+  if (b < 0 || b >= v.nbins) return;  // outside the arrays (set-up keeps N <= maxn)
+  v.lng[b] += ws.wl_logf;
+  const bool inwin = b >= ws.wl_min && b <= ws.wl_max;
+  if (inwin && v.hist[b] == ws.wl_hmin) ws.wl_hmin_cnt -= 1;
+  v.hist[b] += 1;
+  if (inwin) {
+    ws.wl_hsum += 1;
+    if (ws.wl_hmin_cnt == 0) wl_rescan(ws, v);
+  }
+}
+
+// convergence_check (wang_landau.f:413-459) after every flat-histogram step:
+// flat when every window bin was visited and min >= flatcoeff * mean or
+// min >= visittol; then f -> sqrt(f) (adjust_factor, 527-538), ln f = log f,
+// stop once ln f < prectol, else clear the window's visits (513-525).
+__device__ inline void wl_check(const DevParams &P, WState &ws, const WLView &v) {
+  if (ws.wl_hmin < 1) return;
+  const int nb = ws.wl_max - ws.wl_min + 1;
+  const double ave = (double)ws.wl_hsum / (double)nb;
+  if (!((double)ws.wl_hmin >= P.wl_flat * ave || ws.wl_hmin >= P.wl_visittol)) return;
+  ws.wl_sweeps += 1;
+  ws.wl_f = sqrt(ws.wl_f);
+  ws.wl_logf = log(ws.wl_f);
+  if (ws.wl_logf < P.wl_prectol) {
+    ws.wl_done = 1;
+    return;
+  }
+  for (int b = ws.wl_min; b <= ws.wl_max; ++b) v.hist[b] = 0;
+  ws.wl_hsum = 0;
+  ws.wl_hmin = 0;
+  ws.wl_hmin_cnt = nb;
+}
+
 // ------------------------------------------------------------- proposal --
 // Main-loop move choice and step-size adaptation (gcmc.f:805-911), then the
 // trial geometry of mc_insert / mc_delete / mc_displace (gcmc.f:1693-1935)
 // with random_ins, random_disp, random_rot, translate and rotation
 // (utility_pack.f:3161-3563).  Thread 0 only.
+// Move from the cumulative fractions f (gcmc.f:821-844): insert, delete,
+// displace, jump, swap, translate, rotate, switch; jump, swap and switch are
+// not on this path (never configured) and fall to displacement.  skip_ins /
+// skip_del re-choose without that band, as the Wang-Landau loop does at the
+// window edges (wang_landau.f:161-235).
+__device__ inline int choose_move(const float *f, double randmov, bool skip_ins, bool skip_del) {
+  if (!skip_ins && randmov < (double)f[0]) return FMC_MOVE_INSERT;
+  if (skip_ins && randmov < (double)f[1]) return FMC_MOVE_DELETE;
+  if (!skip_ins && !skip_del && randmov < (double)f[1]) return FMC_MOVE_DELETE;
+  if (randmov < (double)f[2]) return FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[3]) return FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[4]) return FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[5]) return FMC_MOVE_TRANSLATE;
+  if (randmov < (double)f[6]) return FMC_MOVE_ROTATE;
+  return FMC_MOVE_DISPLACE;
+}
+
 template <int NS, bool LF>
-__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh) {
+__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh, unsigned char *lds) {
   WState &ws = sh.ws;
-  int g = 1;
-  if (P.ntpguest > 1) g = (int)floor(duni(ws) * P.ntpguest) + 1;
+  int g = 1, kind, wlmode = 0;
+  if (ws.wl_on && ws.nmols[P.locguest[1]] < ws.wl_preload) {
+    // insert_guests (mc_moves.f:84-149) up to the window's lower edge
+    wlmode = 2;
+    kind = FMC_MOVE_INSERT;
+    ws.wl_iter += 1;
+  } else {
+    // the grand-canonical loop draws the guest type only when there is a
+    // choice (gcmc.f:806-808); the Wang-Landau loop always draws (wang_landau.f:137)
+    if (P.ntpguest > 1 || ws.wl_on) g = (int)floor(duni(ws) * P.ntpguest) + 1;
+    const int molg = P.locguest[g];
+    double randmov = duni(ws);
+    const float *f = ws.mcf[g];
+    kind = choose_move(f, randmov, false, false);
+    if (ws.wl_on && g == 1) {
+      // window edges of the flat-histogram loop, wang_landau.f:161-235
+      wlmode = 1;
+      const WLView v = wl_view(P, lds);
+      const int nmol = ws.nmols[molg];
+      if (nmol < ws.wl_min) {
+        kind = FMC_MOVE_INSERT;
+      } else if (nmol <= ws.wl_min && kind == FMC_MOVE_DELETE) {
+        wl_visit(ws, v, nmol);
+        kind = choose_move(f, randmov, false, true);
+      }
+      if (nmol > ws.wl_max) {
+        kind = FMC_MOVE_DELETE;
+      } else if (nmol >= ws.wl_max && kind == FMC_MOVE_INSERT) {
+        wl_visit(ws, v, nmol);
+        kind = choose_move(f, randmov, true, false);
+      }
+      if (nmol == 0 && kind != FMC_MOVE_INSERT) kind = 0;  // nothing to move: an empty step
+    }
+  }
   int mol = P.locguest[g], natms = P.natoms[mol], nmols = ws.nmols[mol];
-  double randmov = duni(ws);
-  const float *f = ws.mcf[g];
-  int kind;
-  if (randmov < (double)f[0]) kind = FMC_MOVE_INSERT;
-  else if (randmov < (double)f[1]) kind = FMC_MOVE_DELETE;
-  else if (randmov < (double)f[2]) kind = FMC_MOVE_DISPLACE;
-  else if (randmov < (double)f[3]) kind = FMC_MOVE_DISPLACE;  // jump: not on this path, never configured
-  else if (randmov < (double)f[4]) kind = FMC_MOVE_DISPLACE;  // swap: idem
-  else if (randmov < (double)f[5]) kind = FMC_MOVE_TRANSLATE;
-  else if (randmov < (double)f[6]) kind = FMC_MOVE_ROTATE;
-  else kind = FMC_MOVE_DISPLACE;
-  if (nmols >= 1 && ws.disp_count[g] >= 1) {
+  sh.wlmode = wlmode;
+  if (kind == 0) {
+    sh.kind = 0;
+    sh.g = g;
+    sh.mol = mol;
+    sh.natms = natms;
+    sh.nmols = nmols;
+    sh.choice = 0;
+    sh.has_new = sh.has_old = 0;
+    sh.excl_lo = sh.excl_hi = 0;
+    return;
+  }
+  // step-size adaptation of the grand-canonical loop (the Wang-Landau loop has none)
+  if (!ws.wl_on && nmols >= 1 && ws.disp_count[g] >= 1) {
     if (ws.disp_count[g] % 100 == 0) {
       if ((ws.accept_disp[g] / (double)ws.disp_count[g]) > 0.5) ws.delrdisp[g] *= 1.05;
       else if (ws.delrdisp[g] > DELRMIN) ws.delrdisp[g] *= 0.95;
@@ -204,15 +322,15 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh)
       else if (ws.rotangle > MINANGLE) ws.rotangle *= 0.95;
     }
   }
-  if (nmols >= 1 && ws.tran_count[g] >= 1 && ws.tran_count[g] % 100 == 0) {
+  if (!ws.wl_on && nmols >= 1 && ws.tran_count[g] >= 1 && ws.tran_count[g] % 100 == 0) {
     if ((ws.accept_tran[g] / (double)ws.tran_count[g]) > 0.5) ws.tran_delr[g] *= 1.05;
     else if (ws.tran_delr[g] > DELRMIN) ws.tran_delr[g] *= 0.95;
   }
-  if (nmols >= 1 && ws.rota_count[g] >= 1 && ws.rota_count[g] % 100 == 0) {
+  if (!ws.wl_on && nmols >= 1 && ws.rota_count[g] >= 1 && ws.rota_count[g] % 100 == 0) {
     if ((ws.accept_rota[g] / (double)ws.rota_count[g]) > 0.5) ws.rota_rotangle *= 1.05;
     else if (ws.rota_rotangle > MINANGLE) ws.rota_rotangle *= 0.95;
   }
-  if (nmols == 0) kind = FMC_MOVE_INSERT;
+  if (wlmode == 0 && nmols == 0) kind = FMC_MOVE_INSERT;
 
   sh.kind = kind;
   sh.g = g;
@@ -783,8 +901,19 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
 // acceptance rules.  Thread 0.  If draw == 0 the Metropolis test is skipped
 // (one-trial-at-a-time seam: the caller decides).
 template <int NS>
-__device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
+__device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char *lds) {
   WState &ws = sh.ws;
+  if (sh.kind == 0) {
+    // an empty Wang-Landau step (no molecule to move): only the flatness test
+    sh.estep = 0.0;
+    sh.ovl_new = 0;
+    sh.rande = -1.0;
+    sh.drew = 0;
+    sh.accepted = 0;
+    for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = 0.0;
+    if (draw && ws.wl_on) wl_check(P, ws, wl_view(P, lds));
+    return;
+  }
   const int g = sh.g, mol = sh.mol, kind = sh.kind;
   const double *t = sh.tot;
   const double C = P.dc_C, engunit = P.engunit;
@@ -859,7 +988,32 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
   sh.rande = -1.0;
   sh.drew = 0;
   int accepted = 0;
-  if (draw) {
+  if (draw && sh.wlmode == 2) {
+    // insert_guests: kept whenever it does not overlap (mc_moves.f:126-142)
+    accepted = !overlap;
+  } else if (draw && sh.wlmode == 1 && (kind == FMC_MOVE_INSERT || kind == FMC_MOVE_DELETE)) {
+    // eval_wl_move, wang_landau.f:540-588: the grand-canonical weight without
+    // the fugacity, biased by exp(ln g(N) - ln g(N +- 1))
+    const double beta = ws.beta;
+    const double *lng = (const double *)(lds + P.wl_off);
+    const int nmol = ws.nmols[mol];
+    if (kind == FMC_MOVE_INSERT) {
+      if (!overlap) {
+        double unbiased = exp(-1.0 * beta * estep) * P.volm / (double)(1 + nmol);
+        double contrib = unbiased / ws.wl_lambda3 * exp(lng[nmol] - lng[nmol + 1]);
+        sh.rande = duni(ws);
+        sh.drew = 1;
+        accepted = sh.rande < contrib;
+      }
+    } else {
+      double e = -estep;
+      double unbiased = exp(-1.0 * beta * e) * (double)nmol / P.volm;
+      double contrib = unbiased * ws.wl_lambda3 * exp(lng[nmol] - lng[nmol - 1]);
+      sh.rande = duni(ws);
+      sh.drew = 1;
+      accepted = sh.rande < contrib;
+    }
+  } else if (draw) {
     // energy_eval, utility_pack.f:3637-3693; sh.pref holds the prefactor of
     // the insertion / deletion test
     const double beta = ws.beta;
@@ -897,12 +1051,29 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw) {
         }
       }
     }
-    if (accepted && kind == FMC_MOVE_INSERT && ws.nmols[mol] >= P.gcap[g]) {
-      accepted = 0;
-      ws.overflow = 1;
-    }
+  }
+  if (draw && accepted && kind == FMC_MOVE_INSERT && ws.nmols[mol] >= P.gcap[g]) {
+    accepted = 0;  // guest slots full: the walker stops growing and says so
+    ws.overflow = 1;
   }
   sh.accepted = accepted;
+  if (draw && ws.wl_on) {
+    const WLView v = wl_view(P, lds);
+    if (sh.wlmode == 1) {
+      // the bin of the resulting N (wang_landau.f:668-677, 757-766)
+      const int nmol = ws.nmols[mol];
+      if (kind == FMC_MOVE_INSERT) wl_visit(ws, v, accepted ? nmol + 1 : nmol);
+      else if (kind == FMC_MOVE_DELETE) wl_visit(ws, v, accepted ? nmol - 1 : nmol);
+    }
+    if (sh.wlmode == 2) {
+      if (ws.wl_iter >= 1000LL * ws.wl_preload && !(accepted && ws.nmols[mol] + 1 >= ws.wl_preload)) {
+        ws.wl_error = FMC_ERR_INSERT_ITER;  // mc_moves.f:125
+        ws.wl_done = 1;
+      }
+    } else {
+      wl_check(P, ws, v);
+    }
+  }
 }
 
 // -------------------------------------------------------------- accept --
@@ -1201,20 +1372,28 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
   load_slots<NS>(P, W, w, ks);
   stage_in<NS>(W, w, sh);
   stage_tables<T, LF>(P, W, w, lds);
+  if (P.wl_nbins > 0) {
+    const WLView v = wl_view(P, lds);
+    for (int b = threadIdx.x; b < P.wl_nbins; b += blockDim.x) {
+      v.lng[b] = W.wl_lng[(size_t)w * P.wl_nbins + b];
+      v.hist[b] = W.wl_hist[(size_t)w * P.wl_nbins + b];
+    }
+  }
   const GuestXYZ<LF> G(P, W, w, lds);
   __syncthreads();
   for (int step = 0; step < nsteps; ++step) {
+    if (sh.ws.wl_done) break;  // converged Wang-Landau walker (uniform: written before the last barrier)
     if (threadIdx.x < 64) {
       if (sh.ws.mz_made - sh.ws.mz_used < 33) {
         mzt_refill(sh.ws, threadIdx.x);
         if (threadIdx.x == 0) mzt_advance(sh.ws);
       }
-      if (threadIdx.x == 0) propose<NS, LF>(P, G, sh);
+      if (threadIdx.x == 0) propose<NS, LF>(P, G, sh, lds);
     }
     __syncthreads();
-    evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
+    if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
-      decide<NS>(P, sh, 1);
+      decide<NS>(P, sh, 1, lds);
       WState &ws = sh.ws;
       int kind = sh.kind;
       ws.att[kind] += 1;
@@ -1254,6 +1433,14 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     }
   }
   store_slots<NS>(P, W, w, ks);
+  if (P.wl_nbins > 0) {
+    __syncthreads();
+    const WLView v = wl_view(P, lds);
+    for (int b = threadIdx.x; b < P.wl_nbins; b += blockDim.x) {
+      W.wl_lng[(size_t)w * P.wl_nbins + b] = v.lng[b];
+      W.wl_hist[(size_t)w * P.wl_nbins + b] = v.hist[b];
+    }
+  }
   stage_out<NS>(W, w, sh);
 }
 
@@ -1293,7 +1480,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
   __syncthreads();
   evaluate<NS, T, false, 0>(P, W, w, sh, lds, ks, true);
   if (threadIdx.x == 0) {
-    decide<NS>(P, sh, 0);
+    decide<NS>(P, sh, 0, lds);
     fmc_trial *o = W.trial_out;
     o->estep = sh.estep;
     for (int i = 0; i < FMC_MAX_SPECIES; ++i) o->delE[i] = (i < NS) ? sh.delE[i + 1] : 0.0;

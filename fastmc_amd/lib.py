@@ -10,8 +10,8 @@ import os
 
 import numpy as np
 
-from .system import (CSystem, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, TRACE_DTYPE,
-                     System, make_walker)
+from .system import (CSystem, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams, FmcWlStatus,
+                     TRACE_DTYPE, System, make_walker)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
@@ -23,7 +23,8 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_trial_insert", "fmc_trial_delete", "fmc_trial_displace", "fmc_accept",
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
-           "fmc_set_resum")
+           "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
+           "fmc_wl_result", "fmc_wl_isotherm")
 
 _lib = None
 
@@ -46,6 +47,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
     L.fmc_create.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+    L.fmc_wl_isotherm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int,
+                                  C.c_int, C.c_void_p]
     _lib = L
     return L
 
@@ -179,3 +182,54 @@ class Handle:
 
     def reset_stats(self):
         _chk("fmc_reset_stats", self.L.fmc_reset_stats(self.h))
+
+    # ---- Wang-Landau (wang_landau.f)
+    def wl_setup(self, maxn: int, minn: int = 0, nwindows: int = 1, visittol: int = 10000,
+                 flatcoeff: float = 0.7, wlprec: float = float(np.e), prectol: float = 1e-8,
+                 create_molecules: int = 0):
+        p = FmcWlParams(maxn, minn, nwindows, visittol, flatcoeff, wlprec, prectol, create_molecules, 0)
+        _chk("fmc_wl_setup", self.L.fmc_wl_setup(self.h, C.byref(p)))
+        self.wl_maxn, self.wl_minn, self.wl_nwin = maxn, minn, nwindows
+
+    def wl_get(self, w: int = 0):
+        lng = np.zeros(self.wl_maxn + 1)
+        hist = np.zeros(self.wl_maxn + 1, dtype=np.int32)
+        st = FmcWlStatus()
+        _chk("fmc_wl_get", self.L.fmc_wl_get(self.h, C.c_int(w), lng.ctypes.data_as(C.c_void_p),
+                                             hist.ctypes.data_as(C.c_void_p), C.byref(st)))
+        return lng, hist, st
+
+    def wl_result(self) -> np.ndarray:
+        out = np.zeros(self.wl_maxn + 1)
+        _chk("fmc_wl_result", self.L.fmc_wl_result(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def wl_window(win: int, nwindows: int, maxn: int, minn: int = 0):
+    """Window of N for window `win` of `nwindows` (divide_jobs)."""
+    L = load_library()
+    mn, mx, t = C.c_int(), C.c_int(), C.c_int()
+    _chk("fmc_wl_window", L.fmc_wl_window(win, nwindows, maxn, minn, C.byref(mn), C.byref(mx), C.byref(t)))
+    return mn.value, mx.value, t.value
+
+
+def wl_stitch(windows, maxn: int, minn: int = 0) -> np.ndarray:
+    """Stitch per-window ln g arrays ([nwin][maxn+1]) into one (stitch_branches)."""
+    L = load_library()
+    a = np.ascontiguousarray(windows, dtype=np.float64)
+    out = np.zeros(maxn + 1)
+    _chk("fmc_wl_stitch", L.fmc_wl_stitch(a.ctypes.data_as(C.c_void_p), C.c_int(a.shape[0]), C.c_int(maxn),
+                                          C.c_int(minn), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def wl_isotherm(lng, maxn: int, minn: int, temperature: float, lambda3: float, p_pa, units: int = 0):
+    """<N>(p) from ln g(N) (compute_isotherm); p in Pa."""
+    L = load_library()
+    g = np.ascontiguousarray(lng, dtype=np.float64)
+    p = np.ascontiguousarray(np.atleast_1d(p_pa), dtype=np.float64)
+    out = np.zeros(len(p))
+    _chk("fmc_wl_isotherm", L.fmc_wl_isotherm(g.ctypes.data_as(C.c_void_p), maxn, minn, temperature, lambda3,
+                                              p.ctypes.data_as(C.c_void_p), len(p), units,
+                                              out.ctypes.data_as(C.c_void_p)))
+    return out

@@ -598,6 +598,18 @@ class FmcWalkerStats(C.Structure):
                 ("resum_drift", C.c_double)]
 
 
+class FmcWlParams(C.Structure):
+    _fields_ = [("maxn", C.c_int), ("minn", C.c_int), ("nwindows", C.c_int), ("visittol", C.c_int),
+                ("flatcoeff", C.c_double), ("wlprec", C.c_double), ("prectol", C.c_double),
+                ("create_molecules", C.c_int), ("pad", C.c_int)]
+
+
+class FmcWlStatus(C.Structure):
+    _fields_ = [("minmol", C.c_int), ("maxmol", C.c_int), ("done", C.c_int), ("error", C.c_int),
+                ("sweeps", C.c_int64), ("steps", C.c_int64), ("f", C.c_double), ("lnf", C.c_double),
+                ("lambda3", C.c_double)]
+
+
 TRACE_DTYPE = np.dtype([("step", "i4"), ("iguest", "i4"), ("move", "i4"), ("choice", "i4"),
                         ("nmols", "i4"), ("overlap", "i4"), ("accepted", "i4"),
                         ("drew_rande", "i4"), ("estep", "f8"), ("rande", "f8"),

@@ -208,6 +208,58 @@ int fmc_get_block(const fmc_handle *h, int *threads);
  * FMC_FP64 (the reference never re-sums). */
 int fmc_set_resum(fmc_handle *h, int every);
 
+/* ---- Wang-Landau flat-histogram sampling in N (wang_landau.f) ---------- */
+
+/* Parameters of the CONTROL directives wl_max_n, wl_min_n, wl_flat_coeff,
+ * wl_visit_tolerance, wl_precision, wl_prec_tolerance (readinputs.f:725-745;
+ * defaults gcmc.f:166-176) plus the window count (the reference's MPI ranks,
+ * divide_jobs wang_landau.f:1742-1781) and "create molecules" of guest 1. */
+typedef struct fmc_wl_params {
+  int maxn, minn;          /* N range sampled (default 200, 0) */
+  int nwindows;            /* N-windows; walkers are split over them in order, evenly */
+  int visittol;            /* a window is flat once every bin has this many visits (10000) */
+  double flatcoeff;        /* ... or once min >= flatcoeff * mean (0.7) */
+  double wlprec;           /* initial f, ln g += ln f per visit (e) */
+  double prectol;          /* a walker stops once ln f < prectol (1e-8) */
+  int create_molecules;    /* preload to max(create_molecules, window minimum) (insert_guests) */
+  int pad;
+} fmc_wl_params;
+
+/* Per-walker Wang-Landau status. */
+typedef struct fmc_wl_status {
+  int minmol, maxmol;      /* the walker's window of N */
+  int done, error;         /* converged (ln f < prectol); error code (2320: preload failed) */
+  int64_t sweeps;          /* flat histograms reached (f -> sqrt f each) */
+  int64_t steps;           /* steps taken */
+  double f, lnf, lambda3;  /* current f, ln f; thermal wavelength cubed (A^3) */
+} fmc_wl_status;
+
+/* Switch every walker of the handle to Wang-Landau sampling of the number of
+ * guest-1 molecules (wang_landau_sim, wang_landau.f:9-411): moves of guest 1
+ * are accepted with eval_wl_move (540-588), ln g(N) and the visits are kept
+ * in FP64 / int per walker, windows from divide_jobs.  The reference stops
+ * with error 2318 for more than one guest type; here other guest types keep
+ * their grand-canonical moves at their fugacities.  fmc_run / fmc_run_graph
+ * then advance the walkers; a converged walker stops. */
+int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p);
+/* Read walker w: ln g [maxn+1], visits [maxn+1] (either may be NULL). */
+int fmc_wl_get(fmc_handle *h, int w, double *lng, int *visits, fmc_wl_status *st);
+/* Window range of window `win` of `nwindows` (divide_jobs, wang_landau.f:1742-1781). */
+int fmc_wl_window(int win, int nwindows, int maxn, int minn, int *minmol, int *maxmol, int *tail);
+/* Stitch per-window ln g (each [maxn+1], window order) into one ln g(N)
+ * (stitch_branches, wang_landau.f:1548-1621). */
+int fmc_wl_stitch(const double *lng_windows, int nwindows, int maxn, int minn, double *lng_out);
+/* Merge the handle's walkers: per window, the walkers' ln g aligned at the
+ * window's first bin and averaged; then stitched. lng_out [maxn+1]. */
+int fmc_wl_result(fmc_handle *h, double *lng_out);
+/* <N>(p) from ln g (compute_isotherm / grand_canonical_partition,
+ * wang_landau.f:1623-1740) with log-sum-exp in FP64 for the reference's
+ * multiprecision sums.  p in Pa.  units 0: the reference's chemical potential
+ * beta*mu = ln(p * lambda3 * beta) with lambda3 in A^3 and beta in mol/kcal;
+ * units 1: beta*mu = ln(p * lambda3 * 1e-30 / (kB T)) (SI ideal gas). */
+int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, double lambda3,
+                    const double *p_pa, int np, int units, double *n_out);
+
 #ifdef __cplusplus
 }
 #endif

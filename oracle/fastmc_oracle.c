@@ -26,6 +26,8 @@
 #define DELRMIN 0.1                 /* utility_pack.f:107 */
 #define MINANGLE (PI / 18.0)        /* utility_pack.f:104 */
 #define MAXGUEST 20000              /* utility_pack.f:89 */
+#define HPLANCK 6.62607004e-34      /* utility_pack.f:144 */
+#define AVO 6.022140857e23          /* utility_pack.f:127 */
 
 /* ------------------------------------------------------------------ RNG -- */
 /* duni, utility_pack.f:2768-2857: Marsaglia-Zaman-Tsang universal generator in
@@ -150,6 +152,12 @@ struct orc_state {
   int tran_count[FMC_MAX_SPECIES + 1], accept_tran[FMC_MAX_SPECIES + 1];
   int rota_count[FMC_MAX_SPECIES + 1], accept_rota[FMC_MAX_SPECIES + 1];
   double disp_ratio, tran_ratio, rota_ratio;
+  /* Wang-Landau walker (wang_landau.f): window, preload target, ln g and visits by N */
+  int wl_on, wl_done, wl_min, wl_max, wl_preload, wl_error, wl_visittol, wl_nbins;
+  long long wl_iter, wl_sweeps;
+  double wl_f, wl_logf, wl_flat, wl_prectol, wl_lambda3;
+  double *wl_lng;
+  int *wl_hist;
 };
 
 static int loc2(int i, int j) {  /* utility_pack.f:2859-2872 */
@@ -1614,6 +1622,7 @@ void orc_destroy(orc_state *s) {
   for (int i = 0; i <= FMC_MAX_SITES; ++i) free(s->gstlist[i]);
   free(s->ckcsum); free(s->ckssum); free(s->ckcsnew); free(s->ckssnew); free(s->ckcsorig); free(s->ckssorig);
   free(s->elc0); free(s->els0); free(s->elc1); free(s->els1); free(s->emc); free(s->ems); free(s->enc); free(s->ens);
+  free(s->wl_lng); free(s->wl_hist);
   free(s);
 }
 
@@ -1862,6 +1871,317 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       r->rotangle = s->rotangle;
       r->rota_rotangle = s->rota_rotangle;
     }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------- Wang-Landau -- */
+/* The flat-histogram loop of wang_landau_sim (wang_landau.f:9-411) with
+ * wl_insert / wl_delete / wl_displace (590-895), eval_wl_move (540-588),
+ * convergence_check (413-459), adjust_factor (527-538) and reset_visit_hist
+ * (513-525); before it, insert_guests (mc_moves.f:84-149) up to the preload
+ * target, one attempt per step.  Bin N of ln g / visits is dos_hist(N+1).
+ * The reference stops for more than one guest type (error 2318); here guest
+ * types other than 1 keep grand-canonical moves.  A displacement with no
+ * molecule present (undefined in the reference) is an empty step. */
+int orc_wl_setup(orc_state *s, int minmol, int maxmol, int maxn, int preload, double wlprec, double flat,
+                 int visittol, double prectol) {
+  if (minmol < 0 || maxmol < minmol || maxn < maxmol) return FMC_ERR_ARG;
+  free(s->wl_lng);
+  free(s->wl_hist);
+  s->wl_nbins = maxn + 2;
+  s->wl_lng = (double *)calloc((size_t)s->wl_nbins, sizeof(double));
+  s->wl_hist = (int *)calloc((size_t)s->wl_nbins, sizeof(int));
+  s->wl_on = 1;
+  s->wl_done = 0;
+  s->wl_error = 0;
+  s->wl_min = minmol;
+  s->wl_max = maxmol;
+  s->wl_preload = preload > minmol ? preload : minmol;
+  s->wl_iter = 0;
+  s->wl_sweeps = 0;
+  s->wl_f = wlprec;
+  s->wl_logf = log(wlprec);
+  s->wl_flat = flat;
+  s->wl_visittol = visittol;
+  s->wl_prectol = prectol;
+  /* step sizes of the Wang-Landau loop, wang_landau.f:105-110 */
+  s->rotangle = PI / 3.0;
+  s->rota_rotangle = s->rotangle;
+  for (int g = 1; g <= s->ntpguest; ++g) {
+    s->delrdisp[g] = s->delr[g];
+    s->tran_delr[g] = s->delr[g];
+  }
+  int mol = s->locguest[1];
+  double molmass = 0.0;
+  for (int i = 1; i <= s->numatoms[mol]; ++i) molmass += s->atmwght[mol][i];
+  double lambda = HPLANCK / sqrt(2.0 * PI * molmass / 1000.0 * BOLTZ * s->temp / AVO) / 1.0e-10;
+  s->wl_lambda3 = lambda * lambda * lambda;
+  return 0;
+}
+
+static void wl_visit(orc_state *s, int n) {
+  s->wl_hist[n] += 1;
+  s->wl_lng[n] += s->wl_logf;
+}
+
+static int wl_flat(const orc_state *s) {
+  double ave = 0.0, minvar = 1.0e99;
+  int ncount = 0;
+  for (int b = s->wl_min; b <= s->wl_max; ++b) {
+    double v = (double)s->wl_hist[b];
+    ++ncount;
+    if (v < minvar) minvar = v;
+    ave += v;
+  }
+  ave /= (double)ncount;
+  if (minvar <= 0.5) return 0;
+  return (minvar >= s->wl_flat * ave) || ((int)minvar >= s->wl_visittol);
+}
+
+static int eval_wl_move(orc_state *s, int iguest, int insert, double estep) {
+  int nmol = s->nummols[s->locguest[iguest]];
+  double contrib;
+  if (insert) {
+    double unbiased = exp(-1.0 * s->beta * estep) * s->volm / (double)(1 + nmol);
+    contrib = unbiased / s->wl_lambda3 * exp(s->wl_lng[nmol] - s->wl_lng[nmol + 1]);
+  } else {
+    double unbiased = exp(-1.0 * s->beta * estep) * (double)nmol / s->volm;
+    contrib = unbiased * s->wl_lambda3 * exp(s->wl_lng[nmol] - s->wl_lng[nmol - 1]);
+  }
+  double randn = mzt_next(&s->rng);
+  return randn < contrib;
+}
+
+static int wl_choose(const float *f, double randmov, int skip_ins, int skip_del) {
+  if (!skip_ins && randmov < (double)f[0]) return FMC_MOVE_INSERT;
+  if (skip_ins && randmov < (double)f[1]) return FMC_MOVE_DELETE;
+  if (!skip_ins && !skip_del && randmov < (double)f[1]) return FMC_MOVE_DELETE;
+  if (randmov < (double)f[2]) return FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[5]) return randmov < (double)f[4] ? FMC_MOVE_DISPLACE : FMC_MOVE_TRANSLATE;
+  if (randmov < (double)f[6]) return FMC_MOVE_ROTATE;
+  return FMC_MOVE_DISPLACE;
+}
+
+static void random_ins_xyz(orc_state *s, int iguest, int natms) {  /* utility_pack.f:3220-3331 */
+  double ra = mzt_next(&s->rng), rb = mzt_next(&s->rng), rcc = mzt_next(&s->rng);
+  double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+  const double *cl = s->cell;
+  double xc = ra * cl[1] + rb * cl[4] + rcc * cl[7];
+  double yc = ra * cl[2] + rb * cl[5] + rcc * cl[8];
+  double zc = ra * cl[3] + rb * cl[6] + rcc * cl[9];
+  for (int i = 1; i <= natms; ++i) {
+    s->newx[i] = s->guestx[iguest][i];
+    s->newy[i] = s->guesty[iguest][i];
+    s->newz[i] = s->guestz[iguest][i];
+  }
+  double q[4];
+  quat_from(r1, r2, r3, r4, 2.0 * PI, q);
+  rotation(s->newx, s->newy, s->newz, 0.0, 0.0, 0.0, natms, q[0], q[1], q[2], q[3]);
+  for (int i = 1; i <= natms; ++i) {
+    s->newx[i] += xc;
+    s->newy[i] += yc;
+    s->newz[i] += zc;
+  }
+}
+
+int orc_wl_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
+  if (!s->wl_on) return FMC_ERR_STATE;
+  const int mol1 = s->locguest[1];
+  for (int step = 1; step <= nsteps; ++step) {
+    if (s->wl_done) break;
+    fmc_trial tr;
+    memset(&tr, 0, sizeof tr);
+    double rande = -1.0;
+    int drew = 0, accepted = 0, choice = 0, iguest = 1, kind, preload = 0;
+    if (s->nummols[mol1] < s->wl_preload) {
+      preload = 1;
+      kind = FMC_MOVE_INSERT;
+      s->wl_iter += 1;
+    } else {
+      iguest = (int)floor(mzt_next(&s->rng) * s->ntpguest) + 1;
+      double randmov = mzt_next(&s->rng);
+      const float *f = s->mcf[iguest];
+      kind = wl_choose(f, randmov, 0, 0);
+      int nmol = s->nummols[s->locguest[iguest]];
+      if (iguest == 1) {
+        if (nmol < s->wl_min) {
+          kind = FMC_MOVE_INSERT;
+        } else if (nmol <= s->wl_min && kind == FMC_MOVE_DELETE) {
+          wl_visit(s, nmol);
+          kind = wl_choose(f, randmov, 0, 1);
+        }
+        if (nmol > s->wl_max) {
+          kind = FMC_MOVE_DELETE;
+        } else if (nmol >= s->wl_max && kind == FMC_MOVE_INSERT) {
+          wl_visit(s, nmol);
+          kind = wl_choose(f, randmov, 1, 0);
+        }
+        if (nmol == 0 && kind != FMC_MOVE_INSERT) kind = 0;
+      } else if (nmol == 0) {
+        kind = FMC_MOVE_INSERT;
+      }
+    }
+    int mol = s->locguest[iguest], natms = s->numatoms[mol], nmols = s->nummols[mol];
+    int wlg = !preload && iguest == 1;
+    double rec[3 * FMC_MAX_SITES];
+    memset(rec, 0, sizeof rec);
+    if (kind == FMC_MOVE_INSERT) {
+      for (int i = 0; i < 64; ++i) s->engsicorig[i] = s->engsic[i];
+      random_ins_xyz(s, iguest, natms);
+      for (int i = 1; i <= natms; ++i) {
+        rec[3 * (i - 1) + 0] = s->newx[i];
+        rec[3 * (i - 1) + 1] = s->newy[i];
+        rec[3 * (i - 1) + 2] = s->newz[i];
+      }
+      insertion(s, iguest, &tr);
+      if (preload) {
+        accepted = !tr.overlap;
+      } else if (!tr.overlap) {
+        if (wlg) {
+          accepted = eval_wl_move(s, iguest, 1, tr.estep);
+        } else {
+          rande = mzt_next(&s->rng);
+          accepted = energy_eval(s, tr.estep, rande, iguest, FMC_MOVE_INSERT);
+        }
+        drew = 1;
+      }
+      if (wlg) wl_visit(s, accepted ? nmols + 1 : nmols);
+      if (accepted) {
+        accept_move(s, FMC_MOVE_INSERT, iguest, 0);
+      } else {
+        for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
+        for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
+      }
+      if (preload && s->wl_iter >= 1000LL * s->wl_preload && s->nummols[mol1] < s->wl_preload) {
+        s->wl_error = FMC_ERR_INSERT_ITER;  /* mc_moves.f:125 */
+        s->wl_done = 1;
+      }
+    } else if (kind == FMC_MOVE_DELETE) {
+      for (int i = 0; i < 64; ++i) s->engsicorig[i] = s->engsic[i];
+      choice = (int)floor(mzt_next(&s->rng) * nmols) + 1;
+      deletion(s, iguest, choice, &tr);
+      for (int i = 1; i <= natms; ++i) {
+        rec[3 * (i - 1) + 0] = s->newx[i];
+        rec[3 * (i - 1) + 1] = s->newy[i];
+        rec[3 * (i - 1) + 2] = s->newz[i];
+      }
+      if (wlg) {
+        accepted = eval_wl_move(s, iguest, 0, -tr.estep);
+      } else {
+        rande = mzt_next(&s->rng);
+        accepted = energy_eval(s, -tr.estep, rande, iguest, FMC_MOVE_DELETE);
+      }
+      drew = 1;
+      if (wlg) wl_visit(s, accepted ? nmols - 1 : nmols);
+      if (accepted) {
+        accept_move(s, FMC_MOVE_DELETE, iguest, choice);
+      } else {
+        for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
+        for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
+      }
+    } else if (kind != 0) {
+      /* wl_displace: fixed delr(iguest), rotangle and rota_rotangle (no adaptation) */
+      double a = 0, b = 0, c = 0, q[4] = {0, 0, 0, 0};
+      double h = s->delr[iguest] * 0.5;
+      if (kind == FMC_MOVE_TRANSLATE) {
+        s->tran_count[iguest] += 1;
+        a = (2.0 * mzt_next(&s->rng) - 1.0) * h;
+        b = (2.0 * mzt_next(&s->rng) - 1.0) * h;
+        c = (2.0 * mzt_next(&s->rng) - 1.0) * h;
+      } else if (kind == FMC_MOVE_ROTATE) {
+        s->rota_count[iguest] += 1;
+        double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+        quat_from(r1, r2, r3, r4, s->rota_rotangle, q);
+      } else {
+        s->disp_count[iguest] += 1;
+        double da = mzt_next(&s->rng), db = mzt_next(&s->rng), dc = mzt_next(&s->rng);
+        a = (2.0 * da - 1.0) * h;
+        b = (2.0 * db - 1.0) * h;
+        c = (2.0 * dc - 1.0) * h;
+        double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+        quat_from(r1, r2, r3, r4, s->rotangle, q);
+      }
+      choice = (int)floor(mzt_next(&s->rng) * nmols) + 1;
+      displace_eval(s, iguest, choice, &tr, 0, NULL, a, b, c, q);
+      for (int i = 1; i <= natms; ++i) {
+        rec[3 * (i - 1) + 0] = s->newx[i];
+        rec[3 * (i - 1) + 1] = s->newy[i];
+        rec[3 * (i - 1) + 2] = s->newz[i];
+      }
+      if (!tr.overlap) {
+        if (tr.estep < 0.0) {
+          accepted = 1;
+        } else {
+          rande = mzt_next(&s->rng);
+          drew = 1;
+          accepted = energy_eval(s, tr.estep, rande, iguest, FMC_MOVE_DISPLACE);
+        }
+      }
+      if (accepted) {
+        accept_move(s, FMC_MOVE_DISPLACE, iguest, choice);
+        if (kind == FMC_MOVE_TRANSLATE) s->accept_tran[iguest] += 1;
+        else if (kind == FMC_MOVE_ROTATE) s->accept_rota[iguest] += 1;
+        else s->accept_disp[iguest] += 1;
+      } else {
+        restore_sk(s, mol);
+        for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
+      }
+    }
+    if (!preload && wl_flat(s)) {
+      s->wl_sweeps += 1;
+      s->wl_f = sqrt(s->wl_f);
+      s->wl_logf = log(s->wl_f);
+      if (s->wl_logf < s->wl_prectol) {
+        s->wl_done = 1;
+      } else {
+        for (int bb = s->wl_min; bb <= s->wl_max; ++bb) s->wl_hist[bb] = 0;
+      }
+    }
+    if (trace) {
+      fmc_trace_rec *r = &trace[step - 1];
+      memset(r, 0, sizeof *r);
+      r->step = step;
+      r->iguest = iguest;
+      r->move = kind;
+      r->choice = choice;
+      r->nmols = nmols;
+      r->overlap = tr.overlap;
+      r->accepted = accepted;
+      r->drew_rande = drew;
+      r->estep = tr.estep;
+      r->rande = rande;
+      for (int i = 0; i < 3 * natms && i < 3 * FMC_MAX_SITES; ++i) r->xyz[i] = kind ? rec[i] : 0.0;
+      for (int i = 1; i <= s->maxmls && i <= FMC_MAX_SPECIES; ++i) {
+        r->energy[i - 1] = s->energy[i];
+        r->delE[i - 1] = accepted ? s->delE[i] : 0.0;
+      }
+      r->delrdisp = s->delrdisp[iguest];
+      r->rotangle = s->rotangle;
+      r->rota_rotangle = s->rota_rotangle;
+    }
+  }
+  return 0;
+}
+
+/* ln g, visits [nbins] and (done, error, sweeps) ; f, ln f, lambda^3 */
+int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst) {
+  if (!s->wl_on) return FMC_ERR_STATE;
+  for (int b = 0; b < s->wl_nbins; ++b) {
+    if (lng) lng[b] = s->wl_lng[b];
+    if (hist) hist[b] = s->wl_hist[b];
+  }
+  if (ist) {
+    ist[0] = s->wl_done;
+    ist[1] = s->wl_error;
+    ist[2] = s->wl_sweeps;
+    ist[3] = s->wl_min;
+    ist[4] = s->wl_max;
+  }
+  if (dst) {
+    dst[0] = s->wl_f;
+    dst[1] = s->wl_logf;
+    dst[2] = s->wl_lambda3;
   }
   return 0;
 }

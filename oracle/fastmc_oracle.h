@@ -25,6 +25,12 @@ int orc_reject(orc_state *s);
 
 int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
 
+/* Wang-Landau walker over N of guest 1 (wang_landau.f). */
+int orc_wl_setup(orc_state *s, int minmol, int maxmol, int maxn, int preload, double wlprec, double flat,
+                 int visittol, double prectol);
+int orc_wl_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
+int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst);
+
 int orc_get_guests(orc_state *s, int iguest, double *xyz, int cap_mols, int *nmols);
 int orc_get_sk(orc_state *s, double *sk);
 int orc_get_table(orc_state *s, int which, int k, double *out, int n);

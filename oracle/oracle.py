@@ -35,9 +35,11 @@ def lib():
         for name in ("orc_info", "orc_single_point", "orc_trial_insert", "orc_trial_delete",
                      "orc_trial_displace", "orc_accept", "orc_reject", "orc_run", "orc_get_guests",
                      "orc_get_sk", "orc_get_table", "orc_get_kvec", "orc_get_energy",
-                     "orc_full_resum"):
+                     "orc_full_resum", "orc_wl_setup", "orc_wl_run", "orc_wl_get"):
             getattr(L, name).restype = C.c_int
         L.orc_duni_seq.restype = None
+        L.orc_wl_setup.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_int, C.c_double]
         _lib = L
     return _lib
 
@@ -151,3 +153,100 @@ class Oracle:
         d = C.c_double()
         lib().orc_full_resum(C.c_void_p(self.h), C.byref(d))
         return d.value
+
+
+# ---------------------------------------------------------------- Wang-Landau
+# wang_landau.f restated for the checks: the walker itself is in the C oracle
+# (orc_wl_setup / orc_wl_run); the window split, stitching and isotherm are
+# restated here in numpy.
+
+def _wl_methods():
+    def wl_setup(self, minmol, maxmol, maxn, preload=0, wlprec=np.e, flat=0.7, visittol=10000, prectol=1e-8):
+        rc = lib().orc_wl_setup(C.c_void_p(self.h), minmol, maxmol, maxn, preload, wlprec, flat, visittol,
+                                prectol)
+        if rc:
+            raise RuntimeError(f"orc_wl_setup rc={rc}")
+        self.wl_maxn = maxn
+
+    def wl_run(self, nsteps):
+        tr = np.zeros(nsteps, dtype=TRACE_DTYPE)
+        rc = lib().orc_wl_run(C.c_void_p(self.h), C.c_int(nsteps), tr.ctypes.data_as(C.c_void_p))
+        if rc:
+            raise RuntimeError(f"orc_wl_run rc={rc}")
+        return tr
+
+    def wl_get(self):
+        nb = self.wl_maxn + 2
+        lng, hist = np.zeros(nb), np.zeros(nb, dtype=np.int32)
+        ist, dst = np.zeros(5, dtype=np.int64), np.zeros(3)
+        lib().orc_wl_get(C.c_void_p(self.h), lng.ctypes.data_as(C.c_void_p), hist.ctypes.data_as(C.c_void_p),
+                         ist.ctypes.data_as(C.c_void_p), dst.ctypes.data_as(C.c_void_p))
+        st = {"done": int(ist[0]), "error": int(ist[1]), "sweeps": int(ist[2]), "minmol": int(ist[3]),
+              "maxmol": int(ist[4]), "f": dst[0], "lnf": dst[1], "lambda3": dst[2]}
+        return lng[: self.wl_maxn + 1], hist[: self.wl_maxn + 1], st
+
+    Oracle.wl_setup, Oracle.wl_run, Oracle.wl_get = wl_setup, wl_run, wl_get
+
+
+_wl_methods()
+
+
+def wl_window(win, nwindows, maxn, minn):
+    """divide_jobs, wang_landau.f:1742-1781 (integer arithmetic)."""
+    tail = 0 if nwindows == 1 else 3
+    nodex = win + 1
+    mx = ((maxn + 1) * nodex ** 2 + (maxn + 1) // nwindows * nodex ** 2) // (nodex ** 2 + nodex + 1) + tail
+    mn = ((maxn + 1) * win ** 2 + (maxn + 1) // nwindows * win ** 2) // (win ** 2 + win + 1) - tail
+    if nodex == 1:
+        mn = minn
+    if nodex == nwindows:
+        mx = maxn
+    return max(mn, 0), min(mx, maxn), tail
+
+
+def wl_stitch(windows, maxn, minn):
+    """stitch_branches, wang_landau.f:1548-1621, over per-window ln g arrays [nwin][maxn+1]."""
+    nwin = len(windows)
+    dos = np.array(windows[0], dtype=np.float64).copy()
+    mn0, mx0, tail = wl_window(0, nwin, maxn, minn)
+    shift = dos[0]
+    dos[: mx0 - mn0] -= shift
+    for w in range(1, nwin):
+        buf = np.asarray(windows[w], dtype=np.float64)
+        mn, mx, tail = wl_window(w, nwin, maxn, minn)
+        const = dos[mn + 1] - buf[mn + 1]
+        for ik, j in enumerate(range(mn, mx + 1), start=2):
+            if 2 <= ik <= 2 * tail - 1:
+                dos[j] = (dos[j] + buf[j] + const) / 2.0
+            else:
+                dos[j] = buf[j] + const
+    return dos
+
+
+KBOLTZ = 1.9872041e-3
+BOLTZ = 1.3806503e-23
+
+
+def wl_isotherm(lng, maxn, minn, temperature, lambda3, p_pa, units=0):
+    """compute_isotherm + grand_canonical_partition, wang_landau.f:1623-1740."""
+    beta = 1.0 / (KBOLTZ * temperature)
+    out = []
+    for p in np.atleast_1d(p_pa):
+        if units == 0:
+            mu = np.log(p * lambda3 * beta) / beta
+            bmu = beta * mu
+        else:
+            bmu = np.log(p * lambda3 * 1e-30 / (BOLTZ * temperature))
+        i = np.arange(0, maxn - minn + 1)
+        zt = lng[i] + bmu * (i - minn)
+        k = np.arange(1, maxn)
+        nt = np.log(k) + lng[k] + bmu * k
+        lz = zt.max() + np.log(np.exp(zt - zt.max()).sum())
+        ln = nt.max() + np.log(np.exp(nt - nt.max()).sum())
+        out.append(np.exp(ln - lz))
+    return np.array(out)
+
+
+def reference_pressures():
+    """compute_isotherm's pressure grid: (ip - 1 + 1e-5) * 0.1 bar, ip = 1..51, in Pa."""
+    return np.array([((ip - 1) + 1e-5) * 1e-1 * 1e5 for ip in range(1, 52)])
