@@ -14,7 +14,7 @@
 #include <map>
 #include <vector>
 
-#include "fmc_kernels.hpp"
+#include "fmc_grad.hpp"
 
 using namespace fmc;
 
@@ -66,6 +66,17 @@ struct fmc_handle {
   std::vector<int> kperm;        // internal k -> reference enumeration index
   size_t lds_base_end = 0;       // LDS bytes of the always-staged part of the plan
   int wl_maxn = 0, wl_minn = 0, wl_nwin = 0;  // Wang-Landau set-up (0: off)
+  // energy-gradient entry: framework types and frozen flags, FIELD LJ rows
+  std::vector<int> fw_type, fw_frz;
+  std::vector<double> pair_eps, pair_sig;   // per FIELD row
+  std::vector<int> pair_row;                // [KTYPE+1][KTYPE+1] -> FIELD row or -1
+  double rcut = 0.0;
+  // its device workspace (grown on demand)
+  float2 *g_efw32 = nullptr;
+  double2 *g_efw64 = nullptr, *g_sfw = nullptr, *g_sc = nullptr;
+  int g_cap = 0, g_gcap = 0, g_nptcap = 0, g_efw_ready = 0;
+  int *g_nmols = nullptr, *g_ptab = nullptr, *g_fwt = nullptr;
+  double *g_xyz = nullptr, *g_fwq = nullptr, *g_out = nullptr, *g_vt = nullptr;
 };
 
 namespace {
@@ -484,6 +495,14 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       vvv[(size_t)(p - 1) * P.vstride + i] = 4.0 * peps[p] * sr6 * (sr6 - 1.0);
     }
   }
+  h->pair_eps.assign(peps.begin() + 1, peps.begin() + 1 + npairs);
+  h->pair_sig.assign(psig.begin() + 1, psig.begin() + 1 + npairs);
+  h->pair_row.assign((KTYPE + 1) * (KTYPE + 1), -1);
+  for (int ti = 1; ti <= P.ntypes; ++ti)
+    for (int tj = 1; tj <= P.ntypes; ++tj) {
+      int k = lstvdw[loc2(ti, tj)];
+      if (k >= 1 && k <= npairs && ltpvdw[k] == 2) h->pair_row[ti * (KTYPE + 1) + tj] = k - 1;
+    }
   for (int ti = 0; ti <= KTYPE; ++ti)
     for (int tj = 0; tj <= KTYPE; ++tj) P.pairidx[ti][tj] = -1;
   for (int ti = 1; ti <= P.ntypes; ++ti)
@@ -622,6 +641,9 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   }
   h->selfq2_all_fw = fwq2;
   P.nfw = (int)fwx.size();
+  h->fw_type = fwtype;
+  h->fw_frz = fwfrz;
+  h->rcut = sys->rcut;
   // decision constants: the same expressions as the per-trial code of
   // ewald1_guest (ewald.f:96-105) and mc_insert (gcmc.f:1729)
   P.dc_C = 2.0 * P.rvolm * R4PIE0 / P.epsq;
@@ -953,6 +975,10 @@ int fmc_destroy(fmc_handle *h) {
   for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
   for (void *p : h->allocs) hipFree(p);
   if (h->dtrace) hipFree(h->dtrace);
+  for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
+                  (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
+                  (void *)h->g_vt})
+    if (p) hipFree(p);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1137,6 +1163,10 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   if (trace) {
     if (h->dtrace_cap < nsteps) {
       if (h->dtrace) hipFree(h->dtrace);
+  for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
+                  (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
+                  (void *)h->g_vt})
+    if (p) hipFree(p);
       h->dtrace = nullptr;
       HIPCHK(hipMalloc(&h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps));
       h->dtrace_cap = nsteps;
@@ -1462,6 +1492,237 @@ int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, d
     for (int i = 0; i <= maxn - minn; ++i) zs += std::exp(zt[i] - zmax);
     for (int k = 1; k < maxn; ++k) ns += std::exp(nt[k] - nmax);
     n_out[ip] = maxn > 1 ? std::exp(std::log(ns) + nmax - std::log(zs) - zmax) : 0.0;
+  }
+  return FMC_OK;
+}
+
+int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int *nmols, const double *xyz,
+                    int gstride, int precision, double *U, double *dU_dq, double *dU_deps, double *dU_dsig) {
+  if (!h || !theta || !theta->fw_charge || !theta->type_eps || !theta->type_sig || nconf < 1 || !nmols || !xyz ||
+      gstride < 1 || (precision != FMC_FP64 && precision != FMC_MIXED))
+    return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  const int nfw = P.nfw, nk = P.nk, ng = P.ntpguest, nt = P.ntypes, TT = KTYPE + 1;
+  for (int a = 0; a < nfw; ++a)
+    if (!h->fw_frz[a]) return FMC_ERR_ARG;  // framework-framework pairs are left out: a rigid, frozen framework only
+  int gmax = 0;
+  for (int c = 0; c < nconf; ++c) {
+    int n = 0;
+    for (int g = 1; g <= ng; ++g) {
+      if (nmols[c * ng + g - 1] < 0) return FMC_ERR_ARG;
+      n += nmols[c * ng + g - 1] * P.natoms[P.locguest[g]];
+    }
+    if (n > gstride) return FMC_ERR_ARG;
+    gmax = std::max(gmax, n);
+  }
+  for (int t = 1; t <= nt; ++t)
+    if (!(theta->type_eps[t - 1] >= 0.0) || !(theta->type_sig[t - 1] > 0.0)) return FMC_ERR_ARG;
+  std::vector<int> isfw(TT, 0);
+  for (int a = 0; a < nfw; ++a) isfw[h->fw_type[a]] = 1;
+  // ---- LJ tables of the theta pairs (forgen, vdw_terms.f:222-234, and their
+  // derivatives with respect to eps and sigma at the same grid points)
+  const int vs = P.vstride, mx = P.mxegrd;
+  std::vector<int> ptab(TT * TT, -1);
+  std::vector<double> vt, peps_t, psig_t;
+  for (int i = 1; i <= nt; ++i)
+    for (int j = 1; j <= i; ++j) {
+      if (!isfw[i] && !isfw[j]) continue;
+      double e = std::sqrt(theta->type_eps[i - 1] * theta->type_eps[j - 1]);
+      double sg = 0.5 * (theta->type_sig[i - 1] + theta->type_sig[j - 1]);
+      size_t o = vt.size();
+      vt.resize(o + 3 * (size_t)vs, 0.0);
+      for (int r = 1; r <= mx; ++r) {
+        double rrr = (double)r * P.dlrpot, x = sg / rrr, x2 = x * x, x6 = x2 * x2 * x2;
+        vt[o + r] = 4.0 * e * x6 * (x6 - 1.0);
+        vt[o + vs + r] = 4.0 * x6 * (x6 - 1.0);
+        vt[o + 2 * vs + r] = 4.0 * e * (12.0 * x6 * x6 - 6.0 * x6) / sg;
+      }
+      int id = (int)peps_t.size();
+      peps_t.push_back(e);
+      psig_t.push_back(sg);
+      // srfrce skip rule, vdw_terms.f:431; -2 marks a theta pair without LJ
+      int v = std::fabs(vt[o + 1]) > 1e-10 ? id : -2;
+      ptab[i * TT + j] = v;
+      ptab[j * TT + i] = v;
+    }
+  const int npt = (int)peps_t.size();
+  // ---- device workspace
+  const size_t NTT = (size_t)TT * TT;
+  const size_t outn = (size_t)nconf * (1 + nfw + 2 + 2 * NTT) + nfw;
+  if (!h->g_efw32) {
+    HIPCHK(hipMalloc(&h->g_efw32, sizeof(float2) * (size_t)nk * nfw));
+    HIPCHK(hipMalloc(&h->g_efw64, sizeof(double2) * (size_t)nk * nfw));
+    HIPCHK(hipMalloc(&h->g_sfw, sizeof(double2) * nk));
+    HIPCHK(hipMalloc(&h->g_fwq, sizeof(double) * std::max(nfw, 1)));
+    HIPCHK(hipMalloc(&h->g_fwt, sizeof(int) * std::max(nfw, 1)));
+    HIPCHK(hipMalloc(&h->g_ptab, sizeof(int) * NTT));
+    HIPCHK(hipMemcpy(h->g_fwt, h->fw_type.data(), sizeof(int) * nfw, hipMemcpyHostToDevice));
+  }
+  if (nconf > h->g_cap || gstride > h->g_gcap) {
+    if (h->g_sc) hipFree(h->g_sc);
+    if (h->g_nmols) hipFree(h->g_nmols);
+    if (h->g_xyz) hipFree(h->g_xyz);
+    if (h->g_out) hipFree(h->g_out);
+    h->g_cap = std::max(nconf, h->g_cap);
+    h->g_gcap = std::max(gstride, h->g_gcap);
+    HIPCHK(hipMalloc(&h->g_sc, sizeof(double2) * (size_t)h->g_cap * nk));
+    HIPCHK(hipMalloc(&h->g_nmols, sizeof(int) * (size_t)h->g_cap * ng));
+    HIPCHK(hipMalloc(&h->g_xyz, sizeof(double) * 3 * (size_t)h->g_cap * h->g_gcap));
+    HIPCHK(hipMalloc(&h->g_out, sizeof(double) * ((size_t)h->g_cap * (1 + nfw + 2 + 2 * NTT) + nfw)));
+  }
+  if (npt > h->g_nptcap) {
+    if (h->g_vt) hipFree(h->g_vt);
+    h->g_nptcap = npt;
+    HIPCHK(hipMalloc(&h->g_vt, sizeof(double) * 3 * (size_t)vs * std::max(npt, 1)));
+  }
+  GradArgs A{};
+  A.nconf = nconf;
+  A.nk = nk;
+  A.nfw = nfw;
+  A.gstride = gstride;
+  A.ntpguest = ng;
+  A.nmols = h->g_nmols;
+  A.gx = h->g_xyz;
+  A.fwq = h->g_fwq;
+  A.fx = P.fw_per_walker ? h->dw.fwx : P.fwx;
+  A.fy = P.fw_per_walker ? h->dw.fwy : P.fwy;
+  A.fz = P.fw_per_walker ? h->dw.fwz : P.fwz;
+  A.fwt = h->g_fwt;
+  A.efw32 = h->g_efw32;
+  A.efw64 = h->g_efw64;
+  A.sfw = h->g_sfw;
+  A.sc = h->g_sc;
+  A.urec = h->g_out;
+  A.dq = A.urec + nconf;
+  A.e2 = A.dq + (size_t)nconf * nfw;
+  A.ev = A.e2 + nconf;
+  A.dpe = A.ev + nconf;
+  A.dps = A.dpe + (size_t)nconf * NTT;
+  A.dqfw = A.dps + (size_t)nconf * NTT;
+  A.vt = h->g_vt;
+  A.ptab = h->g_ptab;
+  A.vstride = vs;
+  A.npt = npt;
+  HIPCHK(hipMemcpyAsync(h->g_nmols, nmols, sizeof(int) * (size_t)nconf * ng, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->g_xyz, xyz, sizeof(double) * 3 * (size_t)nconf * gstride, hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipMemcpyAsync(h->g_fwq, theta->fw_charge, sizeof(double) * nfw, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->g_ptab, ptab.data(), sizeof(int) * NTT, hipMemcpyHostToDevice, h->stream));
+  if (npt)
+    HIPCHK(hipMemcpyAsync(h->g_vt, vt.data(), sizeof(double) * vt.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemsetAsync(h->g_out, 0, sizeof(double) * outn, h->stream));
+  if (!h->g_efw_ready) {
+    hipLaunchKernelGGL(k_grad_phases, dim3((nfw + 255) / 256, nk), dim3(256), 0, h->stream, h->dp, A);
+    h->g_efw_ready = 1;
+  }
+  hipLaunchKernelGGL(k_grad_sfw, dim3(nk), dim3(256), 0, h->stream, A);
+  hipLaunchKernelGGL(k_grad_sconf, dim3((nk + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A);
+  hipLaunchKernelGGL((k_grad_dq_recip<double>), dim3((nfw + 255) / 256, 1), dim3(256), 0, h->stream, h->dp, A, 1);
+  if (precision == FMC_MIXED)
+    hipLaunchKernelGGL((k_grad_dq_recip<float>), dim3((nfw + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A, 0);
+  else
+    hipLaunchKernelGGL((k_grad_dq_recip<double>), dim3((nfw + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A, 0);
+  hipLaunchKernelGGL(k_grad_pairs, dim3((nfw + gmax + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A);
+  HIPCHK(hipGetLastError());
+  std::vector<double> out(outn);
+  HIPCHK(hipMemcpyAsync(out.data(), h->g_out, sizeof(double) * outn, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double *urec = out.data(), *dq = urec + nconf, *e2 = dq + (size_t)nconf * nfw, *ev = e2 + nconf;
+  const double *dpe = ev + nconf, *dps = dpe + (size_t)nconf * NTT, *dqfw = dps + (size_t)nconf * NTT;
+  // ---- closed forms (FP64): self term, charged-system term, ewald3, tail
+  const double kq = R4PIE0 / P.epsq, C = P.dc_C, alpha = P.alpha, volm = P.volm, twopi = 2.0 * PI;
+  const double rc = h->rcut, rc3 = rc * rc * rc, rc9 = rc3 * rc3 * rc3;
+  double q2fw = 0.0, qfw = 0.0;
+  for (int a = 0; a < nfw; ++a) {
+    q2fw += theta->fw_charge[a] * theta->fw_charge[a];
+    qfw += theta->fw_charge[a];
+  }
+  std::vector<int> nfwt(TT, 0), ffwt(TT, 0);
+  for (int a = 0; a < nfw; ++a) {
+    nfwt[h->fw_type[a]] += 1;
+    ffwt[h->fw_type[a]] += h->fw_frz[a] ? 1 : 0;
+  }
+  for (int c = 0; c < nconf; ++c) {
+    double q2 = q2fw, qs = qfw, e3 = 0.0;
+    std::vector<double> n(TT, 0.0), f(TT, 0.0);
+    for (int t = 0; t < TT; ++t) {
+      n[t] = nfwt[t];
+      f[t] = ffwt[t];
+    }
+    for (int g = 1; g <= ng; ++g) {
+      int mol = P.locguest[g], nm = nmols[c * ng + g - 1];
+      q2 += nm * P.selfq2[g];
+      qs += nm * P.chgtmp[g];
+      e3 += nm * P.ewald3en[mol];
+      for (int i = 0; i < P.natoms[mol]; ++i) {
+        n[P.site_t[mol][i]] += nm;
+        if (P.site_f[mol][i]) f[P.site_t[mol][i]] += nm;
+      }
+    }
+    const double engsic = -kq * alpha * q2 / SQRPI;
+    const double qfix = -(0.5 * PI * kq) * ((qs / alpha) * (qs / alpha) / volm);
+    // tail (lrcorrect, vdw_terms.f:581-645) with theta pairs mixed
+    double elrc = 0.0;
+    std::vector<double> gte(TT, 0.0), gts(TT, 0.0);
+    for (int i = 1; i <= nt; ++i)
+      for (int j = 1; j <= i; ++j) {
+        double w = twopi * (n[i] * n[j] - f[i] * f[j]) / volm;
+        double two = i != j ? 2.0 : 1.0;
+        if (isfw[i] || isfw[j]) {
+          double e = std::sqrt(theta->type_eps[i - 1] * theta->type_eps[j - 1]);
+          double sg = 0.5 * (theta->type_sig[i - 1] + theta->type_sig[j - 1]);
+          double s2 = sg * sg, s6 = s2 * s2 * s2;
+          double shape = 4.0 * (s6 * s6 / (9.0 * rc9) - s6 / (3.0 * rc3));
+          elrc += w * two * e * shape;
+          double deadd_de = w * two * shape;
+          double deadd_ds = w * two * 4.0 * e * (12.0 * s6 * s6 / sg / (9.0 * rc9) - 6.0 * s6 / sg / (3.0 * rc3));
+          // Lorentz-Berthelot chain rule to the type parameters
+          if (i == j) {
+            gte[i] += deadd_de;
+            gts[i] += deadd_ds;
+          } else {
+            if (theta->type_eps[i - 1] > 0.0) gte[i] += deadd_de * e / (2.0 * theta->type_eps[i - 1]);
+            if (theta->type_eps[j - 1] > 0.0) gte[j] += deadd_de * e / (2.0 * theta->type_eps[j - 1]);
+            gts[i] += 0.5 * deadd_ds;
+            gts[j] += 0.5 * deadd_ds;
+          }
+        } else {
+          int k = h->pair_row[i * TT + j];
+          if (k >= 0) {
+            double s2 = h->pair_sig[k] * h->pair_sig[k], s6 = s2 * s2 * s2;
+            elrc += w * two * 4.0 * h->pair_eps[k] * (s6 * s6 / (9.0 * rc9) - s6 / (3.0 * rc3));
+          }
+        }
+      }
+    // pair LJ gradients, chain rule to the type parameters
+    for (int i = 1; i <= nt; ++i)
+      for (int j = 1; j <= i; ++j) {
+        double de = dpe[(size_t)c * NTT + i * TT + j], ds = dps[(size_t)c * NTT + i * TT + j];
+        if (de == 0.0 && ds == 0.0) continue;
+        double e = std::sqrt(theta->type_eps[i - 1] * theta->type_eps[j - 1]);
+        if (i == j) {
+          gte[i] += de;
+          gts[i] += ds;
+        } else {
+          if (theta->type_eps[i - 1] > 0.0) gte[i] += de * e / (2.0 * theta->type_eps[i - 1]);
+          if (theta->type_eps[j - 1] > 0.0) gte[j] += de * e / (2.0 * theta->type_eps[j - 1]);
+          gts[i] += 0.5 * ds;
+          gts[j] += 0.5 * ds;
+        }
+      }
+    const double eu = P.engunit;
+    if (U) U[c] = (C * urec[c] + engsic + qfix + e2[c] - e3 + ev[c] + elrc) / eu;
+    if (dU_dq) {
+      const double dfix = -(PI * kq) * qs / (alpha * alpha * volm);
+      for (int a = 0; a < nfw; ++a) {
+        double dself = -2.0 * kq * alpha * theta->fw_charge[a] / SQRPI;
+        dU_dq[(size_t)c * nfw + a] = (dqfw[a] + dq[(size_t)c * nfw + a] + dself + dfix) / eu;
+      }
+    }
+    for (int t = 1; t <= nt; ++t) {
+      if (dU_deps) dU_deps[(size_t)c * nt + t - 1] = isfw[t] ? gte[t] / eu : 0.0;
+      if (dU_dsig) dU_dsig[(size_t)c * nt + t - 1] = isfw[t] ? gts[t] / eu : 0.0;
+    }
   }
   return FMC_OK;
 }

@@ -10,8 +10,8 @@ import os
 
 import numpy as np
 
-from .system import (CSystem, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams, FmcWlStatus,
-                     TRACE_DTYPE, System, make_walker)
+from .system import (CSystem, FmcTheta, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams,
+                     FmcWlStatus, TRACE_DTYPE, System, make_walker)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
@@ -24,7 +24,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
-           "fmc_wl_result", "fmc_wl_isotherm")
+           "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad")
 
 _lib = None
 
@@ -182,6 +182,32 @@ class Handle:
 
     def reset_stats(self):
         _chk("fmc_reset_stats", self.L.fmc_reset_stats(self.h))
+
+    # ---- energy gradient with respect to framework parameters
+    def energy_grad(self, fw_charge, type_eps, type_sig, nmols, xyz, precision: int = FMC_FP64,
+                    want_grad: bool = True):
+        """U[nconf] and dU/dq[nconf][nfw], dU/deps, dU/dsig [nconf][ntypes] (FIELD units).
+
+        nmols: [nconf][ntpguest] molecule counts; xyz: [nconf][gstride][3] guest sites."""
+        q = np.ascontiguousarray(fw_charge, dtype=np.float64)
+        e = np.ascontiguousarray(type_eps, dtype=np.float64)
+        sg = np.ascontiguousarray(type_sig, dtype=np.float64)
+        nm = np.ascontiguousarray(nmols, dtype=np.int32)
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        nconf, gstride = x.shape[0], x.shape[1]
+        nt = len(e)
+        th = FmcTheta(q.ctypes.data_as(C.POINTER(C.c_double)), e.ctypes.data_as(C.POINTER(C.c_double)),
+                      sg.ctypes.data_as(C.POINTER(C.c_double)))
+        U = np.zeros(nconf)
+        dq = np.zeros((nconf, len(q))) if want_grad else None
+        de = np.zeros((nconf, nt)) if want_grad else None
+        ds = np.zeros((nconf, nt)) if want_grad else None
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        _chk("fmc_energy_grad", self.L.fmc_energy_grad(self.h, C.byref(th), C.c_int(nconf),
+                                                       nm.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                                                       C.c_int(gstride), C.c_int(precision), ptr(U), ptr(dq),
+                                                       ptr(de), ptr(ds)))
+        return U, dq, de, ds
 
     # ---- Wang-Landau (wang_landau.f)
     def wl_setup(self, maxn: int, minn: int = 0, nwindows: int = 1, visittol: int = 10000,

@@ -598,6 +598,11 @@ class FmcWalkerStats(C.Structure):
                 ("resum_drift", C.c_double)]
 
 
+class FmcTheta(C.Structure):
+    _fields_ = [("fw_charge", C.POINTER(C.c_double)), ("type_eps", C.POINTER(C.c_double)),
+                ("type_sig", C.POINTER(C.c_double))]
+
+
 class FmcWlParams(C.Structure):
     _fields_ = [("maxn", C.c_int), ("minn", C.c_int), ("nwindows", C.c_int), ("visittol", C.c_int),
                 ("flatcoeff", C.c_double), ("wlprec", C.c_double), ("prectol", C.c_double),

@@ -260,6 +260,30 @@ int fmc_wl_result(fmc_handle *h, double *lng_out);
 int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, double lambda3,
                     const double *p_pa, int np, int units, double *n_out);
 
+/* ---- energy gradient with respect to framework parameters -------------- */
+
+/* theta: the framework site charges and the LJ parameters of the atom types.
+ * Every LJ pair with a framework type is mixed from the type parameters
+ * (Lorentz-Berthelot: eps_ij = sqrt(eps_i eps_j), sig_ij = (sig_i + sig_j)/2);
+ * guest-guest pairs keep their FIELD rows. */
+typedef struct fmc_theta {
+  const double *fw_charge;   /* [nframework] framework site charges, FIELD/CONFIG order */
+  const double *type_eps;    /* [ntypes] LJ epsilon per atom type, internal units (x engunit) */
+  const double *type_sig;    /* [ntypes] LJ sigma per atom type (A) */
+} fmc_theta;
+
+/* Total energy (single_point, gcmc.f:2549-2661) of nconf guest configurations
+ * in the handle's (rigid, frozen) framework, and its gradient with respect to
+ * theta, from one set of device kernels (no counterpart in the reference).
+ * Configuration c: nmols[c*ntpguest + g-1] molecules of guest g; site
+ * coordinates at xyz + 3*gstride*c, guest types in order, molecules in order.
+ * Outputs in FIELD units, any may be NULL: U[nconf], dU_dq[nconf][nframework],
+ * dU_deps[nconf][ntypes], dU_dsig[nconf][ntypes] (guest-type entries are 0).
+ * precision FMC_FP64, or FMC_MIXED: the reciprocal contraction over (atom, k)
+ * in FP32 with FP64 accumulation. */
+int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int *nmols, const double *xyz,
+                    int gstride, int precision, double *U, double *dU_dq, double *dU_deps, double *dU_dsig);
+
 #ifdef __cplusplus
 }
 #endif
