@@ -46,7 +46,25 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="threads per walker (0: autotune)")
+    ap.add_argument("--eager", action="store_true",
+                    help="plain launches instead of graph replays (for counter collection)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (calibrates the HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE per
+    dispatch against 256 MiB is the scale applied to the kernel's counters."""
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "dispatches": 5}), flush=True)
 
 
 def device_sync():
@@ -110,6 +128,48 @@ def reference_cpu(state_dir: str, seconds: float):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def reference_cpu_concurrent(state_dir: str, steps: int, nproc: int):
+    """The reference's parallel model (ranks are independent chains,
+    gcmc.f:388): nproc serial processes at once on the host's cores;
+    aggregate moves/s = sum of steps / slowest wall time."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "gcmc_ser.x")
+    if not os.path.exists(exe) or nproc < 1:
+        return None
+    works, procs = [], []
+    try:
+        for i in range(nproc):
+            work = tempfile.mkdtemp(prefix=f"fmc_cpu{i}_")
+            works.append(work)
+            for f in ("FIELD", "CONFIG", "CONTROL"):
+                shutil.copy(os.path.join(state_dir, f), os.path.join(work, f))
+            ctl = open(os.path.join(work, "CONTROL")).read()
+            ctl = re.sub(r"(?m)^steps .*$", f"steps    {steps}", ctl)
+            ctl = re.sub(r"(?m)^equilibration .*$", "equilibration    0", ctl)
+            open(os.path.join(work, "CONTROL"), "w").write(ctl)
+        t0 = time.time()
+        for work in works:
+            procs.append(subprocess.Popen([exe], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+        for pr in procs:
+            pr.wait(timeout=600)
+        wall = time.time() - t0
+        total = 0
+        for work in works:
+            out = open(os.path.join(work, "OUTPUT")).read()
+            m = re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out)
+            if not m:
+                return None
+            total += int(m.group(1))
+        return {"value": total / wall, "unit": "trial moves/s", "cores": nproc, "kind": "reference",
+                "sample": f"{nproc} concurrent reference serial processes (the MPI build's independent-chain "
+                          f"model), {steps} GCMC steps each on the same state point, {wall:.1f} s wall"}
+    finally:
+        for pr in procs:
+            if pr.poll() is None:
+                pr.kill()
+        for work in works:
+            shutil.rmtree(work, ignore_errors=True)
+
+
 def oracle_cpu(sysm, seconds: float):
     from oracle.oracle import Oracle
     o = Oracle(sysm, seed=(31, 41, 59, 26))
@@ -126,6 +186,9 @@ def oracle_cpu(sysm, seconds: float):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
               "(state points shard across GPUs with no per-trial communication; see DESIGN.md)",
@@ -146,19 +209,34 @@ def main():
                                        temperature=s.temperature, pressures=s.pressures,
                                        framework_xyz=s.framework_xyz(), keep=keep))
         h = Handle(base, walkers=walkers, precision=FMC_MIXED)
-        h.set_block(0)  # autotune threads per walker on a snapshot
+        h.set_block(args.block)  # 0: autotune threads per walker on a snapshot
         block = h.get_block()
-        # warm-up: fills each state point to its loading, then captures the graph
-        for _ in range(max(1, args.warmup // 10)):
-            h.run_graph(10, args.moves)
-        h.run_graph(args.steps, args.moves)   # capture + one untimed replay of the timed graph
-        h.reset_stats()
-        device_sync()
-        t0 = time.perf_counter()
-        h.run_graph(args.steps, args.moves, count_stats=True)
-        device_sync()
-        wall = time.perf_counter() - t0
+        if args.eager:
+            for _ in range(args.warmup):
+                h.run(args.moves)
+            h.reset_stats()
+            device_sync()
+            t0 = time.perf_counter()
+            ms_tot = 0.0
+            for _ in range(args.steps):
+                h.run(args.moves, count_stats=True)
+                ms_tot += h.last_timing()[0]
+            device_sync()
+            wall = time.perf_counter() - t0
+        else:
+            # warm-up: fills each state point to its loading, then captures the graph
+            for _ in range(max(1, args.warmup // 10)):
+                h.run_graph(10, args.moves)
+            h.run_graph(args.steps, args.moves)   # capture + one untimed replay of the timed graph
+            h.reset_stats()
+            device_sync()
+            t0 = time.perf_counter()
+            h.run_graph(args.steps, args.moves, count_stats=True)
+            device_sync()
+            wall = time.perf_counter() - t0
         gpu_ms, launches = h.last_timing()
+        if args.eager:
+            gpu_ms, launches = ms_tot, args.steps
         moves = args.walkers * args.moves * args.steps
         value = moves / wall
         # guest sites present on average during the timed sweeps
@@ -177,7 +255,8 @@ def main():
         pmc = os.path.join(ROOT, "profiles", "k_run_batched_isotherm_counters.json")
         if os.path.exists(pmc):
             try:
-                traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+                per_trial = json.load(open(pmc)).get("hbm_bytes_per_trial")
+                traffic = per_trial * args.walkers * args.moves if per_trial else None
             except Exception:
                 traffic = None
         import math
@@ -201,11 +280,16 @@ def main():
                               f"P={mid.pressures[0] / 1e5:.4g} bar), same device kernel, 1 walker",
                     "threads_per_walker": hs.get_block()}
             hs.close()
-        cpu = None
+        cpu = cpu_mpi = None
         if not args.no_cpu:
             cpu = reference_cpu(dirs[mid_i], args.cpu_seconds)
             if cpu is None:
                 cpu = oracle_cpu(systems[mid_i], args.cpu_seconds)
+            elif cpu["kind"] == "reference":
+                # the host's CPU share for one GPU, at most 16 cores
+                nproc = max(1, min(16, os.cpu_count() or 1))
+                steps = int(cpu["value"] * args.cpu_seconds * 0.5)
+                cpu_mpi = reference_cpu_concurrent(dirs[mid_i], steps, nproc)
         h.close()
         line = {
             "metric": METRIC, "value": value, "unit": "trial moves/s", "n_gpus": 1,
@@ -220,9 +304,10 @@ def main():
                        "guests_min_max": [nmin, nmax]},
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK, "traffic": traffic,
-                         "kernel": "k_run<2,float>", "bytes_per_trial": B,
+                         "kernel": "k_run", "bytes_per_trial": B,
                          "avg_launch_ms": launch_s * 1e3},
             "cpu_baseline": cpu,
+            "cpu_baseline_concurrent": cpu_mpi,
             "fp64_single": fp64,
             "batched_vs_fp64_single": (value / fp64["value"]) if fp64 else None,
         }

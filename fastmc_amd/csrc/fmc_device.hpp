@@ -192,17 +192,26 @@ __device__ inline void mzt_advance(WState &w) {
   w.mz_made += 33;
 }
 
-// images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half away)
+// A value every lane of the wave holds (read from LDS, so the compiler cannot
+// tell): taken from the first active lane, so that what it indexes is read
+// through the scalar unit instead of one vector load per lane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half
+// away).  In the orthogonal cells the nearest integer is taken with rint
+// (one instruction instead of six): the two differ only when a * dx is an
+// exact half-integer, and then both images lie at distance cell / 2, so the
+// squared distance is the same.  The triclinic cell keeps nint.
 __device__ inline void min_image(const DevParams &P, double &dx, double &dy, double &dz) {
   if (P.imcon == 1) {
     double a = P.inv_cell[0];
-    dx = dx - P.cell[0] * round(a * dx);
-    dy = dy - P.cell[0] * round(a * dy);
-    dz = dz - P.cell[0] * round(a * dz);
+    dx = dx - P.cell[0] * rint(a * dx);
+    dy = dy - P.cell[0] * rint(a * dy);
+    dz = dz - P.cell[0] * rint(a * dz);
   } else if (P.imcon == 2) {
-    dx = dx - P.cell[0] * round(P.inv_cell[0] * dx);
-    dy = dy - P.cell[4] * round(P.inv_cell[1] * dy);
-    dz = dz - P.cell[8] * round(P.inv_cell[2] * dz);
+    dx = dx - P.cell[0] * rint(P.inv_cell[0] * dx);
+    dy = dy - P.cell[4] * rint(P.inv_cell[1] * dy);
+    dz = dz - P.cell[8] * rint(P.inv_cell[2] * dz);
   } else {
     const double *r = P.rcell, *c = P.cell;
     double sx = r[0] * dx + r[3] * dy + r[6] * dz;
@@ -215,6 +224,24 @@ __device__ inline void min_image(const DevParams &P, double &dx, double &dy, dou
     dy = c[1] * sx + c[4] * sy + c[7] * sz;
     dz = c[2] * sx + c[5] * sy + c[8] * sz;
   }
+}
+
+// Square root of a squared pair distance: the correctly rounded FP64 sequence
+// the compiler emits (reciprocal square root, one Goldschmidt step, two
+// residual corrections) without its rescaling of tiny or huge arguments and
+// its zero / infinity fix-up, which pair distances never need (x is clamped
+// to 1e-200; an exactly coincident pair is an overlap, never accepted).
+__device__ inline double pair_sqrt(double x) {
+  x = x > 1e-200 ? x : 1e-200;
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double e = fma(-h, g, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
 }
 
 // 3-point interpolation of ewald.f:703-715 / vdw_terms.f:441-453
@@ -254,6 +281,51 @@ __device__ inline double wave_sum_dpp(double v) {
   s += dpp_f64<0x142, 0xa, 0xf>(s);             // row_bcast:15 into rows 1, 3
   s += dpp_f64<0x143, 0xc, 0xf>(s);             // row_bcast:31 into rows 2, 3
   return s;
+}
+
+// One halving step of wave_sum_rows: lanes exchange half of the C values
+// they carry with their DPP partner (CTRL, an involution within each row of
+// 16 lanes that flips lane bit `up`), keep the other half and add.
+template <int C, int CTRL>
+__device__ inline void wave_halve(double *v, bool up) {
+  constexpr int H = C / 2;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double send = up ? v[i] : v[i + H];
+    const double keep = up ? v[i + H] : v[i];
+    v[i] = keep + dpp_f64<CTRL, 0xf, 0xf>(send);
+  }
+}
+
+// Sums of N per-lane values over the 64 lanes of a wave, all N at once by
+// recursive halving: row mirror, half-row mirror, quad xor 2 and quad xor 1
+// each halve the values a lane carries (K = N rounded up to 16 or 32, J =
+// K / 16 left per lane), so that lane L holds its row's partial sums of
+// values J * (L & 15) + j; two exchanges across rows (xor 16, xor 32) finish
+// them.  Lanes 0..15 write the totals to out[0..N).  About K adds and K
+// exchanges per lane instead of 6 N of each for N separate reductions.
+template <int N>
+__device__ inline void wave_sum_rows(const double *acc, int lane, double *out) {
+  static_assert(N >= 1 && N <= 32, "wave_sum_rows: 1..32 values");
+  constexpr int K = N <= 16 ? 16 : 32;
+  constexpr int J = K / 16;
+  double v[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) v[i] = i < N ? acc[i] : 0.0;
+  wave_halve<K, 0x140>(v, (lane >> 3) & 1);      // row_mirror: r <-> 15 - r
+  wave_halve<K / 2, 0x141>(v, (lane >> 2) & 1);  // row_half_mirror: r <-> 7 - r within 8
+  wave_halve<K / 4, 0x4E>(v, (lane >> 1) & 1);   // quad_perm [2,3,0,1]
+  wave_halve<K / 8, 0xB1>(v, lane & 1);          // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    v[j] += __shfl_xor(v[j], 16, 64);
+    v[j] += __shfl_xor(v[j], 32, 64);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (J * lane + j < N) out[J * lane + j] = v[j];
+  }
 }
 
 }  // namespace fmc

@@ -29,6 +29,15 @@ struct Smem {
   double estep, rande, delrc;
   double delE[KMAXS + 1], delrc_mol[KMAXC + 1];
   double comp[4];
+  // reciprocal cell and kmax per axis for lane-indexed access (from DevParams)
+  double rc9[9];
+  int kmx[3];
+  // per species: site charges (as ewald2 uses them and bare); per guest kind:
+  // species, sites, first guest slot.  Copies of DevParams entries, so that a
+  // step reads them with independent LDS loads instead of chained scalar loads.
+  double scq[KMAXS + 1][KSITE], sq[KMAXS + 1][KSITE];
+  double ovl2;
+  int kmol[KMAXS + 1], kna[KMAXS + 1], kslot[KMAXS + 1];
   // constants of the current trial, computed while the energy terms run
   double qfix0, pref;
   double qfm[KMAXS + 1];
@@ -80,6 +89,7 @@ __device__ inline void store_slots(const DevParams &P, const DevWalkers &W, int 
 template <int NS>
 __device__ double gstlrcorrect(const DevParams &P, const WState &ws, int g, int lexisting,
                                double *delrc_mol) {
+  g = uni(g);
   int mol = P.locguest[g];
   double delrc = P.tl_c0[g][lexisting];
   for (int gg = 1; gg <= P.ntpguest; ++gg) delrc += P.tl_c1[g][lexisting][gg] * (double)ws.nmols[P.locguest[gg]];
@@ -274,7 +284,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   } else {
     // the grand-canonical loop draws the guest type only when there is a
     // choice (gcmc.f:806-808); the Wang-Landau loop always draws (wang_landau.f:137)
-    if (P.ntpguest > 1 || ws.wl_on) g = (int)floor(duni(ws) * P.ntpguest) + 1;
+    if (P.ntpguest > 1 || ws.wl_on) g = uni((int)floor(duni(ws) * P.ntpguest) + 1);
     const int molg = P.locguest[g];
     double randmov = duni(ws);
     const float *f = ws.mcf[g];
@@ -446,7 +456,7 @@ __device__ inline void pair_eval(const DevParams &P, const double *erc, const T 
   double rsq = dx * dx + dy * dy + dz * dz;
   ovl |= rsq < ovl2;
   bool in = rsq < P.rcut2;
-  double r = sqrt(rsq);
+  double r = pair_sqrt(rsq);
   const int lmax = P.mxegrd;
   double x = r * P.rdrewd;
   int l = min((int)x, lmax);
@@ -587,7 +597,7 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
 template <int NS>
 __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
   const WState &ws = sh.ws;
-  const int g = sh.g, mol = sh.mol, kind = sh.kind;
+  const int g = uni(sh.g), mol = uni(sh.mol), kind = uni(sh.kind);
   const double alpha = P.alpha, volm = P.volm;
   const double chg = P.chgtmp[g];
   sh.qfix0 = -(0.5 * PI * R4PIE0 / P.epsq) * ((chg * ws.sumchg) / alpha + (chg / alpha) * (chg / alpha)) / volm;
@@ -621,28 +631,41 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   constexpr bool LV = LF && sizeof(T) == 4;
   constexpr int NSR = NSITE > 0 ? NSITE : KSITE;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int natms = sh.natms, mol = sh.mol, g = sh.g;
-  const int has_new = sh.has_new, has_old = sh.has_old;
-  if (tid == nt - 1) prepare<NS>(P, sh);
-  // per-site 1-D phase tables, built by the reference recursion (ewald.f:108-147)
-  for (int t = tid; t < 2 * P.nsmax * 3; t += nt) {
-    int cfg = t / (P.nsmax * 3), rem = t % (P.nsmax * 3), s = rem / 3, ax = rem % 3;
-    if (s >= natms || (cfg == 0 && !has_new) || (cfg == 1 && !has_old)) continue;
-    const double *x = cfg == 0 ? sh.xn[s] : sh.xo[s];
-    double fr = P.rcell[ax] * x[0] + P.rcell[3 + ax] * x[1] + P.rcell[6 + ax] * x[2];
-    int kmax = ax == 0 ? P.kmax1 : (ax == 1 ? P.kmax2 : P.kmax3);
-    double s1, c1;
-    sincospi(2.0 * fr, &s1, &c1);
-    double2 *row = ph_site(lds, P, cfg, s) + ax * P.kcp;
-    row[0] = make_double2(1.0, 0.0);
-    row[1] = make_double2(c1, s1);
-    double c = c1, sn = s1;
-    for (int l = 2; l <= kmax; ++l) {
-      double cn = c * c1 - sn * s1;
-      double snn = sn * c1 + c * s1;
-      c = cn;
-      sn = snn;
-      row[l] = make_double2(c, sn);
+  const int natms = uni(sh.natms), mol = uni(sh.mol), g = uni(sh.g);
+  const int has_new = uni(sh.has_new), has_old = uni(sh.has_old);
+  // The trial constants (prepare) are needed only by the decision and the
+  // phase tables only by the reciprocal terms: both are built while the pair
+  // terms run, and the barrier that publishes the tables sits before the
+  // reciprocal loop.  The table rows are spread over the waves, a few lanes
+  // each, so that no wave carries them all.
+  const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
+  if (tid == (nwaves >> 1) * 64 + 63) prepare<NS>(P, sh);
+  {
+    constexpr int NSC = NSITE > 0 ? NSITE : 0;
+    const int nsr = NSITE > 0 ? NSC : P.nsmax;
+    const int ntask = 2 * nsr * 3, tpw = (ntask + nwaves - 1) / nwaves;
+    const int t = wave * tpw + lane;
+    if (lane < tpw && t < ntask) {
+      // per-site 1-D phase tables, built by the reference recursion (ewald.f:108-147)
+      const int cfg = t / (nsr * 3), rem = t - cfg * (nsr * 3), s = rem / 3, ax = rem - 3 * s;
+      if (s < natms && (cfg == 0 ? has_new : has_old)) {
+        const double *x = cfg == 0 ? sh.xn[s] : sh.xo[s];
+        double fr = sh.rc9[ax] * x[0] + sh.rc9[3 + ax] * x[1] + sh.rc9[6 + ax] * x[2];
+        int kmax = sh.kmx[ax];
+        double s1, c1;
+        sincospi(2.0 * fr, &s1, &c1);
+        double2 *row = ph_site(lds, P, cfg, s) + ax * P.kcp;
+        row[0] = make_double2(1.0, 0.0);
+        row[1] = make_double2(c1, s1);
+        double c = c1, sn = s1;
+        for (int l = 2; l <= kmax; ++l) {
+          double cn = c * c1 - sn * s1;
+          double snn = sn * c1 + c * s1;
+          c = cn;
+          sn = snn;
+          row[l] = make_double2(c, sn);
+        }
+      }
     }
   }
   double acc[A::NACC];
@@ -662,31 +685,38 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   constexpr int NSA = NSITE > 0 ? NSITE : 1;
   double xs[2][NSA][3], cq[NSA], ov2[NSA], qs[NSA];
   if constexpr (NSITE > 0) {
+    const double ovl2 = sh.ovl2;
 #pragma unroll
     for (int s = 0; s < NSITE; ++s) {
-      bool real = s < natms;
-      cq[s] = real ? P.site_chgea[mol][s] : 0.0;
-      ov2[s] = real ? P.overlap2 : -1.0;
-      qs[s] = real ? P.site_q[mol][s] : 0.0;
+      const bool real = s < natms;
+      const double c = sh.scq[mol][s], q = sh.sq[mol][s];
+      cq[s] = real ? c : 0.0;
+      ov2[s] = real ? ovl2 : -1.0;
+      qs[s] = real ? q : 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        xs[0][s][d] = real ? sh.xn[s][d] : 0.0;
-        xs[1][s][d] = real ? sh.xo[s][d] : 0.0;
+        const double a = sh.xn[s][d], b = sh.xo[s][d];
+        xs[0][s][d] = real ? a : 0.0;
+        xs[1][s][d] = real ? b : 0.0;
       }
     }
   }
-  // atom index space: framework atoms, then the guests of each kind
-  int nact[KMAXS + 1];
+  // atom index space: framework atoms, then the guests of each kind (per
+  // guest kind: first index, species, sites, first slot; all uniform)
+  int gstart[KMAXS + 1], gmol[KMAXS + 1], gna[KMAXS + 1], gslot[KMAXS + 1];
   int ntot = P.nfw;
 #pragma unroll
   for (int gg = 1; gg <= KMAXS; ++gg) {
-    nact[gg] = 0;
-    if (gg <= P.ntpguest) {
-      int molg = P.locguest[gg];
-      nact[gg] = sh.ws.nmols[molg] * P.natoms[molg];
-      ntot += nact[gg];
-    }
+    const int km = uni(sh.kmol[gg]), kn = uni(sh.kna[gg]), kslt = uni(sh.kslot[gg]);
+    const int nm = uni(sh.ws.nmols[km]);
+    const bool real = gg <= P.ntpguest;
+    gstart[gg] = real ? ntot : 0x7fffffff;
+    gmol[gg] = km;
+    gna[gg] = kn;
+    gslot[gg] = kslt;
+    ntot += real ? nm * kn : 0;
   }
+  const int excl_lo = uni(sh.excl_lo), excl_hi = uni(sh.excl_hi);
   const GuestXYZ<LF> G(P, W, w, lds);
   const double *fx, *fy, *fz, *fq;
   const int *fi;
@@ -703,7 +733,6 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     fq = P.fwq;
     fi = P.fwinfo;
   }
-  __syncthreads();  // phase tables written
   // ---- pair terms
   for (int j = tid; j < ntot; j += nt) {
     double xj, yj, zj, qj;
@@ -718,21 +747,27 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       tj = inf & 255;
       spj = (inf >> 8) & 255;
     } else {
-      int jj = j - P.nfw, gg = 1;
-      while (gg < P.ntpguest && jj >= nact[gg]) {
-        jj -= nact[gg];
-        ++gg;
-      }
-      int molg = P.locguest[gg], na = P.natoms[molg];
+      // the guest kind holding index j, selected without lane-indexed loads
+      int gg = 1, start = gstart[1], molg = gmol[1], na = gna[1], slot = gslot[1];
+#pragma unroll
+      for (int k = 2; k <= KMAXS; ++k)
+        if (j >= gstart[k]) {
+          gg = k;
+          start = gstart[k];
+          molg = gmol[k];
+          na = gna[k];
+          slot = gslot[k];
+        }
+      int jj = j - start;
       int site = jj % na;
-      int a = P.gbase[gg] + jj;
+      int a = slot + jj;
       xj = G.x[a];
       yj = G.y[a];
       zj = G.z[a];
       qj = lsq[molg * KSITE + site];
       tj = lst[molg * KSITE + site];
       spj = molg;
-      skip = gg == g && jj >= sh.excl_lo && jj < sh.excl_hi;
+      skip = gg == g && jj >= excl_lo && jj < excl_hi;
     }
     if (skip) continue;
     int col[NSR];
@@ -753,8 +788,8 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         if constexpr (NSITE > 0)
           pair_eval<T>(P, erc, vvv, xs[c][s], cq[s], col[s], ov2[s], xj, yj, zj, qj, e2, ev, ov);
         else
-          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], P.site_chgea[mol][s], col[s], P.overlap2, xj, yj,
-                       zj, qj, e2, ev, ov);
+          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, xj, yj, zj, qj,
+                       e2, ev, ov);
         er += e2;
         evd += ev;
       }
@@ -767,6 +802,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       else ovl_old |= ov;
     }
   }
+  __syncthreads();  // phase tables written
   // ---- reciprocal terms
   const int *kl;
   const double *ak;
@@ -861,17 +897,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       scr[P.nk + k] = so;
     }
   }
-  // ---- workgroup reduction: DPP within each wave, then across waves; only
-  // the accumulators of the configurations present are reduced
-  const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
-#pragma unroll
-  for (int i = 0; i < A::NACC; ++i) {
-    bool is_new = (i < 2 * NS) || (i == 4 * NS) || (i >= 4 * NS + 2 && i < 5 * NS + 2);
-    if (is_new ? has_new : has_old) {
-      double v = wave_sum_dpp(acc[i]);
-      if (lane == 63) sh.red[wave][i] = v;
-    }
-  }
+  // ---- workgroup reduction: within each wave by recursive halving (the
+  // accumulators of an absent configuration hold zeros), then across waves
+  wave_sum_rows<A::NACC>(acc, lane, sh.red[wave]);
   unsigned long long bn = __ballot(ovl_new != 0), bo = __ballot(ovl_old != 0);
   if (lane == 0) {
     sh.ovlw[wave][0] = bn != 0ULL;
@@ -879,9 +907,15 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   }
   __syncthreads();
   if (tid < A::NACC) {
-    double v = 0.0;
-    for (int wv = 0; wv < nwaves; ++wv) v += sh.red[wv][tid];
-    sh.tot[tid] = v;
+    // pairwise over the waves (missing waves count as zero)
+    double v[KWAVES];
+#pragma unroll
+    for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][tid] : 0.0;
+#pragma unroll
+    for (int st = KWAVES / 2; st >= 1; st >>= 1)
+#pragma unroll
+      for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
+    sh.tot[tid] = v[0];
   } else if (tid == A::NACC) {
     int a = 0, b = 0;
     for (int wv = 0; wv < nwaves; ++wv) {
@@ -914,7 +948,7 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
     if (draw && ws.wl_on) wl_check(P, ws, wl_view(P, lds));
     return;
   }
-  const int g = sh.g, mol = sh.mol, kind = sh.kind;
+  const int g = uni(sh.g), mol = uni(sh.mol), kind = uni(sh.kind);
   const double *t = sh.tot;
   const double C = P.dc_C, engunit = P.engunit;
   const double engsic0 = P.dc_esic[g], qfix0 = sh.qfix0;
@@ -1103,7 +1137,7 @@ template <int NS, bool LF, bool REG>
 __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                       KSlot<NS> *ks) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int g = sh.g, mol = sh.mol, natms = sh.natms, kind = sh.kind;
+  const int g = uni(sh.g), mol = uni(sh.mol), natms = uni(sh.natms), kind = uni(sh.kind);
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
   const double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
   int kfirst = 0;
@@ -1130,10 +1164,10 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
     ly = lx + P.gatoms_total;
     lz = ly + P.gatoms_total;
   }
-  int nm = sh.ws.nmols[mol];
+  const int nm = uni(sh.ws.nmols[mol]), choice = uni(sh.choice);
   if (kind != FMC_MOVE_DELETE) {
     if (tid < natms) {
-      int a = (kind == FMC_MOVE_INSERT ? nm : sh.choice - 1) * natms + tid;
+      int a = (kind == FMC_MOVE_INSERT ? nm : choice - 1) * natms + tid;
       double x = sh.xn[tid][0], y = sh.xn[tid][1], z = sh.xn[tid][2];
       W.gx[base + a] = x;
       W.gy[base + a] = y;
@@ -1146,7 +1180,7 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
     }
   } else {
     // shift the molecules after `choice` down by one, keeping their order
-    int lo = (sh.choice - 1) * natms, hi = (nm - 1) * natms;
+    int lo = (choice - 1) * natms, hi = (nm - 1) * natms;
     for (int b0 = lo; b0 < hi; b0 += nt) {
       int j = b0 + tid;
       double x = 0, y = 0, z = 0;
@@ -1312,10 +1346,30 @@ __device__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Sme
 }
 
 template <int NS>
-__device__ inline void stage_in(const DevWalkers &W, int w, Smem<NS> &sh) {
+__device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
   const int *src = (const int *)(W.ws + w);
   int *dst = (int *)&sh.ws;
   for (int i = threadIdx.x; i < (int)(sizeof(WState) / 4); i += blockDim.x) dst[i] = src[i];
+  const int t = threadIdx.x;
+  if (t < 9) sh.rc9[t] = P.rcell[t];
+  if (t == 9) {
+    sh.kmx[0] = P.kmax1;
+    sh.kmx[1] = P.kmax2;
+    sh.kmx[2] = P.kmax3;
+    sh.ovl2 = P.overlap2;
+  }
+  for (int i = t; i < (KMAXS + 1) * KSITE; i += blockDim.x) {
+    sh.scq[i / KSITE][i % KSITE] = P.site_chgea[i / KSITE][i % KSITE];
+    sh.sq[i / KSITE][i % KSITE] = P.site_q[i / KSITE][i % KSITE];
+  }
+  if (t <= KMAXS) {
+    // guest kinds beyond ntpguest: species 1 (never read), no sites
+    const bool real = t >= 1 && t <= P.ntpguest;
+    const int m = real ? P.locguest[t] : 1;
+    sh.kmol[t] = m;
+    sh.kna[t] = real ? P.natoms[m] : 1;
+    sh.kslot[t] = real ? P.gbase[t] : 0;
+  }
   __syncthreads();
 }
 
@@ -1370,7 +1424,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
   const int w = blockIdx.x;
   KSlot<NS> ks[KPT];
   load_slots<NS>(P, W, w, ks);
-  stage_in<NS>(W, w, sh);
+  stage_in<NS>(P, W, w, sh);
   stage_tables<T, LF>(P, W, w, lds);
   if (P.wl_nbins > 0) {
     const WLView v = wl_view(P, lds);
@@ -1454,7 +1508,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
   const DevParams &P = *Pp;
   KSlot<NS> ks[KPT];
   load_slots<NS>(P, W, w, ks);
-  stage_in<NS>(W, w, sh);
+  stage_in<NS>(P, W, w, sh);
   stage_tables<T, false>(P, W, w, lds);
   if (threadIdx.x == 0) {
     WState &ws = sh.ws;
@@ -1506,7 +1560,7 @@ __global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp
   extern __shared__ __align__(16) unsigned char lds[];
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
-  stage_in<NS>(W, w, sh);
+  stage_in<NS>(P, W, w, sh);
   if (threadIdx.x == 0) {
     WState &ws = sh.ws;
     int g = ws.pend_iguest, mol = P.locguest[g];
