@@ -197,6 +197,13 @@ __device__ inline void mzt_advance(WState &w) {
 // through the scalar unit instead of one vector load per lane.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Lane k's value of v (k uniform).
+__device__ __forceinline__ double rdlane(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
 // images, utility_pack.f:1697-1902 for imcon 1, 2, 3 (nint = round half
 // away).  In the orthogonal cells the nearest integer is taken with rint
 // (one instruction instead of six): the two differ only when a * dx is an

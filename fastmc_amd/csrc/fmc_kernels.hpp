@@ -37,6 +37,8 @@ struct Smem {
   // step reads them with independent LDS loads instead of chained scalar loads.
   double scq[KMAXS + 1][KSITE], sq[KMAXS + 1][KSITE];
   double ovl2;
+  // cell, site masses by species and guest templates (COM-centred) by guest kind
+  double cl9[9], sm[KMAXS + 1][KSITE], tmpl[KMAXS + 1][KSITE][3];
   int kmol[KMAXS + 1], kna[KMAXS + 1], kslot[KMAXS + 1];
   // constants of the current trial, computed while the energy terms run
   double qfix0, pref;
@@ -101,25 +103,8 @@ __device__ double gstlrcorrect(const DevParams &P, const WState &ws, int g, int 
 }
 
 // ------------------------------------------------------------- geometry --
-// com, utility_pack.f:3565-3596
-__device__ inline void com_of(const DevParams &P, int mol, int natms, const double (*x)[3],
-                              double *c) {
-  double wsum = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
-  for (int i = 0; i < natms; ++i) {
-    double m = P.site_m[mol][i];
-    wsum += m;
-    ax += x[i][0] * m;
-    ay += x[i][1] * m;
-    az += x[i][2] * m;
-  }
-  c[0] = ax / wsum;
-  c[1] = ay / wsum;
-  c[2] = az / wsum;
-}
-
-// rotation, utility_pack.f:3480-3524
-__device__ inline void rotate_about(double (*x)[3], int natms, const double *c, const double *q) {
-  double Q[9];
+// Rotation matrix of quaternion q, rotation (utility_pack.f:3480-3524).
+__device__ inline void rot_matrix(const double *q, double *Q) {
   double q1 = q[0], q2 = q[1], q3 = q[2], q4 = q[3];
   Q[0] = 2.0 * q2 * q2 + 2.0 * q1 * q1 - 1.0;
   Q[1] = 2.0 * q2 * q3 - 2.0 * q1 * q4;
@@ -130,15 +115,6 @@ __device__ inline void rotate_about(double (*x)[3], int natms, const double *c, 
   Q[6] = 2.0 * q2 * q4 - 2.0 * q1 * q3;
   Q[7] = 2.0 * q3 * q4 + 2.0 * q1 * q2;
   Q[8] = 2.0 * q4 * q4 + 2.0 * q1 * q1 - 1.0;
-  for (int i = 0; i < natms; ++i) {
-    double a = x[i][0] - c[0], b = x[i][1] - c[1], d = x[i][2] - c[2];
-    double tx = Q[0] * a + Q[1] * b + Q[2] * d;
-    double ty = Q[3] * a + Q[4] * b + Q[5] * d;
-    double tz = Q[6] * a + Q[7] * b + Q[8] * d;
-    x[i][0] = tx + c[0];
-    x[i][1] = ty + c[1];
-    x[i][2] = tz + c[2];
-  }
 }
 
 // random_ins / random_rot quaternion, utility_pack.f:3291-3310,3544-3560
@@ -272,168 +248,224 @@ __device__ inline int choose_move(const float *f, double randmov, bool skip_ins,
   return FMC_MOVE_DISPLACE;
 }
 
+// Wave-parallel proposal: the 64 lanes of wave 0 run it with wave-uniform
+// control flow.  The uniforms of the step are read once from the duni ring
+// (lane i holds call used+i+1; the n-th draw is a readlane), lane i builds
+// site i of the trial, and values every lane computes alike stay uniform.
+// LDS state is written by lane 0, or by all lanes with identical values
+// (the Wang-Landau visit bookkeeping).
 template <int NS, bool LF>
 __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh, unsigned char *lds) {
   WState &ws = sh.ws;
+  const int lane = threadIdx.x & 63;
+  const long long used0 = ws.mz_used;
+  const int rbits = __float_as_int(ws.rq[(int)((used0 + lane) & 127)]);
+  int nu = 0;
+  auto draw = [&]() -> double {
+    const float v = __int_as_float(__builtin_amdgcn_readlane(rbits, nu));
+    ++nu;
+    return (double)v;
+  };
+  const int wl_on = uni(ws.wl_on);
   int g = 1, kind, wlmode = 0;
-  if (ws.wl_on && ws.nmols[P.locguest[1]] < ws.wl_preload) {
+  if (wl_on && uni(ws.nmols[P.locguest[1]]) < uni(ws.wl_preload)) {
     // insert_guests (mc_moves.f:84-149) up to the window's lower edge
     wlmode = 2;
     kind = FMC_MOVE_INSERT;
-    ws.wl_iter += 1;
+    const long long it = ws.wl_iter;
+    if (lane == 0) ws.wl_iter = it + 1;
   } else {
     // the grand-canonical loop draws the guest type only when there is a
     // choice (gcmc.f:806-808); the Wang-Landau loop always draws (wang_landau.f:137)
-    if (P.ntpguest > 1 || ws.wl_on) g = uni((int)floor(duni(ws) * P.ntpguest) + 1);
+    if (P.ntpguest > 1 || wl_on) g = uni((int)floor(draw() * P.ntpguest) + 1);
     const int molg = P.locguest[g];
-    double randmov = duni(ws);
+    const double randmov = draw();
     const float *f = ws.mcf[g];
-    kind = choose_move(f, randmov, false, false);
-    if (ws.wl_on && g == 1) {
-      // window edges of the flat-histogram loop, wang_landau.f:161-235
+    kind = uni(choose_move(f, randmov, false, false));
+    if (wl_on && g == 1) {
+      // window edges of the flat-histogram loop, wang_landau.f:161-235 (the
+      // visit bookkeeping is written by every lane with the same values)
       wlmode = 1;
       const WLView v = wl_view(P, lds);
-      const int nmol = ws.nmols[molg];
-      if (nmol < ws.wl_min) {
+      const int nmol = uni(ws.nmols[molg]);
+      const int wmin = uni(ws.wl_min), wmax = uni(ws.wl_max);
+      if (nmol < wmin) {
         kind = FMC_MOVE_INSERT;
-      } else if (nmol <= ws.wl_min && kind == FMC_MOVE_DELETE) {
+      } else if (nmol <= wmin && kind == FMC_MOVE_DELETE) {
         wl_visit(ws, v, nmol);
-        kind = choose_move(f, randmov, false, true);
+        kind = uni(choose_move(f, randmov, false, true));
       }
-      if (nmol > ws.wl_max) {
+      if (nmol > wmax) {
         kind = FMC_MOVE_DELETE;
-      } else if (nmol >= ws.wl_max && kind == FMC_MOVE_INSERT) {
+      } else if (nmol >= wmax && kind == FMC_MOVE_INSERT) {
         wl_visit(ws, v, nmol);
-        kind = choose_move(f, randmov, true, false);
+        kind = uni(choose_move(f, randmov, true, false));
       }
       if (nmol == 0 && kind != FMC_MOVE_INSERT) kind = 0;  // nothing to move: an empty step
     }
   }
-  int mol = P.locguest[g], natms = P.natoms[mol], nmols = ws.nmols[mol];
-  sh.wlmode = wlmode;
+  const int mol = P.locguest[g], natms = P.natoms[mol], nmols = uni(ws.nmols[mol]);
   if (kind == 0) {
-    sh.kind = 0;
+    if (lane == 0) {
+      sh.wlmode = wlmode;
+      sh.kind = 0;
+      sh.g = g;
+      sh.mol = mol;
+      sh.natms = natms;
+      sh.nmols = nmols;
+      sh.choice = 0;
+      sh.has_new = sh.has_old = 0;
+      sh.excl_lo = sh.excl_hi = 0;
+      ws.mz_used = used0 + nu;
+    }
+    return;
+  }
+  // step-size adaptation of the grand-canonical loop (the Wang-Landau loop has
+  // none), gcmc.f:859-900: all counters read first, then the new sizes
+  double delrdisp = ws.delrdisp[g], rotangle = ws.rotangle;
+  double tran_delr = ws.tran_delr[g], rota_rotangle = ws.rota_rotangle;
+  const int dcnt = uni(ws.disp_count[g]), tcnt = uni(ws.tran_count[g]), rcnt = uni(ws.rota_count[g]);
+  const int dacc = uni(ws.accept_disp[g]), tacc = uni(ws.accept_tran[g]), racc = uni(ws.accept_rota[g]);
+  if (!wl_on && nmols >= 1) {
+    bool chd = false, chr = false, cht = false, chq = false;
+    if (dcnt >= 1) {
+      if (dcnt % 100 == 0) {
+        if ((dacc / (double)dcnt) > 0.5) delrdisp *= 1.05, chd = true;
+        else if (delrdisp > DELRMIN) delrdisp *= 0.95, chd = true;
+      } else if (dcnt % 50 == 0) {
+        if ((dacc / (double)dcnt) > 0.5) rotangle *= 1.05, chr = true;
+        else if (rotangle > MINANGLE) rotangle *= 0.95, chr = true;
+      }
+    }
+    if (tcnt >= 1 && tcnt % 100 == 0) {
+      if ((tacc / (double)tcnt) > 0.5) tran_delr *= 1.05, cht = true;
+      else if (tran_delr > DELRMIN) tran_delr *= 0.95, cht = true;
+    }
+    if (rcnt >= 1 && rcnt % 100 == 0) {
+      if ((racc / (double)rcnt) > 0.5) rota_rotangle *= 1.05, chq = true;
+      else if (rota_rotangle > MINANGLE) rota_rotangle *= 0.95, chq = true;
+    }
+    if (lane == 0) {
+      if (chd) ws.delrdisp[g] = delrdisp;
+      if (chr) ws.rotangle = rotangle;
+      if (cht) ws.tran_delr[g] = tran_delr;
+      if (chq) ws.rota_rotangle = rota_rotangle;
+    }
+  }
+  if (wlmode == 0 && nmols == 0) kind = FMC_MOVE_INSERT;
+
+  const bool site = lane < natms;
+  const int i = site ? lane : 0;
+  int choice = 0;
+  if (kind == FMC_MOVE_INSERT) {
+    // random_ins + rotation of the COM-centred template (utility_pack.f:3220-3331, 3480-3524)
+    const double ra = draw(), rb = draw(), rc = draw();
+    const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+    const double *cl = sh.cl9;
+    const double xc = ra * cl[0] + rb * cl[3] + rc * cl[6];
+    const double yc = ra * cl[1] + rb * cl[4] + rc * cl[7];
+    const double zc = ra * cl[2] + rb * cl[5] + rc * cl[8];
+    double q[4], Q[9];
+    quat_from(r1, r2, r3, r4, 2.0 * PI, q);
+    rot_matrix(q, Q);
+    const double a = sh.tmpl[g][i][0], b = sh.tmpl[g][i][1], d = sh.tmpl[g][i][2];
+    const double tx = Q[0] * a + Q[1] * b + Q[2] * d;
+    const double ty = Q[3] * a + Q[4] * b + Q[5] * d;
+    const double tz = Q[6] * a + Q[7] * b + Q[8] * d;
+    if (site) {
+      sh.xn[i][0] = tx + xc;
+      sh.xn[i][1] = ty + yc;
+      sh.xn[i][2] = tz + zc;
+    }
+  } else if (kind == FMC_MOVE_DELETE) {
+    choice = uni((int)floor(draw() * nmols) + 1);
+    const int a = P.gbase[g] + (choice - 1) * natms + i;
+    const double x = G.x[a], y = G.y[a], z = G.z[a];
+    if (site) {
+      sh.xo[i][0] = x;
+      sh.xo[i][1] = y;
+      sh.xo[i][2] = z;
+    }
+  } else {
+    // a translation keeps q = 0, which rotation turns into the inversion
+    // through the COM, as the reference's mc_displace does (gcmc.f:1845,2875)
+    double a = 0.0, b = 0.0, c = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (kind == FMC_MOVE_TRANSLATE) {
+      const double h = tran_delr * 0.5;
+      a = (2.0 * draw() - 1.0) * h;
+      b = (2.0 * draw() - 1.0) * h;
+      c = (2.0 * draw() - 1.0) * h;
+      if (lane == 0) ws.tran_count[g] = tcnt + 1;
+    } else if (kind == FMC_MOVE_ROTATE) {
+      const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+      quat_from(r1, r2, r3, r4, rota_rotangle, q);
+      if (lane == 0) ws.rota_count[g] = rcnt + 1;
+    } else {
+      const double h = delrdisp * 0.5;
+      const double da = draw(), db = draw(), dc = draw();
+      a = (2.0 * da - 1.0) * h;
+      b = (2.0 * db - 1.0) * h;
+      c = (2.0 * dc - 1.0) * h;
+      const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+      quat_from(r1, r2, r3, r4, rotangle, q);
+      if (lane == 0) ws.disp_count[g] = dcnt + 1;
+    }
+    choice = uni((int)floor(draw() * nmols) + 1);
+    const int at = P.gbase[g] + (choice - 1) * natms + i;
+    const double ox = G.x[at], oy = G.y[at], oz = G.z[at];
+    // translate, utility_pack.f:3161-3218: move, then wrap the COM into the cell
+    double nx = a + ox, ny = b + oy, nz = c + oz;
+    double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int k = 0; k < natms; ++k) {
+      const double m = sh.sm[mol][k];
+      wsum += m;
+      cx += rdlane(nx, k) * m;
+      cy += rdlane(ny, k) * m;
+      cz += rdlane(nz, k) * m;
+    }
+    cx = cx / wsum;
+    cy = cy / wsum;
+    cz = cz / wsum;
+    const double *rcl = sh.rc9, *cl = sh.cl9;
+    const double ssx = rcl[0] * cx + rcl[3] * cy + rcl[6] * cz;
+    const double ssy = rcl[1] * cx + rcl[4] * cy + rcl[7] * cz;
+    const double ssz = rcl[2] * cx + rcl[5] * cy + rcl[8] * cz;
+    const double xss = ssx - floor(ssx), yss = ssy - floor(ssy), zss = ssz - floor(ssz);
+    const double wx = cl[0] * xss + cl[3] * yss + cl[6] * zss;
+    const double wy = cl[1] * xss + cl[4] * yss + cl[7] * zss;
+    const double wz = cl[2] * xss + cl[5] * yss + cl[8] * zss;
+    nx += wx - cx;
+    ny += wy - cy;
+    nz += wz - cz;
+    // rotation about the wrapped COM, utility_pack.f:3480-3524
+    double Q[9];
+    rot_matrix(q, Q);
+    const double dx = nx - wx, dy = ny - wy, dz = nz - wz;
+    const double tx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+    const double ty = Q[3] * dx + Q[4] * dy + Q[5] * dz;
+    const double tz = Q[6] * dx + Q[7] * dy + Q[8] * dz;
+    if (site) {
+      sh.xo[i][0] = ox;
+      sh.xo[i][1] = oy;
+      sh.xo[i][2] = oz;
+      sh.xn[i][0] = tx + wx;
+      sh.xn[i][1] = ty + wy;
+      sh.xn[i][2] = tz + wz;
+    }
+  }
+  if (lane == 0) {
+    sh.wlmode = wlmode;
+    sh.kind = kind;
     sh.g = g;
     sh.mol = mol;
     sh.natms = natms;
     sh.nmols = nmols;
-    sh.choice = 0;
-    sh.has_new = sh.has_old = 0;
-    sh.excl_lo = sh.excl_hi = 0;
-    return;
-  }
-  // step-size adaptation of the grand-canonical loop (the Wang-Landau loop has none)
-  if (!ws.wl_on && nmols >= 1 && ws.disp_count[g] >= 1) {
-    if (ws.disp_count[g] % 100 == 0) {
-      if ((ws.accept_disp[g] / (double)ws.disp_count[g]) > 0.5) ws.delrdisp[g] *= 1.05;
-      else if (ws.delrdisp[g] > DELRMIN) ws.delrdisp[g] *= 0.95;
-    } else if (ws.disp_count[g] % 50 == 0) {
-      if ((ws.accept_disp[g] / (double)ws.disp_count[g]) > 0.5) ws.rotangle *= 1.05;
-      else if (ws.rotangle > MINANGLE) ws.rotangle *= 0.95;
-    }
-  }
-  if (!ws.wl_on && nmols >= 1 && ws.tran_count[g] >= 1 && ws.tran_count[g] % 100 == 0) {
-    if ((ws.accept_tran[g] / (double)ws.tran_count[g]) > 0.5) ws.tran_delr[g] *= 1.05;
-    else if (ws.tran_delr[g] > DELRMIN) ws.tran_delr[g] *= 0.95;
-  }
-  if (!ws.wl_on && nmols >= 1 && ws.rota_count[g] >= 1 && ws.rota_count[g] % 100 == 0) {
-    if ((ws.accept_rota[g] / (double)ws.rota_count[g]) > 0.5) ws.rota_rotangle *= 1.05;
-    else if (ws.rota_rotangle > MINANGLE) ws.rota_rotangle *= 0.95;
-  }
-  if (wlmode == 0 && nmols == 0) kind = FMC_MOVE_INSERT;
-
-  sh.kind = kind;
-  sh.g = g;
-  sh.mol = mol;
-  sh.natms = natms;
-  sh.nmols = nmols;
-  sh.choice = 0;
-  sh.excl_lo = sh.excl_hi = 0;
-  if (kind == FMC_MOVE_INSERT) {
-    double ra = duni(ws), rb = duni(ws), rc = duni(ws);
-    double r1 = duni(ws), r2 = duni(ws), r3 = duni(ws), r4 = duni(ws);
-    const double *cl = P.cell;
-    double xc = ra * cl[0] + rb * cl[3] + rc * cl[6];
-    double yc = ra * cl[1] + rb * cl[4] + rc * cl[7];
-    double zc = ra * cl[2] + rb * cl[5] + rc * cl[8];
-    for (int i = 0; i < natms; ++i) {
-      sh.xn[i][0] = P.tmpl[g][i][0];
-      sh.xn[i][1] = P.tmpl[g][i][1];
-      sh.xn[i][2] = P.tmpl[g][i][2];
-    }
-    double q[4], c0[3] = {0.0, 0.0, 0.0};
-    quat_from(r1, r2, r3, r4, 2.0 * PI, q);
-    rotate_about(sh.xn, natms, c0, q);
-    for (int i = 0; i < natms; ++i) {
-      sh.xn[i][0] += xc;
-      sh.xn[i][1] += yc;
-      sh.xn[i][2] += zc;
-    }
-    sh.has_new = 1;
-    sh.has_old = 0;
-  } else if (kind == FMC_MOVE_DELETE) {
-    int choice = (int)floor(duni(ws) * nmols) + 1;
     sh.choice = choice;
-    load_molecule<LF>(P, G, g, choice, sh.xo, natms);
-    sh.has_new = 0;
-    sh.has_old = 1;
-    sh.excl_lo = (choice - 1) * natms;
-    sh.excl_hi = choice * natms;
-  } else {
-    double a = 0.0, b = 0.0, c = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
-    if (kind == FMC_MOVE_TRANSLATE) {
-      ws.tran_count[g] += 1;
-      double h = ws.tran_delr[g] * 0.5;
-      a = (2.0 * duni(ws) - 1.0) * h;
-      b = (2.0 * duni(ws) - 1.0) * h;
-      c = (2.0 * duni(ws) - 1.0) * h;
-    } else if (kind == FMC_MOVE_ROTATE) {
-      ws.rota_count[g] += 1;
-      double r1 = duni(ws), r2 = duni(ws), r3 = duni(ws), r4 = duni(ws);
-      quat_from(r1, r2, r3, r4, ws.rota_rotangle, q);
-    } else {
-      ws.disp_count[g] += 1;
-      double h = ws.delrdisp[g] * 0.5;
-      double da = duni(ws), db = duni(ws), dc = duni(ws);
-      a = (2.0 * da - 1.0) * h;
-      b = (2.0 * db - 1.0) * h;
-      c = (2.0 * dc - 1.0) * h;
-      double r1 = duni(ws), r2 = duni(ws), r3 = duni(ws), r4 = duni(ws);
-      quat_from(r1, r2, r3, r4, ws.rotangle, q);
-    }
-    int choice = (int)floor(duni(ws) * nmols) + 1;
-    sh.choice = choice;
-    load_molecule<LF>(P, G, g, choice, sh.xo, natms);
-    // translate, utility_pack.f:3161-3218
-    for (int i = 0; i < natms; ++i) {
-      sh.xn[i][0] = a + sh.xo[i][0];
-      sh.xn[i][1] = b + sh.xo[i][1];
-      sh.xn[i][2] = c + sh.xo[i][2];
-    }
-    double cm[3];
-    com_of(P, mol, natms, sh.xn, cm);
-    const double *rc = P.rcell, *cl = P.cell;
-    double ssx = rc[0] * cm[0] + rc[3] * cm[1] + rc[6] * cm[2];
-    double ssy = rc[1] * cm[0] + rc[4] * cm[1] + rc[7] * cm[2];
-    double ssz = rc[2] * cm[0] + rc[5] * cm[1] + rc[8] * cm[2];
-    double xss = ssx - floor(ssx), yss = ssy - floor(ssy), zss = ssz - floor(ssz);
-    double x = cl[0] * xss + cl[3] * yss + cl[6] * zss;
-    double y = cl[1] * xss + cl[4] * yss + cl[7] * zss;
-    double z = cl[2] * xss + cl[5] * yss + cl[8] * zss;
-    double sx = x - cm[0], sy = y - cm[1], sz = z - cm[2];
-    cm[0] = x;
-    cm[1] = y;
-    cm[2] = z;
-    for (int i = 0; i < natms; ++i) {
-      sh.xn[i][0] += sx;
-      sh.xn[i][1] += sy;
-      sh.xn[i][2] += sz;
-    }
-    rotate_about(sh.xn, natms, cm, q);
-    sh.has_new = 1;
-    sh.has_old = 1;
-    sh.excl_lo = (choice - 1) * natms;
-    sh.excl_hi = choice * natms;
+    sh.has_new = kind != FMC_MOVE_DELETE;
+    sh.has_old = kind != FMC_MOVE_INSERT;
+    sh.excl_lo = kind == FMC_MOVE_INSERT ? 0 : (choice - 1) * natms;
+    sh.excl_hi = kind == FMC_MOVE_INSERT ? 0 : choice * natms;
+    ws.mz_used = used0 + nu;
   }
 }
 
@@ -1351,7 +1383,14 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
   int *dst = (int *)&sh.ws;
   for (int i = threadIdx.x; i < (int)(sizeof(WState) / 4); i += blockDim.x) dst[i] = src[i];
   const int t = threadIdx.x;
-  if (t < 9) sh.rc9[t] = P.rcell[t];
+  if (t < 9) {
+    sh.rc9[t] = P.rcell[t];
+    sh.cl9[t] = P.cell[t];
+  }
+  for (int i = t; i < (KMAXS + 1) * KSITE * 3; i += blockDim.x) {
+    const int gs = i / 3;
+    sh.tmpl[gs / KSITE][gs % KSITE][i % 3] = P.tmpl[gs / KSITE][gs % KSITE][i % 3];
+  }
   if (t == 9) {
     sh.kmx[0] = P.kmax1;
     sh.kmx[1] = P.kmax2;
@@ -1361,6 +1400,7 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
   for (int i = t; i < (KMAXS + 1) * KSITE; i += blockDim.x) {
     sh.scq[i / KSITE][i % KSITE] = P.site_chgea[i / KSITE][i % KSITE];
     sh.sq[i / KSITE][i % KSITE] = P.site_q[i / KSITE][i % KSITE];
+    sh.sm[i / KSITE][i % KSITE] = P.site_m[i / KSITE][i % KSITE];
   }
   if (t <= KMAXS) {
     // guest kinds beyond ntpguest: species 1 (never read), no sites
@@ -1442,7 +1482,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
         mzt_refill(sh.ws, threadIdx.x);
         if (threadIdx.x == 0) mzt_advance(sh.ws);
       }
-      if (threadIdx.x == 0) propose<NS, LF>(P, G, sh, lds);
+      propose<NS, LF>(P, G, sh, lds);
     }
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
