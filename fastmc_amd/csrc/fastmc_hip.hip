@@ -441,6 +441,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   P.alpha = sys->alpha;
   P.epsq = sys->epsq > 0 ? sys->epsq : 1.0;
   P.engunit = sys->engunit;
+  P.rengunit = 1.0 / sys->engunit;
   P.rvolm = 2.0 * PI / P.volm;
   P.kmax1 = sys->kmax[0];
   P.kmax2 = sys->kmax[1];

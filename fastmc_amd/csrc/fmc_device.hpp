@@ -71,7 +71,7 @@ struct DevParams {
   int gatoms_total;               // guest atom slots per walker
   double cell[9], rcell[9], inv_cell[3];  // inv_cell: 1/cell(1), 1/cell(5), 1/cell(9) (images)
   double volm, statvolm, rcut2, rclim2, overlap2, drewd, rdrewd, dlrpot, rdlrpot;
-  double alpha, epsq, engunit, rvolm;
+  double alpha, epsq, engunit, rvolm, rengunit;  // rengunit = 1 / engunit
   double site_q[KMAXS + 1][KSITE], site_m[KMAXS + 1][KSITE];
   double site_chgea[KMAXS + 1][KSITE];          // q / epsq * r4pie0 (ewald.f:686)
   double tmpl[KMAXS + 1][KSITE][3];            // guest g template (COM-centred)

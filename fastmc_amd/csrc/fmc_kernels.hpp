@@ -938,26 +938,30 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     sh.ovlw[wave][1] = bo != 0ULL;
   }
   __syncthreads();
-  if (tid < A::NACC) {
-    // pairwise over the waves (missing waves count as zero)
-    double v[KWAVES];
+  // wave 0 finishes the sums; the decision that reads them runs on wave 0
+  // too, so no further barrier is needed before it (callers that read
+  // sh.tot or the overlap flags from other waves synchronise first)
+  if (wave == 0) {
+    if (lane < A::NACC) {
+      // pairwise over the waves (missing waves count as zero)
+      double v[KWAVES];
 #pragma unroll
-    for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][tid] : 0.0;
+      for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
 #pragma unroll
-    for (int st = KWAVES / 2; st >= 1; st >>= 1)
+      for (int st = KWAVES / 2; st >= 1; st >>= 1)
 #pragma unroll
-      for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
-    sh.tot[tid] = v[0];
-  } else if (tid == A::NACC) {
-    int a = 0, b = 0;
-    for (int wv = 0; wv < nwaves; ++wv) {
-      a |= sh.ovlw[wv][0];
-      b |= sh.ovlw[wv][1];
+        for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
+      sh.tot[lane] = v[0];
+    } else if (lane == A::NACC) {
+      int a = 0, b = 0;
+      for (int wv = 0; wv < nwaves; ++wv) {
+        a |= sh.ovlw[wv][0];
+        b |= sh.ovlw[wv][1];
+      }
+      sh.ovl_new = a;
+      sh.ovl_old = b;
     }
-    sh.ovl_new = a;
-    sh.ovl_old = b;
   }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------- decision --
@@ -967,7 +971,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
 // acceptance rules.  Thread 0.  If draw == 0 the Metropolis test is skipped
 // (one-trial-at-a-time seam: the caller decides).
 template <int NS>
-__device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char *lds) {
+__device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char *lds, bool want_comp) {
   WState &ws = sh.ws;
   if (sh.kind == 0) {
     // an empty Wang-Landau step (no molecule to move): only the flatness test
@@ -982,7 +986,7 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
   }
   const int g = uni(sh.g), mol = uni(sh.mol), kind = uni(sh.kind);
   const double *t = sh.tot;
-  const double C = P.dc_C, engunit = P.engunit;
+  const double C = P.dc_C, reu = P.rengunit;
   const double engsic0 = P.dc_esic[g], qfix0 = sh.qfix0;
   double e1new[KMAXS + 1], e1old[KMAXS + 1];
   for (int i = 1; i <= NS; ++i) {
@@ -1007,47 +1011,53 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
     overlap = sh.ovl_new;
     if (!overlap) {
       double ewld1eng = C * t[4 * NS] + engsic0 + qfix0;
-      estep = (ewld1eng + real_n - ewld3 + vdw_n + delrc) / engunit;
+      estep = (ewld1eng + real_n - ewld3 + vdw_n + delrc) * reu;
       for (int i = 1; i <= NS; ++i) {
         int ik = loc2(mol, i);
-        if (i != mol) sh.delE[i] += (e1new[i] + t[i - 1] + t[NS + i - 1] + sh.delrc_mol[ik]) / engunit;
+        if (i != mol) sh.delE[i] += (e1new[i] + t[i - 1] + t[NS + i - 1] + sh.delrc_mol[ik]) * reu;
       }
       sh.delE[mol] += estep;
-      sh.comp[0] = (ewld1eng - ewld3) / engunit;
-      sh.comp[1] = real_n / engunit;
-      sh.comp[2] = vdw_n / engunit;
-      sh.comp[3] = delrc / engunit;
+      if (want_comp) {
+        sh.comp[0] = (ewld1eng - ewld3) * reu;
+        sh.comp[1] = real_n * reu;
+        sh.comp[2] = vdw_n * reu;
+        sh.comp[3] = delrc * reu;
+      }
     }
   } else if (kind == FMC_MOVE_DELETE) {
     double delrc = sh.delrc;
     overlap = sh.ovl_old;
     double ewld1eng = C * t[4 * NS + 1] + (-engsic0) + (-qfix0);
-    estep = (-ewld1eng + real_o + vdw_o - ewld3) / engunit;
-    estep = estep - delrc / engunit;
+    estep = (-ewld1eng + real_o + vdw_o - ewld3) * reu;
+    estep = estep - delrc * reu;
     for (int i = 1; i <= NS; ++i) {
       int ik = loc2(mol, i);
-      if (i != mol) sh.delE[i] -= (e1old[i] + t[2 * NS + i - 1] + t[3 * NS + i - 1] + sh.delrc_mol[ik]) / engunit;
+      if (i != mol) sh.delE[i] -= (e1old[i] + t[2 * NS + i - 1] + t[3 * NS + i - 1] + sh.delrc_mol[ik]) * reu;
     }
     sh.delE[mol] -= estep;
-    sh.comp[0] = (-ewld1eng - ewld3) / engunit;
-    sh.comp[1] = real_o / engunit;
-    sh.comp[2] = vdw_o / engunit;
-    sh.comp[3] = -delrc / engunit;
+    if (want_comp) {
+      sh.comp[0] = (-ewld1eng - ewld3) * reu;
+      sh.comp[1] = real_o * reu;
+      sh.comp[2] = vdw_o * reu;
+      sh.comp[3] = -delrc * reu;
+    }
   } else {
     overlap = sh.ovl_new;
     double ewld1_old = C * t[4 * NS + 1] + (-engsic0) + (-qfix0);
-    double est_old = sh.ovl_old ? 0.0 : (-ewld1_old + real_o + vdw_o - ewld3) / engunit;
+    double est_old = sh.ovl_old ? 0.0 : (-ewld1_old + real_o + vdw_o - ewld3) * reu;
     double ewld1_new = C * t[4 * NS] + engsic0 + qfix0;
-    double est_new = overlap ? 0.0 : (ewld1_new + real_n + vdw_n - ewld3) / engunit;
+    double est_new = overlap ? 0.0 : (ewld1_new + real_n + vdw_n - ewld3) * reu;
     estep = est_new - est_old;
     for (int i = 1; i <= NS; ++i)
       if (i != mol)
-        sh.delE[i] += (e1new[i] - e1old[i] + t[i - 1] - t[2 * NS + i - 1] + t[NS + i - 1] - t[3 * NS + i - 1]) / engunit;
+        sh.delE[i] += (e1new[i] - e1old[i] + t[i - 1] - t[2 * NS + i - 1] + t[NS + i - 1] - t[3 * NS + i - 1]) * reu;
     sh.delE[mol] += estep;
-    sh.comp[0] = ((ewld1_new - ewld3) - (-ewld1_old - ewld3)) / engunit;
-    sh.comp[1] = (real_n - real_o) / engunit;
-    sh.comp[2] = (vdw_n - vdw_o) / engunit;
-    sh.comp[3] = 0.0;
+    if (want_comp) {
+      sh.comp[0] = ((ewld1_new - ewld3) - (-ewld1_old - ewld3)) * reu;
+      sh.comp[1] = (real_n - real_o) * reu;
+      sh.comp[2] = (vdw_n - vdw_o) * reu;
+      sh.comp[3] = 0.0;
+    }
   }
   sh.estep = estep;
   sh.ovl_new = overlap;
@@ -1487,7 +1497,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
-      decide<NS>(P, sh, 1, lds);
+      decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == 0);
       WState &ws = sh.ws;
       int kind = sh.kind;
       ws.att[kind] += 1;
@@ -1574,7 +1584,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
   __syncthreads();
   evaluate<NS, T, false, 0>(P, W, w, sh, lds, ks, true);
   if (threadIdx.x == 0) {
-    decide<NS>(P, sh, 0, lds);
+    decide<NS>(P, sh, 0, lds, true);
     fmc_trial *o = W.trial_out;
     o->estep = sh.estep;
     for (int i = 0; i < FMC_MAX_SPECIES; ++i) o->delE[i] = (i < NS) ? sh.delE[i + 1] : 0.0;
