@@ -40,6 +40,9 @@ struct Smem {
   // cell, site masses by species and guest templates (COM-centred) by guest kind
   double cl9[9], sm[KMAXS + 1][KSITE], tmpl[KMAXS + 1][KSITE][3];
   int kmol[KMAXS + 1], kna[KMAXS + 1], kslot[KMAXS + 1];
+  // first index of each guest kind in the pair loop's atom space (framework
+  // atoms first) and the end of the space; kept current by apply
+  int gstart[KMAXS + 1], ntot;
   // constants of the current trial, computed while the energy terms run
   double qfix0, pref;
   double qfm[KMAXS + 1];
@@ -671,13 +674,17 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   // reciprocal loop.  The table rows are spread over the waves, a few lanes
   // each, so that no wave carries them all.
   const int lane = tid & 63, wave = tid >> 6, nwaves = (nt + 63) >> 6;
-  if (tid == (nwaves >> 1) * 64 + 63) prepare<NS>(P, sh);
-  {
+  // the top wave has the fewest pair atoms (they are dealt from thread 0 up),
+  // so it builds the phase tables (at most 2 * KSITE * 3 = 48 rows, one lane
+  // each) and the wave below it the trial constants
+  if (tid == (nwaves >= 2 ? nt - 65 : nt - 1)) prepare<NS>(P, sh);
+  if (wave == nwaves - 1) {
     constexpr int NSC = NSITE > 0 ? NSITE : 0;
     const int nsr = NSITE > 0 ? NSC : P.nsmax;
-    const int ntask = 2 * nsr * 3, tpw = (ntask + nwaves - 1) / nwaves;
-    const int t = wave * tpw + lane;
-    if (lane < tpw && t < ntask) {
+    const int ntask = 2 * nsr * 3;
+    static_assert(2 * KSITE * 3 <= 64, "phase-table rows exceed one wave");
+    const int t = lane;
+    if (t < ntask) {
       // per-site 1-D phase tables, built by the reference recursion (ewald.f:108-147)
       const int cfg = t / (nsr * 3), rem = t - cfg * (nsr * 3), s = rem / 3, ax = rem - 3 * s;
       if (s < natms && (cfg == 0 ? has_new : has_old)) {
@@ -736,18 +743,14 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   // atom index space: framework atoms, then the guests of each kind (per
   // guest kind: first index, species, sites, first slot; all uniform)
   int gstart[KMAXS + 1], gmol[KMAXS + 1], gna[KMAXS + 1], gslot[KMAXS + 1];
-  int ntot = P.nfw;
 #pragma unroll
   for (int gg = 1; gg <= KMAXS; ++gg) {
-    const int km = uni(sh.kmol[gg]), kn = uni(sh.kna[gg]), kslt = uni(sh.kslot[gg]);
-    const int nm = uni(sh.ws.nmols[km]);
-    const bool real = gg <= P.ntpguest;
-    gstart[gg] = real ? ntot : 0x7fffffff;
-    gmol[gg] = km;
-    gna[gg] = kn;
-    gslot[gg] = kslt;
-    ntot += real ? nm * kn : 0;
+    gstart[gg] = uni(sh.gstart[gg]);
+    gmol[gg] = uni(sh.kmol[gg]);
+    gna[gg] = uni(sh.kna[gg]);
+    gslot[gg] = uni(sh.kslot[gg]);
   }
+  const int ntot = uni(sh.ntot);
   const int excl_lo = uni(sh.excl_lo), excl_hi = uni(sh.excl_hi);
   const GuestXYZ<LF> G(P, W, w, lds);
   const double *fx, *fy, *fz, *fq;
@@ -1275,6 +1278,7 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
       ws.elrc += sg * sh.delrc;
       for (int t = 1; t <= KMAXC; ++t) ws.elrc_mol[t] += sg * sh.delrc_mol[t];
       ws.nmols[mol] += dn;
+      gstart_from(P, ws.nmols, sh);
     }
   }
   __syncthreads();
@@ -1387,6 +1391,23 @@ __device__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Sme
   __syncthreads();
 }
 
+// Guest kind gg's first index in the atom space of the pair loop, from the
+// molecule counts by species (one thread).
+template <int NS>
+__device__ inline void gstart_from(const DevParams &P, const int *nmols, Smem<NS> &sh) {
+  int n = P.nfw;
+  for (int gg = 1; gg <= KMAXS; ++gg) {
+    if (gg <= P.ntpguest) {
+      const int m = P.locguest[gg];
+      sh.gstart[gg] = n;
+      n += nmols[m] * P.natoms[m];
+    } else {
+      sh.gstart[gg] = 0x7fffffff;
+    }
+  }
+  sh.ntot = n;
+}
+
 template <int NS>
 __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
   const int *src = (const int *)(W.ws + w);
@@ -1412,6 +1433,7 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.sq[i / KSITE][i % KSITE] = P.site_q[i / KSITE][i % KSITE];
     sh.sm[i / KSITE][i % KSITE] = P.site_m[i / KSITE][i % KSITE];
   }
+  if (t == 0) gstart_from(P, W.ws[w].nmols, sh);
   if (t <= KMAXS) {
     // guest kinds beyond ntpguest: species 1 (never read), no sites
     const bool real = t >= 1 && t <= P.ntpguest;
