@@ -2187,6 +2187,76 @@ int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst
 }
 
 /* ----------------------------------------------------------- getters -- */
+/* --------------------------------------------------------- Widom grid -- */
+/* widom_grid, gcmc.f:2454-2548, with the grid set-up of gcmc.f:258-281,
+ * 605-625: grid spacing griddim, factors gf; points (i/wngrida, j/wngridb,
+ * k/wngridc) for i = 1..ngrida etc.; at each point nstep orientations, each a
+ * random_rot quaternion (4 draws, angle 2 pi) applied to the COM-centred
+ * template about the origin, then moved to the point; insertion energy; sum of
+ * exp(-estep / (kboltz T)) over non-overlapping attempts, averaged over all
+ * attempts.  out: [0] <exp(-beta estep)>, [1] attempts, [2] non-overlapping
+ * attempts, [3] Henry's constant as printed (gcmc.f:1319-1321), [4..6]
+ * ngrida, ngridb, ngridc, [7..9] wngrida, wngridb, wngridc. */
+int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[3], double out[10]) {
+  if (iguest < 1 || iguest > s->ntpguest || nstep < 1 || griddim <= 0.0) return FMC_ERR_ARG;
+  double celprp[11];
+  dcell(s->cell, celprp);
+  double gd[3];
+  int ng[3], wng[3];
+  for (int a = 0; a < 3; ++a) {
+    gd[a] = celprp[1 + a] / ceil(celprp[1 + a] / gf[a] / griddim) / gf[a];
+    ng[a] = (int)ceil(celprp[1 + a] / (gd[a] * gf[a]));
+    wng[a] = (int)ceil(celprp[1 + a] / gd[a]);
+  }
+  const int mol = s->locguest[iguest], natms = s->numatoms[mol];
+  double hsum = 0.0;
+  long long widcount = 0, prodcount = 0;
+  fmc_trial tr;
+  const double *cl = s->cell;
+  for (int i = 1; i <= ng[0]; ++i)
+    for (int j = 1; j <= ng[1]; ++j)
+      for (int k = 1; k <= ng[2]; ++k) {
+        double afr = (double)i / (double)wng[0], bfr = (double)j / (double)wng[1], cfr = (double)k / (double)wng[2];
+        /* cartesian, utility_pack.f:3694-3709 */
+        double cx = afr * cl[1] + bfr * cl[4] + cfr * cl[7];
+        double cy = afr * cl[2] + bfr * cl[5] + cfr * cl[8];
+        double cz = afr * cl[3] + bfr * cl[6] + cfr * cl[9];
+        for (int st = 0; st < nstep; ++st) {
+          for (int a = 1; a <= natms; ++a) {
+            s->newx[a] = s->guestx[iguest][a];
+            s->newy[a] = s->guesty[iguest][a];
+            s->newz[a] = s->guestz[iguest][a];
+          }
+          double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+          double q[4];
+          quat_from(r1, r2, r3, r4, 2.0 * PI, q);
+          rotation(s->newx, s->newy, s->newz, 0.0, 0.0, 0.0, natms, q[0], q[1], q[2], q[3]);
+          for (int a = 1; a <= natms; ++a) {
+            s->newx[a] += cx;
+            s->newy[a] += cy;
+            s->newz[a] += cz;
+          }
+          widcount += 1;
+          int ov = insertion(s, iguest, &tr);
+          for (int m = 0; m < 64; ++m) s->delE[m] = 0.0;
+          if (!ov) {
+            prodcount += 1;
+            hsum += exp(-1.0 * tr.estep / KBOLTZ / s->temp);
+          }
+        }
+      }
+  const double avg = hsum / (double)widcount;
+  out[0] = avg;
+  out[1] = (double)widcount;
+  out[2] = (double)prodcount;
+  out[3] = avg / AVO / BOLTZ / s->temp * 1.0e5;
+  for (int a = 0; a < 3; ++a) {
+    out[4 + a] = ng[a];
+    out[7 + a] = wng[a];
+  }
+  return 0;
+}
+
 int orc_get_guests(orc_state *s, int iguest, double *xyz, int cap_mols, int *nmols) {
   if (iguest < 1 || iguest > s->ntpguest) return FMC_ERR_ARG;
   int mol = s->locguest[iguest], na = s->numatoms[mol], nm = s->nummols[mol];

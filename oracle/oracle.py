@@ -78,6 +78,18 @@ class Oracle:
         except Exception:
             pass
 
+    def widom(self, iguest, nstep, griddim, gridfactor=(1, 1, 1)):
+        """widom_grid (gcmc.f:2454-2548): dict of <exp(-beta estep)>, attempts,
+        non-overlapping attempts, Henry's constant as printed, grid sizes."""
+        out = np.zeros(10)
+        gf = (C.c_int * 3)(*gridfactor)
+        rc = lib().orc_widom(C.c_void_p(self.h), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim), gf,
+                             out.ctypes.data_as(C.c_void_p))
+        if rc:
+            raise RuntimeError(f"orc_widom: {rc}")
+        return {"avg_boltz": out[0], "attempts": int(out[1]), "accepted": int(out[2]), "henry": out[3],
+                "ngrid": tuple(int(x) for x in out[4:7]), "wngrid": tuple(int(x) for x in out[7:10])}
+
     def single_point(self):
         out = np.zeros(5)
         lib().orc_single_point(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
