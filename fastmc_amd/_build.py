@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastmc_hip.so")
 SOURCES = [os.path.join(CSRC, "fastmc_hip.hip")]
 HEADERS = [os.path.join(CSRC, "fmc_device.hpp"), os.path.join(CSRC, "fmc_kernels.hpp"),
-           os.path.join(CSRC, "fmc_grad.hpp"),
+           os.path.join(CSRC, "fmc_grad.hpp"), os.path.join(CSRC, "fmc_widom.hpp"),
            os.path.join(os.path.dirname(HERE), "include", "fastmc.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

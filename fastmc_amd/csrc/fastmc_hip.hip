@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "fmc_grad.hpp"
+#include "fmc_widom.hpp"
 
 using namespace fmc;
 
@@ -1164,10 +1165,6 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   if (trace) {
     if (h->dtrace_cap < nsteps) {
       if (h->dtrace) hipFree(h->dtrace);
-  for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
-                  (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
-                  (void *)h->g_vt})
-    if (p) hipFree(p);
       h->dtrace = nullptr;
       HIPCHK(hipMalloc(&h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps));
       h->dtrace_cap = nsteps;
@@ -1493,6 +1490,117 @@ int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, d
     for (int i = 0; i <= maxn - minn; ++i) zs += std::exp(zt[i] - zmax);
     for (int k = 1; k < maxn; ++k) ns += std::exp(nt[k] - nmax);
     n_out[ip] = maxn > 1 ? std::exp(std::log(ns) + nmax - std::log(zs) - zmax) : 0.0;
+  }
+  return FMC_OK;
+}
+
+// widom_grid (gcmc.f:2454-2548) on walker w: grid set-up of gcmc.f:258-281 and
+// 605-610, attempts in the reference's loop order, orientations from the
+// walker's duni sequence (which advances by 4 draws per attempt).
+int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+              double *estep_out, int *overlap_out, fmc_widom_result *res) {
+  if (!h || !res || w < 0 || w >= h->nwalkers || iguest < 1 || iguest > h->hp.ntpguest || nstep < 1 ||
+      !(griddim > 0.0))
+    return FMC_ERR_ARG;
+  const int gf[3] = {gridfactor ? gridfactor[0] : 1, gridfactor ? gridfactor[1] : 1, gridfactor ? gridfactor[2] : 1};
+  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  double celprp[10];
+  dcell(P.cell, celprp);
+  int ng[3], wng[3];
+  for (int a = 0; a < 3; ++a) {
+    const double gd = celprp[a] / std::ceil(celprp[a] / gf[a] / griddim) / gf[a];
+    ng[a] = (int)std::ceil(celprp[a] / (gd * gf[a]));
+    wng[a] = (int)std::ceil(celprp[a] / gd);
+  }
+  const long long total = (long long)ng[0] * ng[1] * ng[2] * nstep;
+  WState ws;
+  HIPCHK(hipMemcpy(&ws, h->dw.ws + w, sizeof(WState), hipMemcpyDeviceToHost));
+  const long long chunk = std::min<long long>(total, 1LL << 22);
+  const int nblocks = (int)std::min<long long>((chunk + WIDOM_WPB - 1) / WIDOM_WPB, 4096);
+  float *duni = nullptr;
+  double *dest = nullptr, *dbsum = nullptr;
+  int *dovl = nullptr;
+  long long *dbcnt = nullptr;
+  int rc = FMC_OK;
+  auto fail = [&](int code) {
+    rc = code;
+    return code;
+  };
+  if (hipMalloc(&duni, sizeof(float) * 4 * (size_t)chunk) != hipSuccess ||
+      hipMalloc(&dbsum, sizeof(double) * nblocks) != hipSuccess ||
+      hipMalloc(&dbcnt, sizeof(long long) * nblocks) != hipSuccess ||
+      (estep_out && hipMalloc(&dest, sizeof(double) * (size_t)chunk) != hipSuccess) ||
+      (overlap_out && hipMalloc(&dovl, sizeof(int) * (size_t)chunk) != hipSuccess))
+    fail(FMC_ERR_ALLOC);
+  const WidomLds L = widom_lds(P);
+  const bool sites3 = h->nsite == 3;
+  if (rc == FMC_OK) {
+    auto allow = [&](const void *f) {
+      return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total) == hipSuccess;
+    };
+    const void *kf = h->precision == FMC_FP64
+                         ? (sites3 ? (const void *)k_widom<double, 3> : (const void *)k_widom<double, 0>)
+                         : (sites3 ? (const void *)k_widom<float, 3> : (const void *)k_widom<float, 0>);
+    if (L.total > 160 * 1024 || !allow(kf)) fail(FMC_ERR_ARG);
+  }
+  std::vector<double> bsum(nblocks);
+  std::vector<long long> bcnt(nblocks);
+  double hsum = 0.0;
+  long long accepted = 0;
+  for (long long a0 = 0; rc == FMC_OK && a0 < total; a0 += chunk) {
+    const long long cnt = std::min(chunk, total - a0);
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(64), 0, h->stream, h->dw, w, 4 * cnt, duni);
+    WidomArgs A{};
+    A.w = w;
+    A.g = iguest;
+    A.nstep = nstep;
+    for (int a = 0; a < 3; ++a) {
+      A.ng[a] = ng[a];
+      A.wng[a] = wng[a];
+    }
+    A.a0 = a0;
+    A.count = cnt;
+    A.uni = duni;
+    A.estep = dest;
+    A.overlap = dovl;
+    A.bsum = dbsum;
+    A.bcnt = dbcnt;
+    A.kboltz = KBOLTZ;
+    const int nb = (int)std::min<long long>((cnt + WIDOM_WPB - 1) / WIDOM_WPB, nblocks);
+    if (h->precision == FMC_FP64) {
+      if (sites3) hipLaunchKernelGGL((k_widom<double, 3>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
+      else hipLaunchKernelGGL((k_widom<double, 0>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
+    } else {
+      if (sites3) hipLaunchKernelGGL((k_widom<float, 3>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
+      else hipLaunchKernelGGL((k_widom<float, 0>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+      fail(FMC_ERR_HIP);
+      break;
+    }
+    if (hipMemcpy(bsum.data(), dbsum, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(bcnt.data(), dbcnt, sizeof(long long) * nb, hipMemcpyDeviceToHost) != hipSuccess ||
+        (estep_out && hipMemcpy(estep_out + a0, dest, sizeof(double) * cnt, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (overlap_out && hipMemcpy(overlap_out + a0, dovl, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess)) {
+      fail(FMC_ERR_HIP);
+      break;
+    }
+    for (int b = 0; b < nb; ++b) {
+      hsum += bsum[b];
+      accepted += bcnt[b];
+    }
+  }
+  for (void *p : {(void *)duni, (void *)dest, (void *)dovl, (void *)dbsum, (void *)dbcnt})
+    if (p) hipFree(p);
+  if (rc != FMC_OK) return rc;
+  res->attempts = total;
+  res->accepted = accepted;
+  res->avg_boltz = hsum / (double)total;
+  res->henry = res->avg_boltz / AVO / BOLTZ / ws.temp * 1.0e5;
+  for (int a = 0; a < 3; ++a) {
+    res->ngrid[a] = ng[a];
+    res->wngrid[a] = wng[a];
   }
   return FMC_OK;
 }

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .system import (CSystem, FmcTheta, FmcTrial, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams,
+from .system import (CSystem, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams,
                      FmcWlStatus, TRACE_DTYPE, System, make_walker)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -24,7 +24,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
-           "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad")
+           "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom")
 
 _lib = None
 
@@ -210,6 +210,38 @@ class Handle:
         return U, dq, de, ds
 
     # ---- Wang-Landau (wang_landau.f)
+    def widom(self, iguest: int, nstep: int, griddim: float, gridfactor=(1, 1, 1), w: int = 0,
+              per_attempt: bool = False):
+        """Widom insertions on a grid (fmc_widom): dict of <exp(-beta estep)>, attempts, non-overlapping
+        attempts, Henry's constant as the reference prints it, grid sizes; with per_attempt also the
+        estep and overlap arrays in the reference's loop order."""
+        gf = (C.c_int * 3)(*gridfactor)
+        res = FmcWidomResult()
+        est = ovl = None
+        if per_attempt:
+            # size from the same grid rule, so the arrays can be handed over
+            n = self._widom_attempts(iguest, nstep, griddim, gridfactor)
+            est = np.zeros(n)
+            ovl = np.zeros(n, dtype=np.int32)
+        _chk("fmc_widom", self.L.fmc_widom(self.h, C.c_int(w), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim),
+                                           gf, est.ctypes.data_as(C.c_void_p) if per_attempt else None,
+                                           ovl.ctypes.data_as(C.c_void_p) if per_attempt else None, C.byref(res)))
+        out = {"avg_boltz": res.avg_boltz, "attempts": res.attempts, "accepted": res.accepted, "henry": res.henry,
+               "ngrid": tuple(res.ngrid), "wngrid": tuple(res.wngrid)}
+        if per_attempt:
+            out["estep"], out["overlap"] = est, ovl
+        return out
+
+    def _widom_attempts(self, iguest, nstep, griddim, gridfactor):
+        import math
+        cell = np.asarray(self.sysm.cell, dtype=np.float64).reshape(3, 3)
+        n = nstep
+        for a in range(3):
+            length = float(np.sqrt((cell[a] ** 2).sum()))
+            gd = length / math.ceil(length / gridfactor[a] / griddim) / gridfactor[a]
+            n *= math.ceil(length / (gd * gridfactor[a]))
+        return n
+
     def wl_setup(self, maxn: int, minn: int = 0, nwindows: int = 1, visittol: int = 10000,
                  flatcoeff: float = 0.7, wlprec: float = float(np.e), prectol: float = 1e-8,
                  create_molecules: int = 0):

@@ -598,6 +598,11 @@ class FmcWalkerStats(C.Structure):
                 ("resum_drift", C.c_double)]
 
 
+class FmcWidomResult(C.Structure):
+    _fields_ = [("avg_boltz", C.c_double), ("henry", C.c_double), ("attempts", C.c_int64),
+                ("accepted", C.c_int64), ("ngrid", C.c_int * 3), ("wngrid", C.c_int * 3)]
+
+
 class FmcTheta(C.Structure):
     _fields_ = [("fw_charge", C.POINTER(C.c_double)), ("type_eps", C.POINTER(C.c_double)),
                 ("type_sig", C.POINTER(C.c_double))]

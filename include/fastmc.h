@@ -284,6 +284,27 @@ typedef struct fmc_theta {
 int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int *nmols, const double *xyz,
                     int gstride, int precision, double *U, double *dU_dq, double *dU_deps, double *dU_dsig);
 
+/* Widom insertions on a grid: widom_grid, gcmc.f:2454-2548, with the grid
+ * set-up of gcmc.f:258-281,605-610 (CONTROL "henry", "widom insertions N",
+ * "grid spacing D", "grid factor a b c"; utility_pack.f:390,436-442,
+ * readinputs.f:598-599).  At every grid point (i/wngrida, j/wngridb,
+ * k/wngridc), i = 1..ngrida etc., nstep random orientations (random_rot,
+ * 4 duni draws each, from walker w's generator) of guest iguest's template
+ * are inserted into walker w's current state and rejected.  Outputs:
+ * avg_boltz = sum over non-overlapping attempts of exp(-estep / (kboltz T)),
+ * divided by all attempts (chainstats(istat+7)); henry = the printed Henry's
+ * constant avg_boltz / avo / boltz / T * 1e5 (gcmc.f:1319-1321); per attempt,
+ * in the reference's loop order, estep (0 on overlap) and the overlap flag if
+ * the arrays are given (attempts = ngrida*ngridb*ngridc*nstep entries). */
+typedef struct fmc_widom_result {
+  double avg_boltz, henry;
+  long long attempts, accepted;  /* accepted: attempts without overlap */
+  int ngrid[3], wngrid[3];
+} fmc_widom_result;
+
+int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+              double *estep_out, int *overlap_out, fmc_widom_result *res);
+
 #ifdef __cplusplus
 }
 #endif

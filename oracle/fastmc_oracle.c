@@ -2196,8 +2196,10 @@ int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst
  * exp(-estep / (kboltz T)) over non-overlapping attempts, averaged over all
  * attempts.  out: [0] <exp(-beta estep)>, [1] attempts, [2] non-overlapping
  * attempts, [3] Henry's constant as printed (gcmc.f:1319-1321), [4..6]
- * ngrida, ngridb, ngridc, [7..9] wngrida, wngridb, wngridc. */
-int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[3], double out[10]) {
+ * ngrida, ngridb, ngridc, [7..9] wngrida, wngridb, wngridc.  estep / ovl (may
+ * be NULL): per attempt in loop order, estep 0 on overlap. */
+int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[3], double out[10], double *estep,
+              int *ovl) {
   if (iguest < 1 || iguest > s->ntpguest || nstep < 1 || griddim <= 0.0) return FMC_ERR_ARG;
   double celprp[11];
   dcell(s->cell, celprp);
@@ -2236,9 +2238,11 @@ int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[
             s->newy[a] += cy;
             s->newz[a] += cz;
           }
-          widcount += 1;
           int ov = insertion(s, iguest, &tr);
           for (int m = 0; m < 64; ++m) s->delE[m] = 0.0;
+          if (estep) estep[widcount] = ov ? 0.0 : tr.estep;
+          if (ovl) ovl[widcount] = ov;
+          widcount += 1;
           if (!ov) {
             prodcount += 1;
             hsum += exp(-1.0 * tr.estep / KBOLTZ / s->temp);

@@ -78,17 +78,26 @@ class Oracle:
         except Exception:
             pass
 
-    def widom(self, iguest, nstep, griddim, gridfactor=(1, 1, 1)):
+    def widom(self, iguest, nstep, griddim, gridfactor=(1, 1, 1), max_attempts=0):
         """widom_grid (gcmc.f:2454-2548): dict of <exp(-beta estep)>, attempts,
-        non-overlapping attempts, Henry's constant as printed, grid sizes."""
+        non-overlapping attempts, Henry's constant as printed, grid sizes; with
+        max_attempts > 0 (at least the attempt count) also per-attempt estep and
+        overlap arrays."""
         out = np.zeros(10)
         gf = (C.c_int * 3)(*gridfactor)
+        est = np.zeros(max(max_attempts, 1))
+        ovl = np.zeros(max(max_attempts, 1), dtype=np.int32)
         rc = lib().orc_widom(C.c_void_p(self.h), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim), gf,
-                             out.ctypes.data_as(C.c_void_p))
+                             out.ctypes.data_as(C.c_void_p), est.ctypes.data_as(C.c_void_p) if max_attempts else None,
+                             ovl.ctypes.data_as(C.c_void_p) if max_attempts else None)
         if rc:
             raise RuntimeError(f"orc_widom: {rc}")
-        return {"avg_boltz": out[0], "attempts": int(out[1]), "accepted": int(out[2]), "henry": out[3],
-                "ngrid": tuple(int(x) for x in out[4:7]), "wngrid": tuple(int(x) for x in out[7:10])}
+        r = {"avg_boltz": out[0], "attempts": int(out[1]), "accepted": int(out[2]), "henry": out[3],
+             "ngrid": tuple(int(x) for x in out[4:7]), "wngrid": tuple(int(x) for x in out[7:10])}
+        if max_attempts:
+            assert max_attempts >= r["attempts"]
+            r["estep"], r["overlap"] = est[: r["attempts"]], ovl[: r["attempts"]]
+        return r
 
     def single_point(self):
         out = np.zeros(5)
