@@ -15,6 +15,9 @@ Printed as one JSON line (rank 0, N=1):
   cpu_baseline   -- the reference's serial build on one state point of the
                     same workload, this box's host, one core
   fp64_single    -- plain FP64, one state point, same device kernel
+  widom_grid     -- Widom insertions on a grid (config-0 framework, CO2):
+                    attempts/s on the device and of the reference's serial
+                    build on this box's host
 """
 from __future__ import annotations
 
@@ -47,6 +50,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per walker (0: autotune)")
+    ap.add_argument("--no-widom", action="store_true")
     ap.add_argument("--eager", action="store_true",
                     help="plain launches instead of graph replays (for counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -170,6 +174,50 @@ def reference_cpu_concurrent(state_dir: str, steps: int, nproc: int):
             shutil.rmtree(work, ignore_errors=True)
 
 
+def widom_device(sysm, griddim: float = 0.5, nstep: int = 20):
+    """Widom grid (fmc_widom) on one walker of the config-0 system, FP64 and mixed."""
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    out = {}
+    for name, prec in (("fp64", FMC_FP64), ("mixed", FMC_MIXED)):
+        h = Handle(sysm, precision=prec, seeds=[(12, 34, 56, 78)])
+        try:
+            h.widom(1, 2, 2.0)  # warm-up (kernel load, buffers)
+            t0 = time.perf_counter()
+            r = h.widom(1, nstep, griddim)
+            dt = time.perf_counter() - t0
+        finally:
+            h.close()
+        out[name] = {"attempts_per_s": r["attempts"] / dt, "seconds": dt, "henry": r["henry"]}
+        out["attempts"] = r["attempts"]
+        out["grid"] = list(r["ngrid"])
+    return out
+
+
+def widom_reference_cpu(state_dir: str, griddim: float = 2.0, nstep: int = 5):
+    """The reference's serial build in Widom mode ("henry") on the same system, this host, one core."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "gcmc_ser.x")
+    if not os.path.exists(exe):
+        return None
+    work = tempfile.mkdtemp(prefix="fmc_widcpu_")
+    try:
+        for f in ("FIELD", "CONFIG", "CONTROL"):
+            shutil.copy(os.path.join(state_dir, f), os.path.join(work, f))
+        ctl = open(os.path.join(work, "CONTROL")).read()
+        ctl = ctl.replace("finish", f"henry\nwidom insertions {nstep}\ngrid spacing {griddim}\nfinish")
+        open(os.path.join(work, "CONTROL"), "w").write(ctl)
+        r = subprocess.run([exe], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+        out = open(os.path.join(work, "OUTPUT")).read()
+        m = re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps\s*:\s*([0-9.Ee+-]+)", out)
+        if r.returncode != 0 or not m:
+            return None
+        n, t = int(m.group(1)), float(m.group(2))
+        return {"attempts_per_s": n / t, "cores": 1, "kind": "reference",
+                "sample": f"reference serial build in Widom mode, grid spacing {griddim} A x {nstep} "
+                          f"orientations ({n} attempts, {t:.2f} s)"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def oracle_cpu(sysm, seconds: float):
     from oracle.oracle import Oracle
     o = Oracle(sysm, seed=(31, 41, 59, 26))
@@ -291,6 +339,21 @@ def main():
                 steps = int(cpu["value"] * args.cpu_seconds * 0.5)
                 cpu_mpi = reference_cpu_concurrent(dirs[mid_i], steps, nproc)
         h.close()
+        widom = None
+        if not args.no_widom:
+            from fastmc_amd import inputs as I
+            from fastmc_amd.system import read_system
+            wdir = os.path.join(tmp, "widom")
+            os.makedirs(wdir, exist_ok=True)
+            I.config0(wdir, pressure_bar=1.0)
+            widom = {"workload": "config-0 framework (424 atoms, 871 k-vectors), CO2, grid spacing 0.5 A, "
+                                 "20 orientations per point, empty framework, one walker"}
+            widom.update(widom_device(read_system(wdir)))
+            if not args.no_cpu:
+                widom["cpu_reference"] = widom_reference_cpu(wdir)
+                if widom["cpu_reference"]:
+                    widom["fp64_vs_cpu_reference"] = (widom["fp64"]["attempts_per_s"] /
+                                                      widom["cpu_reference"]["attempts_per_s"])
         line = {
             "metric": METRIC, "value": value, "unit": "trial moves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
@@ -310,6 +373,7 @@ def main():
             "cpu_baseline_concurrent": cpu_mpi,
             "fp64_single": fp64,
             "batched_vs_fp64_single": (value / fp64["value"]) if fp64 else None,
+            "widom_grid": widom,
         }
         print(json.dumps(line), flush=True)
     finally:
