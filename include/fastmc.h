@@ -50,6 +50,9 @@ extern "C" {
 #define FMC_MOVE_DISPLACE 3   /* translate + rotate (mc_displace default branch) */
 #define FMC_MOVE_TRANSLATE 4
 #define FMC_MOVE_ROTATE 5
+#define FMC_MOVE_JUMP 6       /* mc_jump, gcmc.f:1937-2039 */
+#define FMC_MOVE_SWAP 7       /* mc_swap, gcmc.f:2269-2452 */
+#define FMC_MOVE_SWITCH 8     /* mc_switch, gcmc.f:2041-2267 */
 
 /* One molecule type of the FIELD file (readinputs.f:114-281), in FIELD order. */
 typedef struct fmc_species {

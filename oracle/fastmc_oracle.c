@@ -147,6 +147,7 @@ struct orc_state {
   float mcf[FMC_MAX_SPECIES + 1][FMC_NMOVES];
   double gstfuga[FMC_MAX_SPECIES + 1], delr[FMC_MAX_SPECIES + 1];
   double delrdisp[FMC_MAX_SPECIES + 1], tran_delr[FMC_MAX_SPECIES + 1];
+  long long mv_att[FMC_NMOVES + 1], mv_acc[FMC_NMOVES + 1];  /* by move kind, summed over guests */
   double rotangle, rota_rotangle;
   int disp_count[FMC_MAX_SPECIES + 1], accept_disp[FMC_MAX_SPECIES + 1];
   int tran_count[FMC_MAX_SPECIES + 1], accept_tran[FMC_MAX_SPECIES + 1];
@@ -1694,6 +1695,168 @@ int orc_reject(orc_state *s) {
 /* gcmc.f:704-1045 restricted to insert / delete / displace / translate /
  * rotate, with mc_insert (1693-1752), mc_delete (1754-1810) and mc_displace
  * (1812-1935). */
+/* mc_swap, gcmc.f:2269-2452: molecule ichoice of iguest is deleted (accepted
+ * provisionally) and a molecule of another kind jguest inserted at its COM
+ * with the template rotated by random_rot(rotangle) (the displacement's
+ * current angle, as the reference passes it), also accepted provisionally;
+ * then two tests with the reference's own factors (volm in A^3,
+ * gcmc.f:2400-2413).  On rejection the coordinates, S(k), per-species
+ * charges and energies, elrc and nummols are restored (reject_move restores
+ * engsic, elrc_mol and the type counts); sumchg is not, as in the reference.
+ * The provisional insertion runs accept_move even when it overlaps (with the
+ * deletion's structure factor, charge and self-energy still in place), as
+ * the reference does. */
+static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, double *rande_out, int *drew_out,
+                     int *accepted_out, double *rec) {
+  const int maxmls = s->maxmls, imol = s->locguest[iguest], natms = s->numatoms[imol], nmols = s->nummols[imol];
+  if (s->ntpguest < 2) return FMC_ERR_ARG;
+  s->mv_att[FMC_MOVE_SWAP] += 1;
+  double chgsum_orig[FMC_MAX_SPECIES + 1], energy_orig[FMC_MAX_SPECIES + 2], elrc_mol_orig[64];
+  for (int i = 0; i <= FMC_MAX_SPECIES; ++i) chgsum_orig[i] = s->chgsum_mol[i];
+  for (int i = 0; i < FMC_MAX_SPECIES + 2; ++i) energy_orig[i] = s->energy[i];
+  for (int i = 0; i < 64; ++i) {
+    s->engsicorig[i] = s->engsic[i];
+    elrc_mol_orig[i] = s->elrc_mol[i];
+  }
+  const double elrc_orig = s->elrc;
+  int chosen[FMC_MAX_SPECIES + 1], n = 0;
+  for (int j = 1; j <= s->ntpguest; ++j)
+    if (j != iguest) chosen[++n] = j;
+  const int ichoice = (int)floor(mzt_next(&s->rng) * nmols + 1);
+  get_guest(s, iguest, ichoice);
+  double comx, comy, comz;
+  com(s, natms, imol, s->newx, s->newy, s->newz, &comx, &comy, &comz);
+  const int jguest = chosen[(int)floor(mzt_next(&s->rng) * (s->ntpguest - 1) + 1)];
+  const int jmol = s->locguest[jguest], jnatms = s->numatoms[jmol];
+  /* back-ups of both species' coordinates and S(k) rows */
+  const int ni = natms * s->nummols[imol], nj = jnatms * s->nummols[jmol];
+  double *bx = (double *)malloc(sizeof(double) * 3 * (size_t)(ni + nj + 2));
+  if (!bx) return FMC_ERR_ALLOC;
+  for (int a = 1; a <= ni; ++a) {
+    bx[3 * a - 3] = s->molx[imol][a];
+    bx[3 * a - 2] = s->moly[imol][a];
+    bx[3 * a - 1] = s->molz[imol][a];
+  }
+  for (int a = 1; a <= nj; ++a) {
+    bx[3 * (ni + a) - 3] = s->molx[jmol][a];
+    bx[3 * (ni + a) - 2] = s->moly[jmol][a];
+    bx[3 * (ni + a) - 1] = s->molz[jmol][a];
+  }
+  const int rows[3] = {imol, jmol, maxmls + 1};
+  for (int r = 0; r < 3; ++r)
+    for (int ik = 1; ik <= s->newld; ++ik) {
+      SK(ckcsorig, rows[r], ik) = SK(ckcsum, rows[r], ik);
+      SK(ckssorig, rows[r], ik) = SK(ckssum, rows[r], ik);
+    }
+  /* delete molecule ichoice of iguest */
+  fmc_trial td;
+  deletion(s, iguest, ichoice, &td);
+  accept_move(s, FMC_MOVE_DELETE, iguest, ichoice);
+  const double estepi = -td.estep;
+  /* insert a jguest molecule at its COM */
+  for (int a = 1; a <= jnatms; ++a) {
+    s->newx[a] = s->guestx[jguest][a] + comx;
+    s->newy[a] = s->guesty[jguest][a] + comy;
+    s->newz[a] = s->guestz[jguest][a] + comz;
+  }
+  double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+  double q[4];
+  quat_from(r1, r2, r3, r4, s->rotangle, q);
+  rotation(s->newx, s->newy, s->newz, comx, comy, comz, jnatms, q[0], q[1], q[2], q[3]);
+  for (int a = 1; a <= jnatms; ++a) {
+    rec[3 * (a - 1) + 0] = s->newx[a];
+    rec[3 * (a - 1) + 1] = s->newy[a];
+    rec[3 * (a - 1) + 2] = s->newz[a];
+  }
+  fmc_trial ti;
+  const int ov = insertion(s, jguest, &ti);
+  accept_move(s, FMC_MOVE_INSERT, jguest, 0);
+  const double estepj = ov ? 0.0 : ti.estep;
+  int accepted = 0, drew = 0;
+  double rande = -1.0;
+  if (!ov) {
+    rande = mzt_next(&s->rng);
+    drew = 1;
+    const double jpress = s->gstfuga[jguest], ipress = s->gstfuga[iguest];
+    const int jnm = s->nummols[jmol], inm = s->nummols[imol] + 1;
+    const double testi = (double)inm * BOLTZ * s->temp / (s->volm * ipress) * exp(-1.0 * s->beta * estepi);
+    const double testj = s->volm * jpress / ((double)jnm * BOLTZ * s->temp) * exp(-1.0 * s->beta * estepj);
+    if (rande < testi) {
+      rande = mzt_next(&s->rng);
+      if (rande < testj) accepted = 1;
+    }
+  }
+  if (accepted) {
+    s->mv_acc[FMC_MOVE_SWAP] += 1;
+    condense(s);
+  } else {
+    /* reject_move(swap), mc_moves.f:623-673, then mc_swap's restores */
+    for (int i = 0; i < 64; ++i) {
+      s->delE[i] = 0.0;
+      s->engsic[i] = s->engsicorig[i];
+      s->elrc_mol[i] = elrc_mol_orig[i];
+    }
+    for (int a = 1; a <= natms; ++a) {
+      int ka = s->ltpsit[imol][a];
+      s->numtyp[ka] += 1;
+      s->numtyp_mol[imol][ka] += 1;
+      if (s->lfzsite[imol][a] != 0) {
+        s->numfrz[ka] += 1;
+        s->numfrz_mol[imol][ka] += 1;
+      }
+    }
+    for (int a = 1; a <= jnatms; ++a) {
+      int ka = s->ltpsit[jmol][a];
+      s->numtyp[ka] -= 1;
+      s->numtyp_mol[jmol][ka] -= 1;
+      if (s->lfzsite[jmol][a] != 0) {
+        s->numfrz[ka] -= 1;
+        s->numfrz_mol[jmol][ka] -= 1;
+      }
+    }
+    s->nummols[imol] += 1;
+    s->nummols[jmol] -= 1;
+    ensure_cap(s, imol, ni + natms);
+    for (int a = 1; a <= ni; ++a) {
+      s->molx[imol][a] = bx[3 * a - 3];
+      s->moly[imol][a] = bx[3 * a - 2];
+      s->molz[imol][a] = bx[3 * a - 1];
+    }
+    for (int a = 1; a <= nj; ++a) {
+      s->molx[jmol][a] = bx[3 * (ni + a) - 3];
+      s->moly[jmol][a] = bx[3 * (ni + a) - 2];
+      s->molz[jmol][a] = bx[3 * (ni + a) - 1];
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int ik = 1; ik <= s->newld; ++ik) {
+        SK(ckcsum, rows[r], ik) = SK(ckcsorig, rows[r], ik);
+        SK(ckssum, rows[r], ik) = SK(ckssorig, rows[r], ik);
+      }
+    s->chgsum_mol[imol] = chgsum_orig[imol];
+    s->chgsum_mol[jmol] = chgsum_orig[jmol];
+    s->elrc = elrc_orig;
+    condense(s);
+    for (int i = 0; i < FMC_MAX_SPECIES + 2; ++i) s->energy[i] = energy_orig[i];
+  }
+  free(bx);
+  memset(tr, 0, sizeof *tr);
+  tr->estep = estepi + estepj;
+  tr->overlap = ov;
+  *choice_out = ichoice;
+  *rande_out = rande;
+  *drew_out = drew;
+  *accepted_out = accepted;
+  return 0;
+}
+
+int orc_move_counts(const orc_state *s, long long *att, long long *acc) {
+  for (int k = 0; k <= FMC_NMOVES; ++k) {
+    att[k] = s->mv_att[k];
+    acc[k] = s->mv_acc[k];
+  }
+  return 0;
+}
+
 int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
   for (int step = 1; step <= nsteps; ++step) {
     int iguest = 1;
@@ -1705,11 +1868,11 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
     if (randmov < (double)f[0]) kind = FMC_MOVE_INSERT;
     else if (randmov < (double)f[1]) kind = FMC_MOVE_DELETE;
     else if (randmov < (double)f[2]) kind = FMC_MOVE_DISPLACE;
-    else if (randmov < (double)f[3]) return FMC_ERR_ARG;  /* jump: not on this path */
-    else if (randmov < (double)f[4]) return FMC_ERR_ARG;  /* swap */
+    else if (randmov < (double)f[3]) kind = FMC_MOVE_JUMP;
+    else if (randmov < (double)f[4]) kind = FMC_MOVE_SWAP;
     else if (randmov < (double)f[5]) kind = FMC_MOVE_TRANSLATE;
     else if (randmov < (double)f[6]) kind = FMC_MOVE_ROTATE;
-    else if (randmov < (double)f[7]) return FMC_ERR_ARG;  /* switch */
+    else if (randmov < (double)f[7]) kind = FMC_MOVE_SWITCH;
     else kind = FMC_MOVE_DISPLACE;
     /* step-size adaptation, gcmc.f:859-900 */
     if (nmols >= 1 && s->disp_count[iguest] >= 1) {
@@ -1772,12 +1935,14 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
         rec[3 * (i - 1) + 2] = s->newz[i];
       }
       insertion(s, iguest, &tr);
+      s->mv_att[FMC_MOVE_INSERT] += 1;
       if (!tr.overlap) {
         rande = mzt_next(&s->rng);
         drew = 1;
         accepted = energy_eval(s, tr.estep, rande, iguest, FMC_MOVE_INSERT);
       }
       if (accepted) {
+        s->mv_acc[FMC_MOVE_INSERT] += 1;
         accept_move(s, FMC_MOVE_INSERT, iguest, 0);
       } else {
         for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
@@ -1795,12 +1960,63 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       rande = mzt_next(&s->rng);
       drew = 1;
       accepted = energy_eval(s, -tr.estep, rande, iguest, FMC_MOVE_DELETE);
+      s->mv_att[FMC_MOVE_DELETE] += 1;
       if (accepted) {
+        s->mv_acc[FMC_MOVE_DELETE] += 1;
         accept_move(s, FMC_MOVE_DELETE, iguest, choice);
       } else {
         for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
         for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
       }
+    } else if (kind == FMC_MOVE_JUMP) {
+      /* mc_jump, gcmc.f:1937-2039: a displacement to a random point of the
+       * cell (random_jump, utility_pack.f:3359-3380) with a random rotation */
+      s->mv_att[FMC_MOVE_JUMP] += 1;
+      choice = (int)floor(mzt_next(&s->rng) * nmols) + 1;
+      get_guest(s, iguest, choice);
+      double comx, comy, comz;
+      com(s, natms, mol, s->newx, s->newy, s->newz, &comx, &comy, &comz);
+      double ra = mzt_next(&s->rng), rb = mzt_next(&s->rng), rcc = mzt_next(&s->rng);
+      const double *cl = s->cell;
+      double xc = ra * cl[1] + rb * cl[4] + rcc * cl[7];
+      double yc = ra * cl[2] + rb * cl[5] + rcc * cl[8];
+      double zc = ra * cl[3] + rb * cl[6] + rcc * cl[9];
+      double r1 = mzt_next(&s->rng), r2 = mzt_next(&s->rng), r3 = mzt_next(&s->rng), r4 = mzt_next(&s->rng);
+      double q[4];
+      quat_from(r1, r2, r3, r4, 8.0 * atan(1.0), q);
+      displace_eval(s, iguest, choice, &tr, 0, NULL, xc - comx, yc - comy, zc - comz, q);
+      for (int i = 1; i <= natms; ++i) {
+        rec[3 * (i - 1) + 0] = s->newx[i];
+        rec[3 * (i - 1) + 1] = s->newy[i];
+        rec[3 * (i - 1) + 2] = s->newz[i];
+      }
+      if (!tr.overlap) {
+        if (tr.estep < 0.0) {
+          accepted = 1;
+        } else {
+          rande = mzt_next(&s->rng);
+          drew = 1;
+          accepted = energy_eval(s, tr.estep, rande, iguest, FMC_MOVE_DISPLACE);
+        }
+      }
+      if (accepted) {
+        s->mv_acc[FMC_MOVE_JUMP] += 1;
+        accept_move(s, FMC_MOVE_DISPLACE, iguest, choice);
+      } else {
+        restore_sk(s, mol);
+        for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
+      }
+    } else if (kind == FMC_MOVE_SWAP) {
+      int rc = swap_move(s, iguest, &tr, &choice, &rande, &drew, &accepted, rec);
+      if (rc) return rc;
+    } else if (kind == FMC_MOVE_SWITCH) {
+      /* mc_switch, gcmc.f:2041-2267: needs two other guest kinds with
+       * molecules present (nswitchgst >= 2, line 2098); otherwise the step
+       * does nothing */
+      int nsw = 0;
+      for (int g = 1; g <= s->ntpguest; ++g)
+        if (g != iguest && s->nummols[s->locguest[g]] > 0) ++nsw;
+      if (nsw >= 2) return FMC_ERR_ARG;  /* the three-kind switch is not restated */
     } else {
       double a = 0, b = 0, c = 0, q[4] = {0, 0, 0, 0};
       if (kind == FMC_MOVE_TRANSLATE) {
@@ -1839,7 +2055,9 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
           accepted = energy_eval(s, tr.estep, rande, iguest, FMC_MOVE_DISPLACE);
         }
       }
+      s->mv_att[kind] += 1;
       if (accepted) {
+        s->mv_acc[kind] += 1;
         accept_move(s, FMC_MOVE_DISPLACE, iguest, choice);
         if (kind == FMC_MOVE_TRANSLATE) s->accept_tran[iguest] += 1;
         else if (kind == FMC_MOVE_ROTATE) s->accept_rota[iguest] += 1;

@@ -24,6 +24,8 @@ int orc_accept(orc_state *s);
 int orc_reject(orc_state *s);
 
 int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
+/* attempts / acceptances by move kind (index FMC_MOVE_*), summed over guests */
+int orc_move_counts(const orc_state *s, long long *att, long long *acc);
 
 /* Wang-Landau walker over N of guest 1 (wang_landau.f). */
 int orc_wl_setup(orc_state *s, int minmol, int maxmol, int maxn, int preload, double wlprec, double flat,

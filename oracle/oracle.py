@@ -99,6 +99,13 @@ class Oracle:
             r["estep"], r["overlap"] = est[: r["attempts"]], ovl[: r["attempts"]]
         return r
 
+    def move_counts(self):
+        """(attempts, acceptances) by move kind FMC_MOVE_* (index 1..8), summed over guests."""
+        att = np.zeros(9, dtype=np.int64)
+        acc = np.zeros(9, dtype=np.int64)
+        lib().orc_move_counts(C.c_void_p(self.h), att.ctypes.data_as(C.c_void_p), acc.ctypes.data_as(C.c_void_p))
+        return att, acc
+
     def single_point(self):
         out = np.zeros(5)
         lib().orc_single_point(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
