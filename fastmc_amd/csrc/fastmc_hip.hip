@@ -413,6 +413,19 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   if (sys->imcon < 1 || sys->imcon > 3) return FMC_ERR_IMCON;
   if (sys->nspecies < 2 || sys->nspecies > KMAXS || sys->ntypes > KTYPE || sys->ntypes < 1) return FMC_ERR_ARG;
   if (precision != FMC_FP64 && precision != FMC_MIXED) return FMC_ERR_ARG;
+  {
+    // swap needs a second guest kind (mc_swap, gcmc.f:2309-2322); switch is a
+    // step without a move below three guest kinds (gcmc.f:2098) and is not
+    // restated for three or more
+    int nguest = 0;
+    for (int m = 0; m < sys->nspecies; ++m) nguest += sys->species[m].is_guest ? 1 : 0;
+    for (int w = 0; w < nwalkers; ++w)
+      for (int g = 0; g < nguest && g < FMC_MAX_SPECIES; ++g) {
+        const float *c = walkers[w].move_cum[g];
+        if (c[4] > c[3] && nguest < 2) return FMC_ERR_ARG;
+        if (c[7] > c[6] && nguest >= 3) return FMC_ERR_ARG;
+      }
+  }
   fmc_handle *h = new fmc_handle();
   h->nwalkers = nwalkers;
   h->precision = precision;
@@ -826,6 +839,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   rc |= dalloc(h, &h->dw.gz, (size_t)nwalkers * P.gatoms_total);
   rc |= dalloc(h, &h->dw.sk, (size_t)nwalkers * (P.maxmls + 1) * P.nk);
   rc |= dalloc(h, &h->dw.sscr, (size_t)nwalkers * 2 * P.nk);
+  rc |= dalloc(h, &h->dw.sbak, (size_t)nwalkers * (P.maxmls + 1) * P.nk);
   rc |= dalloc(h, (char **)&h->dw.trial_out, 4096);
   rc |= dalloc(h, &h->dscratch, (size_t)nwalkers * 3 * (KMAXC + 1) + 64);
   bool per_walker_fw = false;
@@ -1124,7 +1138,7 @@ int fmc_get_stats(fmc_handle *h, int w, fmc_walker_stats *out) {
   HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
   memset(out, 0, sizeof *out);
   out->steps = s.steps;
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < 9; ++i) {
     out->accepted[i] = s.acc[i];
     out->attempted[i] = s.att[i];
   }
@@ -1152,7 +1166,7 @@ int fmc_reset_stats(fmc_handle *h) {
     s.counted = 0;
     s.sum_e = s.sum_e2 = 0.0;
     for (int i = 0; i <= KMAXS; ++i) s.sum_n[i] = s.sum_n2[i] = 0.0;
-    for (int i = 0; i < 6; ++i) s.acc[i] = s.att[i] = 0;
+    for (int i = 0; i < 9; ++i) s.acc[i] = s.att[i] = 0;
   }
   HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
   return FMC_OK;

@@ -122,7 +122,7 @@ struct WState {
   int pend_kind, pend_iguest, pend_choice, pend_valid;
   double pend_xyz[3 * KSITE];
   // statistics
-  long long steps, acc[6], att[6], counted;
+  long long steps, acc[9], att[9], counted;
   double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
   // Wang-Landau walker: window [wl_min, wl_max] of N, preload target,
   // incremental minimum of the window's visit histogram, f and ln f
@@ -142,6 +142,7 @@ struct DevWalkers {
   double *gx, *gy, *gz;      // [w][gatoms_total]
   double2 *sk;               // [w][maxmls+1][nk]: species rows then the total
   double2 *sscr;             // [w][2][nk]: s_new, s_old of the current trial
+  double2 *sbak;             // [w][maxmls+1][nk]: S(k) rows kept across a swap
   double *fwx, *fwy, *fwz;   // per-walker framework copies (if fw_per_walker)
   fmc_trace_rec *trace;      // walker 0 trace or null
   fmc_trial *trial_out;      // seam output

@@ -21,6 +21,13 @@ struct Smem {
   WState ws;
   int kind, g, mol, natms, choice, nmols, has_new, has_old, excl_lo, excl_hi, nfw;
   int wlmode;  // 0 grand-canonical move, 1 Wang-Landau move, 2 insert_guests preload
+  // move kind of the step (FMC_MOVE_*; jump, swap evaluate as displacement /
+  // deletion + insertion) and the swap's state: phase 1 the provisional
+  // deletion of molecule swap_ichoice of guest swap_ig, phase 2 the
+  // insertion of a guest swap_jg molecule at its COM
+  int mkind, swap_phase, swap_ig, swap_jg, swap_ichoice, swap_ovl, swap_n0;
+  double swap_com[3], swap_estepi, swap_rande;
+  double bk_energy[KMAXS + 2], bk_chg[KMAXS + 1], bk_elrc, bk_elrc_mol[KMAXC + 1], bk_engsic[KMAXC + 1];
   int accepted, drew, ovl_new, ovl_old;
   double xn[KSITE][3], xo[KSITE][3];
   double red[KWAVES][Acc<NS>::NACC];
@@ -63,7 +70,7 @@ struct KSlot {
 __device__ inline int kslot(int m) { return (int)(blockDim.x - 1 - threadIdx.x) + m * (int)blockDim.x; }
 
 template <int NS>
-__device__ inline void load_slots(const DevParams &P, const DevWalkers &W, int w, KSlot<NS> *ks) {
+__device__ __forceinline__ void load_slots(const DevParams &P, const DevWalkers &W, int w, KSlot<NS> *ks) {
   const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
@@ -75,7 +82,7 @@ __device__ inline void load_slots(const DevParams &P, const DevWalkers &W, int w
 }
 
 template <int NS>
-__device__ inline void store_slots(const DevParams &P, const DevWalkers &W, int w, const KSlot<NS> *ks) {
+__device__ __forceinline__ void store_slots(const DevParams &P, const DevWalkers &W, int w, const KSlot<NS> *ks) {
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
@@ -235,19 +242,20 @@ __device__ inline void wl_check(const DevParams &P, WState &ws, const WLView &v)
 // with random_ins, random_disp, random_rot, translate and rotation
 // (utility_pack.f:3161-3563).  Thread 0 only.
 // Move from the cumulative fractions f (gcmc.f:821-844): insert, delete,
-// displace, jump, swap, translate, rotate, switch; jump, swap and switch are
-// not on this path (never configured) and fall to displacement.  skip_ins /
-// skip_del re-choose without that band, as the Wang-Landau loop does at the
-// window edges (wang_landau.f:161-235).
-__device__ inline int choose_move(const float *f, double randmov, bool skip_ins, bool skip_del) {
+// displace, jump, swap, translate, rotate, switch.  gc: the grand-canonical
+// loop; the Wang-Landau loop has no jump, swap or switch and displaces in
+// their bands.  skip_ins / skip_del re-choose without that band, as the
+// Wang-Landau loop does at the window edges (wang_landau.f:161-235).
+__device__ inline int choose_move(const float *f, double randmov, bool skip_ins, bool skip_del, bool gc) {
   if (!skip_ins && randmov < (double)f[0]) return FMC_MOVE_INSERT;
   if (skip_ins && randmov < (double)f[1]) return FMC_MOVE_DELETE;
   if (!skip_ins && !skip_del && randmov < (double)f[1]) return FMC_MOVE_DELETE;
   if (randmov < (double)f[2]) return FMC_MOVE_DISPLACE;
-  if (randmov < (double)f[3]) return FMC_MOVE_DISPLACE;
-  if (randmov < (double)f[4]) return FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[3]) return gc ? FMC_MOVE_JUMP : FMC_MOVE_DISPLACE;
+  if (randmov < (double)f[4]) return gc ? FMC_MOVE_SWAP : FMC_MOVE_DISPLACE;
   if (randmov < (double)f[5]) return FMC_MOVE_TRANSLATE;
   if (randmov < (double)f[6]) return FMC_MOVE_ROTATE;
+  if (randmov < (double)f[7]) return gc ? FMC_MOVE_SWITCH : FMC_MOVE_DISPLACE;
   return FMC_MOVE_DISPLACE;
 }
 
@@ -284,7 +292,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     const int molg = P.locguest[g];
     const double randmov = draw();
     const float *f = ws.mcf[g];
-    kind = uni(choose_move(f, randmov, false, false));
+    kind = uni(choose_move(f, randmov, false, false, !wl_on));
     if (wl_on && g == 1) {
       // window edges of the flat-histogram loop, wang_landau.f:161-235 (the
       // visit bookkeeping is written by every lane with the same values)
@@ -296,13 +304,13 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
         kind = FMC_MOVE_INSERT;
       } else if (nmol <= wmin && kind == FMC_MOVE_DELETE) {
         wl_visit(ws, v, nmol);
-        kind = uni(choose_move(f, randmov, false, true));
+        kind = uni(choose_move(f, randmov, false, true, false));
       }
       if (nmol > wmax) {
         kind = FMC_MOVE_DELETE;
       } else if (nmol >= wmax && kind == FMC_MOVE_INSERT) {
         wl_visit(ws, v, nmol);
-        kind = uni(choose_move(f, randmov, true, false));
+        kind = uni(choose_move(f, randmov, true, false, false));
       }
       if (nmol == 0 && kind != FMC_MOVE_INSERT) kind = 0;  // nothing to move: an empty step
     }
@@ -311,6 +319,8 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   if (kind == 0) {
     if (lane == 0) {
       sh.wlmode = wlmode;
+      sh.mkind = 0;
+      sh.swap_phase = 0;
       sh.kind = 0;
       sh.g = g;
       sh.mol = mol;
@@ -356,11 +366,21 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     }
   }
   if (wlmode == 0 && nmols == 0) kind = FMC_MOVE_INSERT;
+  const int mkind = kind;
+  if (kind == FMC_MOVE_SWITCH) {
+    // mc_switch (gcmc.f:2041-2267) needs two other guest kinds with molecules
+    // present (nswitchgst >= 2, gcmc.f:2098); set-up refuses such systems,
+    // so a switch is a step without a move (and without a count)
+    kind = 0;
+  }
+  if (kind == FMC_MOVE_SWAP) kind = FMC_MOVE_DELETE;  // phase 1: provisional deletion
 
   const bool site = lane < natms;
   const int i = site ? lane : 0;
   int choice = 0;
-  if (kind == FMC_MOVE_INSERT) {
+  if (kind == 0) {
+    // nothing to evaluate
+  } else if (kind == FMC_MOVE_INSERT) {
     // random_ins + rotation of the COM-centred template (utility_pack.f:3220-3331, 3480-3524)
     const double ra = draw(), rb = draw(), rc = draw();
     const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
@@ -381,7 +401,9 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
       sh.xn[i][2] = tz + zc;
     }
   } else if (kind == FMC_MOVE_DELETE) {
-    choice = uni((int)floor(draw() * nmols) + 1);
+    // mc_delete; or mc_swap's choice of molecule (floor(duni*n + 1)) and of
+    // the other guest kind (gcmc.f:2311-2322)
+    choice = uni(mkind == FMC_MOVE_SWAP ? (int)floor(draw() * nmols + 1) : (int)floor(draw() * nmols) + 1);
     const int a = P.gbase[g] + (choice - 1) * natms + i;
     const double x = G.x[a], y = G.y[a], z = G.z[a];
     if (site) {
@@ -389,33 +411,90 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
       sh.xo[i][1] = y;
       sh.xo[i][2] = z;
     }
-  } else {
-    // a translation keeps q = 0, which rotation turns into the inversion
-    // through the COM, as the reference's mc_displace does (gcmc.f:1845,2875)
-    double a = 0.0, b = 0.0, c = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
-    if (kind == FMC_MOVE_TRANSLATE) {
-      const double h = tran_delr * 0.5;
-      a = (2.0 * draw() - 1.0) * h;
-      b = (2.0 * draw() - 1.0) * h;
-      c = (2.0 * draw() - 1.0) * h;
-      if (lane == 0) ws.tran_count[g] = tcnt + 1;
-    } else if (kind == FMC_MOVE_ROTATE) {
-      const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
-      quat_from(r1, r2, r3, r4, rota_rotangle, q);
-      if (lane == 0) ws.rota_count[g] = rcnt + 1;
-    } else {
-      const double h = delrdisp * 0.5;
-      const double da = draw(), db = draw(), dc = draw();
-      a = (2.0 * da - 1.0) * h;
-      b = (2.0 * db - 1.0) * h;
-      c = (2.0 * dc - 1.0) * h;
-      const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
-      quat_from(r1, r2, r3, r4, rotangle, q);
-      if (lane == 0) ws.disp_count[g] = dcnt + 1;
+    if (mkind == FMC_MOVE_SWAP) {
+      double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int k = 0; k < natms; ++k) {
+        const double m = sh.sm[mol][k];
+        wsum += m;
+        cx += rdlane(x, k) * m;
+        cy += rdlane(y, k) * m;
+        cz += rdlane(z, k) * m;
+      }
+      int jg = uni((int)floor(draw() * (P.ntpguest - 1) + 1));
+      jg = jg >= g ? jg + 1 : jg;  // switch_chosen_guest: the guest kinds other than g, in order
+      if (lane == 0) {
+        sh.swap_com[0] = cx / wsum;
+        sh.swap_com[1] = cy / wsum;
+        sh.swap_com[2] = cz / wsum;
+        sh.swap_ig = g;
+        sh.swap_jg = jg;
+        sh.swap_ichoice = choice;
+        sh.swap_n0 = nmols;
+      }
     }
-    choice = uni((int)floor(draw() * nmols) + 1);
-    const int at = P.gbase[g] + (choice - 1) * natms + i;
-    const double ox = G.x[at], oy = G.y[at], oz = G.z[at];
+  } else {
+    // displacement, translation, rotation, jump.  A translation keeps q = 0,
+    // which rotation turns into the inversion through the COM, as the
+    // reference's mc_displace does (gcmc.f:1845,2875)
+    double a = 0.0, b = 0.0, c = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
+    double ox, oy, oz;
+    if (kind == FMC_MOVE_JUMP) {
+      // mc_jump (gcmc.f:1937-2039): molecule first, then random_jump (a point
+      // of the cell) and random_rot(2 pi); the displacement that takes the
+      // old COM there
+      choice = uni((int)floor(draw() * nmols) + 1);
+      const int at = P.gbase[g] + (choice - 1) * natms + i;
+      ox = G.x[at];
+      oy = G.y[at];
+      oz = G.z[at];
+      double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int k = 0; k < natms; ++k) {
+        const double m = sh.sm[mol][k];
+        wsum += m;
+        cx += rdlane(ox, k) * m;
+        cy += rdlane(oy, k) * m;
+        cz += rdlane(oz, k) * m;
+      }
+      cx = cx / wsum;
+      cy = cy / wsum;
+      cz = cz / wsum;
+      const double ra = draw(), rb = draw(), rc = draw();
+      const double *cl = sh.cl9;
+      const double xc = ra * cl[0] + rb * cl[3] + rc * cl[6];
+      const double yc = ra * cl[1] + rb * cl[4] + rc * cl[7];
+      const double zc = ra * cl[2] + rb * cl[5] + rc * cl[8];
+      const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+      quat_from(r1, r2, r3, r4, 2.0 * PI, q);
+      a = xc - cx;
+      b = yc - cy;
+      c = zc - cz;
+    } else {
+      if (kind == FMC_MOVE_TRANSLATE) {
+        const double h = tran_delr * 0.5;
+        a = (2.0 * draw() - 1.0) * h;
+        b = (2.0 * draw() - 1.0) * h;
+        c = (2.0 * draw() - 1.0) * h;
+        if (lane == 0) ws.tran_count[g] = tcnt + 1;
+      } else if (kind == FMC_MOVE_ROTATE) {
+        const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+        quat_from(r1, r2, r3, r4, rota_rotangle, q);
+        if (lane == 0) ws.rota_count[g] = rcnt + 1;
+      } else {
+        const double h = delrdisp * 0.5;
+        const double da = draw(), db = draw(), dc = draw();
+        a = (2.0 * da - 1.0) * h;
+        b = (2.0 * db - 1.0) * h;
+        c = (2.0 * dc - 1.0) * h;
+        const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
+        quat_from(r1, r2, r3, r4, rotangle, q);
+        if (lane == 0) ws.disp_count[g] = dcnt + 1;
+      }
+      choice = uni((int)floor(draw() * nmols) + 1);
+      const int at = P.gbase[g] + (choice - 1) * natms + i;
+      ox = G.x[at];
+      oy = G.y[at];
+      oz = G.z[at];
+    }
     // translate, utility_pack.f:3161-3218: move, then wrap the COM into the cell
     double nx = a + ox, ny = b + oy, nz = c + oz;
     double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
@@ -458,17 +537,64 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   }
   if (lane == 0) {
     sh.wlmode = wlmode;
+    sh.mkind = mkind;
+    sh.swap_phase = mkind == FMC_MOVE_SWAP ? 1 : 0;
     sh.kind = kind;
     sh.g = g;
     sh.mol = mol;
     sh.natms = natms;
     sh.nmols = nmols;
     sh.choice = choice;
-    sh.has_new = kind != FMC_MOVE_DELETE;
-    sh.has_old = kind != FMC_MOVE_INSERT;
-    sh.excl_lo = kind == FMC_MOVE_INSERT ? 0 : (choice - 1) * natms;
-    sh.excl_hi = kind == FMC_MOVE_INSERT ? 0 : choice * natms;
+    sh.has_new = kind != 0 && kind != FMC_MOVE_DELETE;
+    sh.has_old = kind != 0 && kind != FMC_MOVE_INSERT;
+    sh.excl_lo = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : (choice - 1) * natms;
+    sh.excl_hi = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : choice * natms;
     ws.mz_used = used0 + nu;
+  }
+}
+
+// mc_swap's insertion (gcmc.f:2360-2375): a guest swap_jg molecule, its
+// template placed at the deleted molecule's COM and rotated about it by
+// random_rot with the displacement's current angle (rotangle), as the
+// reference passes it.  Wave 0, after the provisional deletion.
+template <int NS>
+__device__ void swap_insert_setup(const DevParams &P, Smem<NS> &sh) {
+  WState &ws = sh.ws;
+  const int lane = threadIdx.x & 63;
+  const int jg = uni(sh.swap_jg), jmol = P.locguest[jg], jnatms = P.natoms[jmol];
+  const long long used0 = ws.mz_used;
+  const int rbits = __float_as_int(ws.rq[(int)((used0 + lane) & 127)]);
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = (double)__int_as_float(__builtin_amdgcn_readlane(rbits, k));
+  double q[4], Q[9];
+  quat_from(r[0], r[1], r[2], r[3], ws.rotangle, q);
+  rot_matrix(q, Q);
+  const double cx = sh.swap_com[0], cy = sh.swap_com[1], cz = sh.swap_com[2];
+  const int i = lane < jnatms ? lane : 0;
+  // newx = guestx + com, then rotation subtracts the COM again
+  const double dx = (sh.tmpl[jg][i][0] + cx) - cx, dy = (sh.tmpl[jg][i][1] + cy) - cy,
+               dz = (sh.tmpl[jg][i][2] + cz) - cz;
+  const double tx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+  const double ty = Q[3] * dx + Q[4] * dy + Q[5] * dz;
+  const double tz = Q[6] * dx + Q[7] * dy + Q[8] * dz;
+  if (lane < jnatms) {
+    sh.xn[lane][0] = tx + cx;
+    sh.xn[lane][1] = ty + cy;
+    sh.xn[lane][2] = tz + cz;
+  }
+  if (lane == 0) {
+    ws.mz_used = used0 + 4;
+    sh.swap_phase = 2;
+    sh.kind = FMC_MOVE_INSERT;
+    sh.g = jg;
+    sh.mol = jmol;
+    sh.natms = jnatms;
+    sh.nmols = ws.nmols[jmol];
+    sh.choice = 0;
+    sh.has_new = 1;
+    sh.has_old = 0;
+    sh.excl_lo = sh.excl_hi = 0;
   }
 }
 
@@ -1008,7 +1134,10 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
   double ewld3 = P.ewald3en[mol];
   double estep = 0.0;
   int overlap = 0;
-  for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = 0.0;
+  const int swap_phase = uni(sh.swap_phase);
+  // a swap's insertion adds to the deletion's delE, as the reference's do
+  if (swap_phase != 2)
+    for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = 0.0;
   if (kind == FMC_MOVE_INSERT) {
     double delrc = sh.delrc;
     overlap = sh.ovl_new;
@@ -1066,6 +1195,40 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
   sh.ovl_new = overlap;
   sh.rande = -1.0;
   sh.drew = 0;
+  if (swap_phase == 1) {
+    // mc_swap's deletion, accepted provisionally (gcmc.f:2345-2353)
+    sh.swap_estepi = -estep;
+    sh.accepted = 1;
+    return;
+  }
+  if (swap_phase == 2) {
+    // mc_swap's test (gcmc.f:2390-2413): the deletion and the insertion each
+    // against the reference's own factors (volume in A^3), the second draw
+    // only when the first passes; jnmols counts the inserted molecule
+    const double estepi = sh.swap_estepi, estepj = overlap ? 0.0 : estep;
+    int acc = 0;
+    if (draw && !overlap) {
+      const int ig = uni(sh.swap_ig), imol = P.locguest[ig];
+      double rande = duni(ws);
+      sh.drew = 1;
+      const double inm = (double)(ws.nmols[imol] + 1), jnm = (double)(ws.nmols[mol] + 1);
+      const double testi = inm * BOLTZ * ws.temp / (P.volm * ws.fug[ig]) * exp(-1.0 * ws.beta * estepi);
+      const double testj = P.volm * ws.fug[g] / (jnm * BOLTZ * ws.temp) * exp(-1.0 * ws.beta * estepj);
+      if (rande < testi) {
+        rande = duni(ws);
+        acc = rande < testj;
+      }
+      sh.rande = rande;
+    }
+    if (acc && ws.nmols[mol] >= P.gcap[g]) {
+      acc = 0;  // guest slots full
+      ws.overflow = 1;
+    }
+    sh.estep = estepi + estepj;
+    sh.swap_ovl = overlap;
+    sh.accepted = acc;
+    return;
+  }
   int accepted = 0;
   if (draw && sh.wlmode == 2) {
     // insert_guests: kept whenever it does not overlap (mc_moves.f:126-142)
@@ -1284,12 +1447,142 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
   __syncthreads();
 }
 
+// mc_swap's back-ups before its provisional deletion (gcmc.f:2297-2343): the
+// S(k) rows (register slots and global rows, each k by the thread that owns
+// it) and the walker totals that a rejected swap restores.
+template <int NS>
+__device__ __forceinline__ void swap_backup(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, KSlot<NS> *ks) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // every row, so that no row is selected by a run-time index (which would
+  // move the register slots to scratch memory)
+  double2 *bak = W.sbak + (size_t)w * (NS + 1) * P.nk;
+  const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    const int k = kslot(m);
+    if (k < P.nk) {
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) bak[(size_t)i * P.nk + k] = ks[m].S[i];
+    }
+  }
+  for (int k = KPT * nt + tid; k < P.nk; k += nt)
+    for (int i = 0; i <= NS; ++i) bak[(size_t)i * P.nk + k] = skw[(size_t)i * P.nk + k];
+  if (tid == 0) {
+    const WState &ws = sh.ws;
+    for (int i = 0; i < KMAXS + 2; ++i) sh.bk_energy[i] = ws.energy[i];
+    for (int i = 0; i <= KMAXS; ++i) sh.bk_chg[i] = ws.chgsum_mol[i];
+    for (int i = 0; i <= KMAXC; ++i) {
+      sh.bk_elrc_mol[i] = ws.elrc_mol[i];
+      sh.bk_engsic[i] = ws.engsic[i];
+    }
+    sh.bk_elrc = ws.elrc;
+  }
+}
+
+// A rejected swap (gcmc.f:2418-2450 and reject_move, mc_moves.f:623-673):
+// S(k) rows and walker totals back, the deleted molecule back in its place.
+// sumchg keeps the provisional insertion's charge as in the reference (the
+// deletion's, if the insertion overlapped: the reference then re-adds the
+// charge its deletion left behind).
+template <int NS, bool LF>
+__device__ __forceinline__ void swap_restore(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
+                             KSlot<NS> *ks) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ig = uni(sh.swap_ig), imol = P.locguest[ig], natms = P.natoms[imol];
+  const double2 *bak = W.sbak + (size_t)w * (NS + 1) * P.nk;
+  double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+#pragma unroll
+  for (int m = 0; m < KPT; ++m) {
+    const int k = kslot(m);
+    if (k < P.nk) {
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) ks[m].S[i] = bak[(size_t)i * P.nk + k];
+    }
+  }
+  for (int k = KPT * nt + tid; k < P.nk; k += nt)
+    for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = bak[(size_t)i * P.nk + k];
+  // molecule swap_ichoice back in place: the ones after it move up one slot
+  const size_t base = (size_t)w * P.gatoms_total + P.gbase[ig];
+  double *lx = nullptr, *ly = nullptr, *lz = nullptr;
+  if constexpr (LF) {
+    lx = (double *)(lds + P.off_g) + P.gbase[ig];
+    ly = lx + P.gatoms_total;
+    lz = ly + P.gatoms_total;
+  }
+  const int nm = uni(sh.ws.nmols[imol]), choice = uni(sh.swap_ichoice);
+  const int lo = (choice - 1) * natms, hi = nm * natms;  // atoms [lo, hi) move to [lo + natms, hi + natms)
+  for (int top = hi; top > lo; top -= nt) {
+    const int j = top - 1 - tid;
+    double x = 0, y = 0, z = 0;
+    if (j >= lo) {
+      if constexpr (LF) {
+        x = lx[j];
+        y = ly[j];
+        z = lz[j];
+      } else {
+        x = W.gx[base + j];
+        y = W.gy[base + j];
+        z = W.gz[base + j];
+      }
+    }
+    __syncthreads();
+    if (j >= lo) {
+      W.gx[base + j + natms] = x;
+      W.gy[base + j + natms] = y;
+      W.gz[base + j + natms] = z;
+      if constexpr (LF) {
+        lx[j + natms] = x;
+        ly[j + natms] = y;
+        lz[j + natms] = z;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < natms) {
+    const double x = sh.xo[tid][0], y = sh.xo[tid][1], z = sh.xo[tid][2];
+    W.gx[base + lo + tid] = x;
+    W.gy[base + lo + tid] = y;
+    W.gz[base + lo + tid] = z;
+    if constexpr (LF) {
+      lx[lo + tid] = x;
+      ly[lo + tid] = y;
+      lz[lo + tid] = z;
+    }
+  }
+  if (tid == 0) {
+    WState &ws = sh.ws;
+    const int jg = sh.swap_jg;
+    for (int i = 0; i < KMAXS + 2; ++i) ws.energy[i] = sh.bk_energy[i];
+    for (int i = 0; i <= KMAXC; ++i) {
+      ws.elrc_mol[i] = sh.bk_elrc_mol[i];
+      ws.engsic[i] = sh.bk_engsic[i];
+    }
+    ws.elrc = sh.bk_elrc;
+    ws.chgsum_mol[imol] = sh.bk_chg[imol];
+    ws.chgsum_mol[P.locguest[jg]] = sh.bk_chg[P.locguest[jg]];
+    ws.sumchg += sh.swap_ovl ? P.chgtmp[ig] : P.chgtmp[jg];
+    for (int i = 0; i < natms; ++i) {
+      const int ka = P.site_t[imol][i];
+      ws.numtyp[ka] += 1;
+      ws.numtyp_mol[imol][ka] += 1;
+      if (P.site_f[imol][i] != 0) {
+        ws.numfrz[ka] += 1;
+        ws.numfrz_mol[imol][ka] += 1;
+      }
+    }
+    ws.nmols[imol] += 1;
+    gstart_from(P, ws.nmols, sh);
+    for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = 0.0;
+  }
+  __syncthreads();
+}
+
 // Full FP64 structure factor of this walker (ewald1, ewald.f:354-661) for
 // the workgroup's k-vectors: register slots and any k beyond them in global
 // memory.  Same arithmetic as k_resum.  Records the largest change of the
 // total row in ws.resum_drift.  Called by every thread.
 template <int NS, bool LF>
-__device__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
+__device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                              KSlot<NS> *ks) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const GuestXYZ<LF> G(P, W, w, lds);
@@ -1433,7 +1726,11 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.sq[i / KSITE][i % KSITE] = P.site_q[i / KSITE][i % KSITE];
     sh.sm[i / KSITE][i % KSITE] = P.site_m[i / KSITE][i % KSITE];
   }
-  if (t == 0) gstart_from(P, W.ws[w].nmols, sh);
+  if (t == 0) {
+    gstart_from(P, W.ws[w].nmols, sh);
+    sh.mkind = 0;
+    sh.swap_phase = 0;
+  }
   if (t <= KMAXS) {
     // guest kinds beyond ntpguest: species 1 (never read), no sites
     const bool real = t >= 1 && t <= P.ntpguest;
@@ -1456,11 +1753,12 @@ __device__ inline void stage_out(const DevWalkers &W, int w, Smem<NS> &sh) {
 template <int NS>
 __device__ void record(const DevParams &P, const DevWalkers &W, Smem<NS> &sh, long long step) {
   fmc_trace_rec *r = W.trace + step;
+  const bool swap = sh.mkind == FMC_MOVE_SWAP;
   r->step = (int)sh.ws.steps;
-  r->iguest = sh.g;
-  r->move = sh.kind;
-  r->choice = sh.choice;
-  r->nmols = sh.nmols;
+  r->iguest = swap ? sh.swap_ig : sh.g;
+  r->move = sh.mkind;
+  r->choice = swap ? sh.swap_ichoice : sh.choice;
+  r->nmols = swap ? sh.swap_n0 : sh.nmols;
   r->overlap = sh.ovl_new;
   r->accepted = sh.accepted;
   r->drew_rande = sh.drew;
@@ -1507,41 +1805,55 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
   }
   const GuestXYZ<LF> G(P, W, w, lds);
   __syncthreads();
-  for (int step = 0; step < nsteps; ++step) {
+  // A swap takes two passes of the loop: its deletion (applied provisionally)
+  // and the insertion of the other kind, accepted or undone as a whole; only
+  // the second pass completes a step.
+  for (int step = 0; step < nsteps;) {
     if (sh.ws.wl_done) break;  // converged Wang-Landau walker (uniform: written before the last barrier)
     if (threadIdx.x < 64) {
-      if (sh.ws.mz_made - sh.ws.mz_used < 33) {
-        mzt_refill(sh.ws, threadIdx.x);
-        if (threadIdx.x == 0) mzt_advance(sh.ws);
+      if (sh.swap_phase == 1) {
+        swap_insert_setup<NS>(P, sh);
+      } else {
+        if (sh.ws.mz_made - sh.ws.mz_used < 33) {
+          mzt_refill(sh.ws, threadIdx.x);
+          if (threadIdx.x == 0) mzt_advance(sh.ws);
+        }
+        propose<NS, LF>(P, G, sh, lds);
       }
-      propose<NS, LF>(P, G, sh, lds);
     }
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
       decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == 0);
-      WState &ws = sh.ws;
-      int kind = sh.kind;
-      ws.att[kind] += 1;
-      if (sh.accepted) {
-        ws.acc[kind] += 1;
-        if (kind == FMC_MOVE_TRANSLATE) ws.accept_tran[sh.g] += 1;
-        else if (kind == FMC_MOVE_ROTATE) ws.accept_rota[sh.g] += 1;
-        else if (kind == FMC_MOVE_DISPLACE) ws.accept_disp[sh.g] += 1;
+      if (sh.swap_phase != 1) {
+        WState &ws = sh.ws;
+        const int mk = sh.mkind;
+        if (mk != FMC_MOVE_SWITCH) ws.att[mk] += 1;
+        if (sh.accepted) {
+          ws.acc[mk] += 1;
+          if (mk == FMC_MOVE_TRANSLATE) ws.accept_tran[sh.g] += 1;
+          else if (mk == FMC_MOVE_ROTATE) ws.accept_rota[sh.g] += 1;
+          else if (mk == FMC_MOVE_DISPLACE) ws.accept_disp[sh.g] += 1;
+        }
+        ws.steps += 1;
       }
-      ws.steps += 1;
     }
     __syncthreads();
+    const int phase = sh.swap_phase;
+    if (phase == 1) swap_backup<NS>(P, W, w, sh, ks);
     if (sh.accepted) {
       apply<NS, LF, true>(P, W, w, sh, lds, ks);
-      if (P.resum_every > 0) {
+      if (phase != 1 && P.resum_every > 0) {
         // apply ends with a barrier, so every thread sees the same count
         int due = sh.ws.acc_since_resum + 1 >= P.resum_every;
         __syncthreads();
         if (threadIdx.x == 0) sh.ws.acc_since_resum += 1;
         if (due) resum_walker<NS, LF>(P, W, w, sh, lds, ks);
       }
+    } else if (phase == 2) {
+      swap_restore<NS, LF>(P, W, w, sh, lds, ks);
     }
+    if (phase == 1) continue;
     if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
     if (threadIdx.x == 0 && count_stats) {
       WState &ws = sh.ws;
@@ -1557,6 +1869,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ws.sum_e2 += e * e;
       ws.counted += 1;
     }
+    ++step;
   }
   store_slots<NS>(P, W, w, ks);
   if (P.wl_nbins > 0) {

@@ -590,7 +590,7 @@ class FmcTraceRec(C.Structure):
 
 
 class FmcWalkerStats(C.Structure):
-    _fields_ = [("steps", C.c_int64), ("accepted", C.c_int64 * 6), ("attempted", C.c_int64 * 6),
+    _fields_ = [("steps", C.c_int64), ("accepted", C.c_int64 * 9), ("attempted", C.c_int64 * 9),
                 ("nmols", C.c_int * FMC_MAX_SPECIES), ("pad", C.c_int),
                 ("energy", C.c_double * FMC_MAX_SPECIES), ("sum_n", C.c_double * FMC_MAX_SPECIES),
                 ("sum_n2", C.c_double * FMC_MAX_SPECIES), ("sum_e", C.c_double),

@@ -123,7 +123,7 @@ typedef struct fmc_trace_rec {
 
 /* Per-walker running totals read back after a run. */
 typedef struct fmc_walker_stats {
-  int64_t steps, accepted[6], attempted[6];  /* index by move kind (1..5) */
+  int64_t steps, accepted[9], attempted[9];  /* index by move kind FMC_MOVE_* (1..8) */
   int nmols[FMC_MAX_SPECIES];
   int pad;
   double energy[FMC_MAX_SPECIES];            /* cumulative per-species energy (FP64) */
