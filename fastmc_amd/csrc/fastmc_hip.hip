@@ -16,6 +16,7 @@
 
 #include "fmc_grad.hpp"
 #include "fmc_widom.hpp"
+#include "fmc_stats.hpp"
 
 using namespace fmc;
 
@@ -527,6 +528,20 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       if (k >= 1 && k <= npairs && ltpvdw[k] < 100 && std::fabs(vvv[(size_t)(k - 1) * P.vstride + 1]) > 1e-10)
         P.pairidx[ti][tj] = (int8_t)(k - 1);
     }
+  // surface_check thresholds (mc_moves.f:26-40): sigma = prmvdw(ivdw,2) of the
+  // type pair's VDW row, zero for a pair with no row of its own
+  for (int ti = 0; ti <= KTYPE; ++ti)
+    for (int tj = 0; tj <= KTYPE; ++tj) {
+      double sg = 0.0;
+      if (ti >= 1 && tj >= 1 && ti <= P.ntypes && tj <= P.ntypes) {
+        int k = lstvdw[loc2(ti, tj)];
+        if (k >= 1 && k <= npairs) sg = psig[k];
+      }
+      P.surf2[ti][tj] = (sys->surftol + sg) * (sys->surftol + sg);
+    }
+  P.nwindsteps = 0;
+  P.accsample = 0;
+  P.maxwin = 0;
   // tail constants, lrcorrect eadd (vdw_terms.f:581-645)
   P.ntri = ntri;
   {
@@ -991,6 +1006,7 @@ int fmc_destroy(fmc_handle *h) {
   for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
   for (void *p : h->allocs) hipFree(p);
   if (h->dtrace) hipFree(h->dtrace);
+  if (h->dw.win) hipFree(h->dw.win);
   for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
                   (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
                   (void *)h->g_vt})
@@ -1152,7 +1168,10 @@ int fmc_get_stats(fmc_handle *h, int w, fmc_walker_stats *out) {
   out->sum_e = s.sum_e;
   out->sum_e2 = s.sum_e2;
   out->counted = s.counted;
-  out->pad = s.overflow;
+  out->overflow = s.overflow;
+  for (int g = 1; g <= P.ntpguest && g <= FMC_MAX_SPECIES; ++g) out->surface[g - 1] = s.surfmols[P.locguest[g]];
+  out->prodcount = s.prodcount;
+  out->windows = s.nwin;
   out->resums = s.nresum;
   out->resum_drift = s.resum_drift;
   return FMC_OK;
@@ -1167,6 +1186,8 @@ int fmc_reset_stats(fmc_handle *h) {
     s.sum_e = s.sum_e2 = 0.0;
     for (int i = 0; i <= KMAXS; ++i) s.sum_n[i] = s.sum_n2[i] = 0.0;
     for (int i = 0; i < 9; ++i) s.acc[i] = s.att[i] = 0;
+    s.prodcount = s.rollcount = s.nwin = 0;
+    memset(s.chain, 0, sizeof s.chain);
   }
   HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
   return FMC_OK;
@@ -1855,6 +1876,55 @@ int fmc_set_resum(fmc_handle *h, int every) {
   h->hp.resum_every = every;
   HIPCHK(hipMemcpy(h->dp, &h->hp, sizeof(DevParams), hipMemcpyHostToDevice));
   return FMC_OK;
+}
+
+int fmc_set_sampling(fmc_handle *h, int nwindsteps, int accepted_only, int max_windows) {
+  if (!h || nwindsteps < 0 || max_windows < 1) return FMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  // captured graphs hold the old window buffer
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  if (h->dw.win) HIPCHK(hipFree(h->dw.win));
+  h->dw.win = nullptr;
+  const size_t n = (size_t)h->nwalkers * (size_t)max_windows;
+  HIPCHK(hipMalloc(&h->dw.win, sizeof(fmc_window) * n));
+  HIPCHK(hipMemset(h->dw.win, 0, sizeof(fmc_window) * n));
+  h->hp.nwindsteps = nwindsteps;
+  h->hp.accsample = accepted_only ? 1 : 0;
+  h->hp.maxwin = max_windows;
+  HIPCHK(hipMemcpy(h->dp, &h->hp, sizeof(DevParams), hipMemcpyHostToDevice));
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  for (auto &s : ws) {
+    s.prodcount = s.rollcount = s.nwin = 0;
+    memset(s.chain, 0, sizeof s.chain);
+    s.overflow &= ~2;
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int fmc_get_windows(fmc_handle *h, int w, fmc_window *out, int cap, int *nwin) {
+  if (!h || w < 0 || w >= h->nwalkers || cap < 0 || (cap > 0 && !out)) return FMC_ERR_ARG;
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  const int n = (int)s.nwin, m = std::min(n, cap);
+  if (nwin) *nwin = n;
+  if (m > 0)
+    HIPCHK(hipMemcpy(out, h->dw.win + (size_t)w * h->hp.maxwin, sizeof(fmc_window) * (size_t)m,
+                     hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+int fmc_production_stats(fmc_handle *h, int w, fmc_guest_stats *out) {
+  if (!h || w < 0 || w >= h->nwalkers || !out) return FMC_ERR_ARG;
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  std::vector<fmc_window> win((size_t)std::max<long long>(s.nwin, 1));
+  int n = 0;
+  int rc = fmc_get_windows(h, w, win.data(), (int)win.size(), &n);
+  if (rc) return rc;
+  return fmc_stats_from_windows(win.data(), n, h->hp.ntpguest, s.temp, out);
 }
 
 }  // extern "C"

@@ -81,6 +81,12 @@ struct DevParams {
   double steadd[KTRI + 1];
   int numtyp_g[KMAXS + 1][KTYPE + 1], numfrz_g[KMAXS + 1][KTYPE + 1];  // numtyp_gstmol
   int8_t pairidx[KTYPE + 1][KTYPE + 1];       // LJ table column or -1 (srfrce skip rule)
+  // surface_check (mc_moves.f:10-44): (surftol + sigma of the type pair)^2,
+  // sigma = prmvdw(ivdw,2) of the pair's VDW row (0 for a pair without one)
+  double surf2[KTYPE + 1][KTYPE + 1];
+  // production statistics (gcmc.f:1050-1130): window length in counted steps
+  // (0: windows close only at the end), accepted steps only, window capacity
+  int nwindsteps, accsample, maxwin;
   // tables
   const double *erc;      // [mxegrd+1]
   const double *vvv;      // [npt][mxegrd+1]
@@ -124,6 +130,12 @@ struct WState {
   // statistics
   long long steps, acc[9], att[9], counted;
   double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
+  // surfacemols by species; production count, steps in the open window,
+  // closed windows, and the open window's sums per guest g of N, E, E*N,
+  // N*N, E*E and the surface count (chainstats, gcmc.f:1062-1075)
+  int surfmols[KMAXS + 1];
+  long long prodcount, rollcount, nwin;
+  double chain[KMAXS + 1][FMC_WIN_STATS];
   // Wang-Landau walker: window [wl_min, wl_max] of N, preload target,
   // incremental minimum of the window's visit histogram, f and ln f
   int wl_on, wl_done, wl_min, wl_max;
@@ -148,6 +160,7 @@ struct DevWalkers {
   fmc_trial *trial_out;      // seam output
   double *wl_lng;            // [w][wl_nbins] ln g(N) (dos_hist)
   int *wl_hist;              // [w][wl_nbins] visits (visit_hist)
+  fmc_window *win;           // [w][maxwin] closed averaging windows
 };
 
 __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872

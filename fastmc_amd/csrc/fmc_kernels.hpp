@@ -25,10 +25,12 @@ struct Smem {
   // deletion + insertion) and the swap's state: phase 1 the provisional
   // deletion of molecule swap_ichoice of guest swap_ig, phase 2 the
   // insertion of a guest swap_jg molecule at its COM
-  int mkind, swap_phase, swap_ig, swap_jg, swap_ichoice, swap_ovl, swap_n0;
+  int mkind, swap_phase, swap_ig, swap_jg, swap_ichoice, swap_ovl, swap_n0, swap_srf;
   double swap_com[3], swap_estepi, swap_rande;
   double bk_energy[KMAXS + 2], bk_chg[KMAXS + 1], bk_elrc, bk_elrc_mol[KMAXC + 1], bk_engsic[KMAXC + 1];
+  int bk_surf[KMAXS + 1];
   int accepted, drew, ovl_new, ovl_old;
+  int srf_new, srf_old;  // surface_check flags of the new / existing position
   double xn[KSITE][3], xo[KSITE][3];
   double red[KWAVES][Acc<NS>::NACC];
   double tot[Acc<NS>::NACC];
@@ -600,8 +602,10 @@ __device__ void swap_insert_setup(const DevParams &P, Smem<NS> &sh) {
 
 // ----------------------------------------------------------- evaluation --
 // One guest-site / atom pair of insertion / guest_energy / deletion: minimum
-// image, overlap test (surface_check, mc_moves.f:40-42), ewald2 and srfrce
-// terms (ewald.f:689-726, vdw_terms.f:418-460).  Branch-free, so that the
+// image, surface and overlap tests (surface_check, mc_moves.f:10-44: bit 0
+// of ovl rsq < overlap^2, bit 1 rsq < sg2 = (surftol + sigma)^2 for a frozen
+// atom, sg2 < 0 otherwise), ewald2 and srfrce terms (ewald.f:689-726,
+// vdw_terms.f:418-460).  Branch-free, so that the
 // pairs of one atom with all trial sites form one straight-line block:
 // table indices are clamped and out-of-range terms selected away.  Geometry
 // and the real-space Coulomb term are FP64 in both modes (pair Coulomb terms
@@ -610,12 +614,12 @@ __device__ void swap_insert_setup(const DevParams &P, Smem<NS> &sh) {
 // ovl2 < 0: a padding site (never overlaps, chgea 0).
 template <typename T>
 __device__ inline void pair_eval(const DevParams &P, const double *erc, const T *vvv, const double *xs, double chgea,
-                                 int col, double ovl2, double xj, double yj, double zj, double qj, double &e2,
-                                 double &ev, int &ovl) {
+                                 int col, double ovl2, double sg2, double xj, double yj, double zj, double qj,
+                                 double &e2, double &ev, int &ovl) {
   double dx = xs[0] - xj, dy = xs[1] - yj, dz = xs[2] - zj;
   min_image(P, dx, dy, dz);
   double rsq = dx * dx + dy * dy + dz * dz;
-  ovl |= rsq < ovl2;
+  ovl |= (int)(rsq < ovl2) | ((int)(rsq < sg2) << 1);
   bool in = rsq < P.rcut2;
   double r = pair_sqrt(rsq);
   const int lmax = P.mxegrd;
@@ -643,10 +647,14 @@ __device__ inline void pair_eval(const DevParams &P, const double *erc, const T 
 //   ltab [KMAXS+1][KSITE][KTYPE+1] int8: guest g, site s, atom type -> LJ column or -1
 //   lsq  [KMAXS+1][KSITE] double: site charge by species
 //   lst  [KMAXS+1][KSITE] int: site type by species
+//   srf  [KMAXS+1][KSITE][KTYPE+1] double: guest g, site s, atom type ->
+//        surface_check threshold (surftol + sigma)^2, -1 for padding sites
 constexpr int CST_LTAB = 0;
 constexpr int CST_LSQ = 1008;
 constexpr int CST_LST = CST_LSQ + 8 * (KMAXS + 1) * KSITE;
-constexpr int CST_BYTES = CST_LST + 4 * (KMAXS + 1) * KSITE;
+constexpr int CST_SRF = CST_LST + 4 * (KMAXS + 1) * KSITE;
+constexpr int CST_BYTES = CST_SRF + 8 * (KMAXS + 1) * KSITE * (KTYPE + 1);
+static_assert(CST_SRF % 8 == 0, "srf alignment");
 static_assert((KMAXS + 1) * KSITE * (KTYPE + 1) <= CST_LSQ, "ltab overlaps lsq");
 
 __device__ inline double2 *ph_site(unsigned char *lds, const DevParams &P, int cfg, int s) {
@@ -692,18 +700,22 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
     int8_t *lt = (int8_t *)(lds + P.off_cst + CST_LTAB);
     double *lsq = (double *)(lds + P.off_cst + CST_LSQ);
     int *lst = (int *)(lds + P.off_cst + CST_LST);
+    double *srf = (double *)(lds + P.off_cst + CST_SRF);
     const int per = KSITE * (KTYPE + 1);
     for (int i = tid; i < (KMAXS + 1) * per; i += nt) {
       int g = i / per, s = (i % per) / (KTYPE + 1), tj = i % (KTYPE + 1);
       int col = -1;
+      double sg2 = -1.0;
       if (g >= 1 && g <= P.ntpguest) {
         int mol = P.locguest[g];
         if (s < P.natoms[mol]) {
           int ts = P.site_t[mol][s];
           col = (LF && sizeof(T) == 4) ? P.lpairidx[ts][tj] : P.pairidx[ts][tj];
+          sg2 = P.surf2[ts][tj];
         }
       }
       lt[i] = (int8_t)col;
+      srf[i] = sg2;
     }
     for (int i = tid; i < (KMAXS + 1) * KSITE; i += nt) {
       lsq[i] = P.site_q[i / KSITE][i % KSITE];
@@ -839,6 +851,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   int ovl_new = 0, ovl_old = 0;
   const double *erc = (const double *)(lds + P.off_erc);
   const int8_t *lt = (const int8_t *)(lds + P.off_cst + CST_LTAB) + g * KSITE * (KTYPE + 1);
+  const double *srf = (const double *)(lds + P.off_cst + CST_SRF) + g * KSITE * (KTYPE + 1);
   const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
   const int *lst = (const int *)(lds + P.off_cst + CST_LST);
   const T *vvv;
@@ -898,7 +911,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   for (int j = tid; j < ntot; j += nt) {
     double xj, yj, zj, qj;
     int tj, spj;
-    bool skip = false;
+    bool skip = false, frz = false;
     if (j < P.nfw) {
       xj = fx[j];
       yj = fy[j];
@@ -907,6 +920,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       int inf = fi[j];
       tj = inf & 255;
       spj = (inf >> 8) & 255;
+      frz = (inf >> 16) & 1;
     } else {
       // the guest kind holding index j, selected without lane-indexed loads
       int gg = 1, start = gstart[1], molg = gmol[1], na = gna[1], slot = gslot[1];
@@ -932,10 +946,12 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     }
     if (skip) continue;
     int col[NSR];
+    double sg2[NSR];
 #pragma unroll
     for (int s = 0; s < NSR; ++s) {
       if (NSITE == 0 && s >= natms) break;
       col[s] = lt[s * (KTYPE + 1) + tj];
+      sg2[s] = frz ? srf[s * (KTYPE + 1) + tj] : -1.0;
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -947,10 +963,10 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         if (NSITE == 0 && s >= natms) break;
         double e2, ev;
         if constexpr (NSITE > 0)
-          pair_eval<T>(P, erc, vvv, xs[c][s], cq[s], col[s], ov2[s], xj, yj, zj, qj, e2, ev, ov);
+          pair_eval<T>(P, erc, vvv, xs[c][s], cq[s], col[s], ov2[s], sg2[s], xj, yj, zj, qj, e2, ev, ov);
         else
-          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, xj, yj, zj, qj,
-                       e2, ev, ov);
+          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, sg2[s], xj, yj,
+                       zj, qj, e2, ev, ov);
         er += e2;
         evd += ev;
       }
@@ -1061,10 +1077,12 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   // ---- workgroup reduction: within each wave by recursive halving (the
   // accumulators of an absent configuration hold zeros), then across waves
   wave_sum_rows<A::NACC>(acc, lane, sh.red[wave]);
-  unsigned long long bn = __ballot(ovl_new != 0), bo = __ballot(ovl_old != 0);
+  // overlap (bit 0) and surface (bit 1) flags of the wave
+  unsigned long long bn = __ballot(ovl_new & 1), bo = __ballot(ovl_old & 1);
+  unsigned long long sn = __ballot(ovl_new & 2), so = __ballot(ovl_old & 2);
   if (lane == 0) {
-    sh.ovlw[wave][0] = bn != 0ULL;
-    sh.ovlw[wave][1] = bo != 0ULL;
+    sh.ovlw[wave][0] = (int)(bn != 0ULL) | ((int)(sn != 0ULL) << 1);
+    sh.ovlw[wave][1] = (int)(bo != 0ULL) | ((int)(so != 0ULL) << 1);
   }
   __syncthreads();
   // wave 0 finishes the sums; the decision that reads them runs on wave 0
@@ -1087,8 +1105,10 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         a |= sh.ovlw[wv][0];
         b |= sh.ovlw[wv][1];
       }
-      sh.ovl_new = a;
-      sh.ovl_old = b;
+      sh.ovl_new = a & 1;
+      sh.ovl_old = b & 1;
+      sh.srf_new = a >> 1;
+      sh.srf_old = b >> 1;
     }
   }
 }
@@ -1106,6 +1126,7 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
     // an empty Wang-Landau step (no molecule to move): only the flatness test
     sh.estep = 0.0;
     sh.ovl_new = 0;
+    sh.srf_new = sh.srf_old = 0;
     sh.rande = -1.0;
     sh.drew = 0;
     sh.accepted = 0;
@@ -1440,6 +1461,8 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
       }
       ws.elrc += sg * sh.delrc;
       for (int t = 1; t <= KMAXC; ++t) ws.elrc_mol[t] += sg * sh.delrc_mol[t];
+      // accept_move's surface tally (mc_moves.f:526,568)
+      if (kind == FMC_MOVE_INSERT ? sh.srf_new : sh.srf_old) ws.surfmols[mol] += dn;
       ws.nmols[mol] += dn;
       gstart_from(P, ws.nmols, sh);
     }
@@ -1476,6 +1499,8 @@ __device__ __forceinline__ void swap_backup(const DevParams &P, const DevWalkers
       sh.bk_engsic[i] = ws.engsic[i];
     }
     sh.bk_elrc = ws.elrc;
+    for (int i = 0; i <= KMAXS; ++i) sh.bk_surf[i] = ws.surfmols[i];
+    sh.swap_srf = sh.srf_old;
   }
 }
 
@@ -1558,6 +1583,7 @@ __device__ __forceinline__ void swap_restore(const DevParams &P, const DevWalker
       ws.engsic[i] = sh.bk_engsic[i];
     }
     ws.elrc = sh.bk_elrc;
+    for (int i = 0; i <= KMAXS; ++i) ws.surfmols[i] = sh.bk_surf[i];
     ws.chgsum_mol[imol] = sh.bk_chg[imol];
     ws.chgsum_mol[P.locguest[jg]] = sh.bk_chg[P.locguest[jg]];
     ws.sumchg += sh.swap_ovl ? P.chgtmp[ig] : P.chgtmp[jg];
@@ -1778,7 +1804,68 @@ __device__ void record(const DevParams &P, const DevWalkers &W, Smem<NS> &sh, lo
   r->delrdisp = sh.ws.delrdisp[sh.g];
   r->rotangle = sh.ws.rotangle;
   r->rota_rotangle = sh.ws.rota_rotangle;
-  r->pad = 0.0;
+  r->surface = sh.srf_new | ((swap ? sh.swap_srf : sh.srf_old) << 1);
+  r->nsurface = sh.ws.surfmols[P.locguest[swap ? sh.swap_ig : sh.g]];
+}
+
+// mc_displace / mc_jump's surface tally after an accepted move of an existing
+// molecule (gcmc.f:1906-1912,2021-2027).  Thread 0.
+template <int NS>
+__device__ inline void surf_displaced(Smem<NS> &sh) {
+  int &n = sh.ws.surfmols[sh.mol];
+  if (sh.srf_old && !sh.srf_new) {
+    n -= 1;
+    if (n < 0) n = 0;
+  } else if (!sh.srf_old && sh.srf_new) {
+    n += 1;
+  }
+}
+
+// The production block after a step (gcmc.f:1050-1130), on wave 0: count the
+// step (only if accepted with the 'sampling' directive), add N, E, E*N, N*N,
+// E*E and the surface count of every guest kind to the open window (lane
+// 6(g-1)+q holds sum q of guest g), and close the window into W.win when the
+// production count reaches a multiple of nwindsteps or, with close, after
+// the run's last step.
+template <int NS>
+__device__ inline void stats_step(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, bool close) {
+  WState &ws = sh.ws;
+  const int lane = threadIdx.x;
+  const int ng = P.ntpguest;
+  const bool take = !P.accsample || sh.accepted;
+  const long long pc = ws.prodcount + (take ? 1 : 0);
+  const long long rc = ws.rollcount + (take ? 1 : 0);
+  const bool active = lane < FMC_WIN_STATS * ng;
+  const int g = lane / FMC_WIN_STATS + 1, q = lane % FMC_WIN_STATS;
+  double sum = 0.0;
+  if (active) {
+    const int m = P.locguest[g];
+    const double n = (double)ws.nmols[m], e = ws.energy[m];
+    double v = n;
+    if (q == 1) v = e;
+    else if (q == 2) v = e * n;
+    else if (q == 3) v = n * n;
+    else if (q == 4) v = e * e;
+    else if (q == 5) v = (double)ws.surfmols[m];
+    sum = ws.chain[g][q] + (take ? v : 0.0);
+  }
+  const bool shut = (P.nwindsteps > 0 && pc % P.nwindsteps == 0) || close;
+  const long long nw = ws.nwin;
+  const bool room = nw < P.maxwin && W.win != nullptr;
+  if (shut && room) {
+    fmc_window *rec = W.win + (size_t)w * P.maxwin + nw;
+    if (lane == 0) rec->count = (double)rc;
+    if (active) rec->sums[g - 1][q] = sum;
+  }
+  if (active) ws.chain[g][q] = shut ? 0.0 : sum;
+  if (lane == 0) {
+    ws.prodcount = pc;
+    ws.rollcount = shut ? 0 : rc;
+    if (shut) {
+      if (room) ws.nwin = nw + 1;
+      else ws.overflow |= 2;
+    }
+  }
 }
 
 // ------------------------------------------------------------- kernels --
@@ -1843,6 +1930,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     if (phase == 1) swap_backup<NS>(P, W, w, sh, ks);
     if (sh.accepted) {
       apply<NS, LF, true>(P, W, w, sh, lds, ks);
+      if (threadIdx.x == 0 && sh.kind != FMC_MOVE_INSERT && sh.kind != FMC_MOVE_DELETE) surf_displaced<NS>(sh);
       if (phase != 1 && P.resum_every > 0) {
         // apply ends with a barrier, so every thread sees the same count
         int due = sh.ws.acc_since_resum + 1 >= P.resum_every;
@@ -1855,6 +1943,8 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     }
     if (phase == 1) continue;
     if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
+    if (threadIdx.x < 64 && (count_stats & FMC_STATS_COUNT))
+      stats_step<NS>(P, W, w, sh, (count_stats & FMC_STATS_CLOSE) && step == nsteps - 1);
     if (threadIdx.x == 0 && count_stats) {
       WState &ws = sh.ws;
       double e = 0.0;
@@ -1928,7 +2018,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     o->vdw = sh.comp[2];
     o->tail = sh.comp[3];
     o->overlap = sh.ovl_new;
-    o->pad = 0;
+    o->surface = sh.srf_new | (sh.srf_old << 1);
     // keep what accept needs
     WState &ws = sh.ws;
     ws.pend_valid = 1;
@@ -1936,6 +2026,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     keep[0] = sh.delrc;
     for (int i = 0; i <= KMAXS; ++i) keep[1 + i] = sh.delE[i];
     for (int i = 0; i <= KMAXC; ++i) keep[2 + KMAXS + i] = sh.delrc_mol[i];
+    keep[3 + KMAXS + KMAXC] = (double)(sh.srf_new | (sh.srf_old << 1));
   }
   stage_out<NS>(W, w, sh);
 }
@@ -1960,6 +2051,9 @@ __global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp
     sh.delrc = keep[0];
     for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = keep[1 + i];
     for (int i = 0; i <= KMAXC; ++i) sh.delrc_mol[i] = keep[2 + KMAXS + i];
+    const int sf = (int)keep[3 + KMAXS + KMAXC];
+    sh.srf_new = sf & 1;
+    sh.srf_old = sf >> 1;
     if (sh.kind == FMC_MOVE_TRANSLATE || sh.kind == FMC_MOVE_ROTATE) sh.kind = FMC_MOVE_DISPLACE;
     ws.pend_valid = 0;
     ws.pend_kind = 0;

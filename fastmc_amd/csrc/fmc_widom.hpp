@@ -237,7 +237,8 @@ __global__ void __launch_bounds__(256) k_widom(const DevParams *__restrict__ Pp,
 #pragma unroll
       for (int s = 0; s < NSA; ++s) {
         double e2, e6;
-        pair_eval<double>(P, erc, vvv, xs[s], cq[s], col[s * (KTYPE + 1) + tj], ov2[s], xj, yj, zj, qj, e2, e6, ov);
+        pair_eval<double>(P, erc, vvv, xs[s], cq[s], col[s * (KTYPE + 1) + tj], ov2[s], -1.0, xj, yj, zj, qj, e2, e6,
+                          ov);
         er += e2;
         ev += e6;
       }

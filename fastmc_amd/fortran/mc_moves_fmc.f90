@@ -17,8 +17,10 @@
 !
 ! Not mirrored to the host: the trial structure factors ckcsnew/ckssnew and
 ! the running ckcsum/ckssum (S(k) lives on the device; gcmc.f only backs it
-! up and restores it around displacements, mc_displace gcmc.f:1863-1930),
-! and the surface-atom flags (lsurf: statistics only, always .false.).
+! up and restores it around displacements, mc_displace gcmc.f:1863-1930).
+! The surface-atom flags (surface_check, mc_moves.f:10-44) come from the
+! device trial; for the molecule in place of a displacement that takes one
+! extra zero-displacement trial, whose flag bit 1 is the position's.
 module mc_moves
   use, intrinsic :: iso_c_binding
   use readinputs
@@ -58,7 +60,7 @@ module mc_moves
     real(c_double) :: estep
     real(c_double) :: dele(8)
     real(c_double) :: recip, creal, vdw, tail
-    integer(c_int) :: overlap, pad
+    integer(c_int) :: overlap, surface
   end type fmc_trial_c
 
   interface
@@ -402,6 +404,7 @@ contains
     fmc_pending = 1
     call fmc_log_trial(1, iguest, 0, natms, xyz, t)
     loverlap = t%overlap /= 0
+    lnewsurf = btest(t%surface, 0)
     if (loverlap) return
     call fmc_selfterms(mol, natms, alpha, epsq, .false., chgtmp, engsictmp)
     estep = estep + t%estep
@@ -451,6 +454,13 @@ contains
       enddo
       fmc_disp_iguest = iguest
       fmc_disp_choice = (ind(1) - atmadd - 1) / natms + 1
+      ! surface flag of the position in place: a zero displacement, rejected
+      call fmc_pack(natms, xyz)
+      rc = fmc_trial_displace(fmc_h, 0_c_int, int(iguest, c_int), int(fmc_disp_choice, c_int), xyz, t)
+      if (rc /= 0) call fmc_fail('fmc_trial_displace', rc)
+      lsurf = btest(t%surface, 1)
+      rc = fmc_reject(fmc_h, 0_c_int)
+      if (rc /= 0) call fmc_fail('fmc_reject', rc)
       return
     endif
     if (fmc_disp_iguest /= iguest) call fmc_fail('guest_energy without a molecule in place', 9002_c_int)
@@ -461,6 +471,7 @@ contains
     fmc_disp_iguest = 0
     call fmc_log_trial(3, iguest, fmc_disp_choice, natms, xyz, t)
     loverlap = t%overlap /= 0
+    lsurf = btest(t%surface, 0)
     estep = t%estep
     ewld1eng = t%recip * engunit
     ewld2sum = t%creal * engunit
@@ -500,6 +511,7 @@ contains
     if (rc /= 0) call fmc_fail('fmc_trial_delete', rc)
     fmc_pending = 1
     call fmc_log_trial(2, iguest, choice, natms, xyz, t)
+    linitsurf = btest(t%surface, 1)
     call fmc_selfterms(mol, natms, alpha, epsq, .true., chgtmp, engsictmp)
     estep = estep + t%estep
     do i = 1, maxmls

@@ -10,21 +10,25 @@ import os
 
 import numpy as np
 
-from .system import (CSystem, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec, FmcWalker, FmcWalkerStats, FmcWlParams,
-                     FmcWlStatus, TRACE_DTYPE, System, make_walker)
+from .system import (CSystem, FmcGuestStats, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec, FmcWalker,
+                     FmcWalkerStats, FmcWindow, FmcWlParams, FmcWlStatus, TRACE_DTYPE, System, make_walker)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
 
 FMC_FP64 = 0
 FMC_MIXED = 1
+FMC_STATS_COUNT = 1
+FMC_STATS_CLOSE = 2
+STAT_NAMES = ("<N>", "<E>", "<E*N>", "<N*N>", "<E*E>", "<NF>", "Qst", "Cv")
 
 EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_full_resum",
            "fmc_trial_insert", "fmc_trial_delete", "fmc_trial_displace", "fmc_accept",
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
-           "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom")
+           "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
+           "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats")
 
 _lib = None
 
@@ -56,6 +60,24 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 def _chk(fn, rc):
     if rc != 0:
         raise FastMCError(fn, rc)
+
+
+def _guest_stats(arr, ntpguest):
+    return [{"avg": dict(zip(STAT_NAMES, arr[g].avg)), "std": dict(zip(STAT_NAMES, arr[g].std)),
+             "avg_final": dict(zip(STAT_NAMES, arr[g].avg_final)),
+             "std_final": dict(zip(STAT_NAMES, arr[g].std_final))} for g in range(ntpguest)]
+
+
+def stats_from_windows(windows, nwin: int, ntpguest: int, temperature: float):
+    """The reference's end-of-run statistics (gcmc.f:1148-1258, avunion /
+    stdunion gcmc.f:1591-1664) from closed averaging windows (an FmcWindow
+    array): per guest type a dict of avg / std (node report) and avg_final /
+    std_final (final report) keyed by STAT_NAMES.  Host arithmetic only."""
+    L = load_library()
+    out = (FmcGuestStats * ntpguest)()
+    _chk("fmc_stats_from_windows", L.fmc_stats_from_windows(windows, C.c_int(nwin), C.c_int(ntpguest),
+                                                             C.c_double(temperature), out))
+    return _guest_stats(out, ntpguest)
 
 
 class Handle:
@@ -134,7 +156,9 @@ class Handle:
         _chk("fmc_reject", self.L.fmc_reject(self.h, C.c_int(w)))
 
     # ---- device chains
-    def run(self, nsteps: int, trace: bool = False, count_stats: bool = False):
+    def run(self, nsteps: int, trace: bool = False, count_stats=False):
+        """nsteps steps of every walker; count_stats: False / True (production
+        steps) or fmc_run's flags (FMC_STATS_COUNT | FMC_STATS_CLOSE)."""
         tr = np.zeros(nsteps, dtype=TRACE_DTYPE) if trace else None
         ptr = tr.ctypes.data_as(C.c_void_p) if trace else None
         _chk("fmc_run", self.L.fmc_run(self.h, C.c_int(nsteps), ptr, C.c_int(int(count_stats))))
@@ -182,6 +206,25 @@ class Handle:
 
     def reset_stats(self):
         _chk("fmc_reset_stats", self.L.fmc_reset_stats(self.h))
+
+    # ---- production statistics
+    def set_sampling(self, nwindsteps: int, accepted_only: bool = False, max_windows: int = 128):
+        _chk("fmc_set_sampling", self.L.fmc_set_sampling(self.h, C.c_int(nwindsteps), C.c_int(int(accepted_only)),
+                                                         C.c_int(max_windows)))
+
+    def windows(self, w: int = 0):
+        """(closed windows of walker w as an FmcWindow array, their count)."""
+        n = C.c_int()
+        _chk("fmc_get_windows", self.L.fmc_get_windows(self.h, C.c_int(w), None, C.c_int(0), C.byref(n)))
+        buf = (FmcWindow * max(n.value, 1))()
+        _chk("fmc_get_windows", self.L.fmc_get_windows(self.h, C.c_int(w), buf, C.c_int(n.value), C.byref(n)))
+        return buf, n.value
+
+    def production_stats(self, w: int = 0):
+        ng = len(self.sysm.guests)
+        out = (FmcGuestStats * ng)()
+        _chk("fmc_production_stats", self.L.fmc_production_stats(self.h, C.c_int(w), out))
+        return _guest_stats(out, ng)
 
     # ---- energy gradient with respect to framework parameters
     def energy_grad(self, fw_charge, type_eps, type_sig, nmols, xyz, precision: int = FMC_FP64,

@@ -187,6 +187,9 @@ class System:
     surftol: float = 0.0
     steps: int = 1
     eqsteps: int = 0
+    nwind: int = 0                  # 'averaging window' (0: not given, initscan default)
+    accsample: bool = False         # 'sampling': count accepted steps only
+    nnumg: int = 0                  # 'numguests' interval (0: not given, readcontrol default)
     pressures: list = field(default_factory=list)   # Pa per guest
     moves: list = field(default_factory=list)       # per guest dict key -> fraction
     guest_delr: list = field(default_factory=list)
@@ -423,6 +426,7 @@ def read_control(path: str, nguests: int):
         lines = f.read().split("\n")
     out = {"temperature": None, "rcut": None, "delr": None, "eps": None, "alpha": None,
            "kmax": None, "rvdw": None, "overlap": 1.5, "surftol": 0.0, "steps": 1, "eqsteps": 0,
+           "nwind": 0, "accsample": False, "nnumg": 0,
            "pressures": [-1.0] * nguests, "moves": [dict() for _ in range(nguests)],
            "guest_delr": [None] * nguests, "topmoves": {}}
     i = 1
@@ -455,6 +459,13 @@ def read_control(path: str, nguests: int):
             continue
         if "temp" in rec:
             out["temperature"], _ = dblstr(rec)
+        elif "averaging window" in rec:
+            out["nwind"], _ = intstr(rec)           # utility_pack.f:394-396
+        elif "sampling" in rec:
+            out["accsample"] = True                 # readinputs.f:608-613
+        elif "numg" in rec:
+            n, _ = intstr(rec)                      # readinputs.f:713-717
+            out["nnumg"] = n if n > 0 else 1000
         elif "steps" in rec:
             out["steps"], _ = intstr(rec)
         elif "equilibr" in rec:
@@ -544,6 +555,7 @@ def read_system(dirpath: str) -> System:
                   rvdw=ctl["rvdw"] if ctl["rvdw"] else rcut, delr=ctl["delr"], alpha=alpha,
                   kmax=kmax, eps_precision=ctl["eps"] or 0.0, overlap=ctl["overlap"],
                   surftol=ctl["surftol"], steps=ctl["steps"], eqsteps=ctl["eqsteps"],
+                  nwind=ctl["nwind"], accsample=ctl["accsample"], nnumg=ctl["nnumg"],
                   pressures=ctl["pressures"], moves=moves, guest_delr=gdelr)
 
 
@@ -576,7 +588,7 @@ class FmcWalker(C.Structure):
 class FmcTrial(C.Structure):
     _fields_ = [("estep", C.c_double), ("delE", C.c_double * FMC_MAX_SPECIES),
                 ("recip", C.c_double), ("real", C.c_double), ("vdw", C.c_double),
-                ("tail", C.c_double), ("overlap", C.c_int), ("pad", C.c_int)]
+                ("tail", C.c_double), ("overlap", C.c_int), ("surface", C.c_int)]
 
 
 class FmcTraceRec(C.Structure):
@@ -586,16 +598,29 @@ class FmcTraceRec(C.Structure):
                 ("xyz", C.c_double * (3 * FMC_MAX_SITES)),
                 ("energy", C.c_double * FMC_MAX_SPECIES), ("delE", C.c_double * FMC_MAX_SPECIES),
                 ("delrdisp", C.c_double), ("rotangle", C.c_double),
-                ("rota_rotangle", C.c_double), ("pad", C.c_double)]
+                ("rota_rotangle", C.c_double), ("surface", C.c_int), ("nsurface", C.c_int)]
 
 
 class FmcWalkerStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("accepted", C.c_int64 * 9), ("attempted", C.c_int64 * 9),
-                ("nmols", C.c_int * FMC_MAX_SPECIES), ("pad", C.c_int),
+                ("nmols", C.c_int * FMC_MAX_SPECIES), ("overflow", C.c_int),
                 ("energy", C.c_double * FMC_MAX_SPECIES), ("sum_n", C.c_double * FMC_MAX_SPECIES),
                 ("sum_n2", C.c_double * FMC_MAX_SPECIES), ("sum_e", C.c_double),
                 ("sum_e2", C.c_double), ("counted", C.c_int64), ("resums", C.c_int64),
-                ("resum_drift", C.c_double)]
+                ("resum_drift", C.c_double), ("surface", C.c_int * FMC_MAX_SPECIES),
+                ("prodcount", C.c_int64), ("windows", C.c_int64)]
+
+
+FMC_WIN_STATS = 6
+
+
+class FmcWindow(C.Structure):
+    _fields_ = [("count", C.c_double), ("sums", (C.c_double * FMC_WIN_STATS) * FMC_MAX_SPECIES)]
+
+
+class FmcGuestStats(C.Structure):
+    _fields_ = [("avg", C.c_double * 8), ("std", C.c_double * 8), ("avg_final", C.c_double * 8),
+                ("std_final", C.c_double * 8)]
 
 
 class FmcWidomResult(C.Structure):
@@ -625,7 +650,8 @@ TRACE_DTYPE = np.dtype([("step", "i4"), ("iguest", "i4"), ("move", "i4"), ("choi
                         ("drew_rande", "i4"), ("estep", "f8"), ("rande", "f8"),
                         ("xyz", "f8", (3 * FMC_MAX_SITES,)), ("energy", "f8", (FMC_MAX_SPECIES,)),
                         ("delE", "f8", (FMC_MAX_SPECIES,)), ("delrdisp", "f8"),
-                        ("rotangle", "f8"), ("rota_rotangle", "f8"), ("pad", "f8")])
+                        ("rotangle", "f8"), ("rota_rotangle", "f8"), ("surface", "i4"),
+                        ("nsurface", "i4")])
 assert TRACE_DTYPE.itemsize == C.sizeof(FmcTraceRec)
 
 

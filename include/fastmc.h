@@ -108,7 +108,8 @@ typedef struct fmc_trial {
   double delE[FMC_MAX_SPECIES];     /* per-species energy change (delE) */
   double recip, real, vdw, tail;    /* components in FIELD units, for diagnostics */
   int overlap;                      /* loverlap */
-  int pad;
+  int surface;                      /* surface_check (mc_moves.f:10-44): bit 0 the new / inserted
+                                       position (lnewsurf, lsurf), bit 1 the existing one (linitsurf) */
 } fmc_trial;
 
 /* One GCMC step of a device-driven chain (the fields a trace replay compares). */
@@ -118,14 +119,16 @@ typedef struct fmc_trace_rec {
   double xyz[3 * FMC_MAX_SITES];
   double energy[FMC_MAX_SPECIES];
   double delE[FMC_MAX_SPECIES];
-  double delrdisp, rotangle, rota_rotangle, pad;
+  double delrdisp, rotangle, rota_rotangle;
+  int surface;                       /* fmc_trial.surface of the step's trial */
+  int nsurface;                      /* surfacemols of the moved guest's species after the step */
 } fmc_trace_rec;
 
 /* Per-walker running totals read back after a run. */
 typedef struct fmc_walker_stats {
   int64_t steps, accepted[9], attempted[9];  /* index by move kind FMC_MOVE_* (1..8) */
   int nmols[FMC_MAX_SPECIES];
-  int pad;
+  int overflow;                              /* bit 0: guest slots full; bit 1: window buffer full */
   double energy[FMC_MAX_SPECIES];            /* cumulative per-species energy (FP64) */
   double sum_n[FMC_MAX_SPECIES];             /* sum of N over counted steps */
   double sum_n2[FMC_MAX_SPECIES];
@@ -133,6 +136,9 @@ typedef struct fmc_walker_stats {
   int64_t counted;                           /* steps that entered the sums */
   int64_t resums;                            /* in-kernel FP64 S(k) re-sums done */
   double resum_drift;                        /* largest |S_total| correction a re-sum made */
+  int surface[FMC_MAX_SPECIES];              /* surfacemols by guest type (mc_moves.f:526,568; gcmc.f:1907-1912) */
+  int64_t prodcount;                         /* production steps counted (gcmc.f:1052) */
+  int64_t windows;                           /* averaging windows closed (avcount, gcmc.f:1095) */
 } fmc_walker_stats;
 
 typedef struct fmc_handle fmc_handle;
@@ -189,7 +195,13 @@ int fmc_reset_stats(fmc_handle *h);
  * trial energy, Metropolis test (energy_eval, utility_pack.f:3637-3693) and
  * accept/reject, with the duni generator (utility_pack.f:2768-2857) per
  * walker.  If trace is non-NULL, walker 0 writes one record per step
- * (trace[nsteps]).  count_stats != 0 adds each step to the N/E moments. */
+ * (trace[nsteps]).  count_stats: bit 0 (FMC_STATS_COUNT) makes every step a
+ * production step (gcmc.f:1050-1130: the N/E moments and the averaging
+ * windows set by fmc_set_sampling); bit 1 (FMC_STATS_CLOSE) closes the open
+ * window after the last step, as the reference does on its final step
+ * (lgchk false, gcmc.f:1090-1092). */
+#define FMC_STATS_COUNT 1
+#define FMC_STATS_CLOSE 2
 int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats);
 
 /* Same, captured once into a HIP graph of `sweeps_per_graph` launches and
@@ -307,6 +319,51 @@ typedef struct fmc_widom_result {
 
 int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
               double *estep_out, int *overlap_out, fmc_widom_result *res);
+
+/* ---- production statistics (gcmc.f:1050-1130, 1148-1258, 1458-1557) ---- */
+
+/* Averaging windows of the production statistics: a window closes when the
+ * production count reaches a multiple of nwindsteps (initscan's
+ * nwindsteps = ceiling(steps / nwind), utility_pack.f:453-470); with
+ * accepted_only only accepted steps are counted ('sampling' directive,
+ * readinputs.f:608-613).  Each walker keeps up to max_windows closed windows
+ * (the reference allocates nwind*20, utility_pack.f:1091).  Resets the
+ * production counters and the windows of every walker. */
+int fmc_set_sampling(fmc_handle *h, int nwindsteps, int accepted_only, int max_windows);
+
+/* One closed window as the kernel records it: the number of steps it counted
+ * (rollcount) and, per guest type, the sums of N, E, E*N, N*N, E*E and the
+ * surface count over those steps (chainstats, gcmc.f:1062-1075). */
+#define FMC_WIN_STATS 6
+typedef struct fmc_window {
+  double count;
+  double sums[FMC_MAX_SPECIES][FMC_WIN_STATS];
+} fmc_window;
+
+/* Copy walker w's closed windows (at most cap) into out; *nwin = count. */
+int fmc_get_windows(fmc_handle *h, int w, fmc_window *out, int cap, int *nwin);
+
+/* The reference's final statistics of one guest type on one node: window
+ * averages with Q_st and C_v per window (calc_Qst / calc_Cv,
+ * utility_pack.f:2916-2956, Rgas = 8.314472/4184), their count-weighted
+ * means and standard deviations (avg, std: the node report, gcmc.f:1156-1236,
+ * 1276-1311) and the final report's union over the one node (avg_final,
+ * std_final: avunion / stdunion, gcmc.f:1591-1664, 1512-1535).  Index order:
+ * <N>, <E>, <E*N>, <N*N>, <E*E>, <surface N>, Q_st, C_v. */
+typedef struct fmc_guest_stats {
+  double avg[8];
+  double std[8];
+  double avg_final[8];
+  double std_final[8];
+} fmc_guest_stats;
+
+/* Host arithmetic only (no device access): the statistics above from nwin
+ * windows of ntpguest guest types at temperature temp (K). */
+int fmc_stats_from_windows(const fmc_window *win, int nwin, int ntpguest, double temp, fmc_guest_stats *out);
+
+/* fmc_get_windows + fmc_stats_from_windows at walker w's temperature;
+ * out[ntpguest]. */
+int fmc_production_stats(fmc_handle *h, int w, fmc_guest_stats *out);
 
 #ifdef __cplusplus
 }

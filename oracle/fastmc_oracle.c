@@ -91,7 +91,7 @@ struct orc_state {
   /* system */
   int imcon, maxmls, ntpguest, ntpfram, ntpatm, ntpvdw, maxvdw, mxcmls;
   double cell[10], rcell[10], volm, statvolm;
-  double rcut, rvdw, ddelr, alpha, epsq, engunit, overlap, temp, beta, sumchg;
+  double rcut, rvdw, ddelr, alpha, epsq, engunit, overlap, temp, beta, sumchg, surftol;
   int kmax1, kmax2, kmax3, newld, mxegrd;
   double drewd, dlrpot;
   double *erc;                 /* [mxegrd+1] */
@@ -139,6 +139,16 @@ struct orc_state {
   double *xdf, *ydf, *zdf, *rsqdf;
   int *ilist, *moldf;
   double *elc0, *els0, *emc, *ems, *enc, *ens, *elc1, *els1, *clm, *slm;
+  /* surface flags of the last trial (lnewsurf / lsurf, linitsurf) and
+   * surfacemols by species */
+  int lnewsurf, linitsurf;
+  int surfacemols[FMC_MAX_SPECIES + 1];
+  /* production statistics (gcmc.f:1050-1130): windows of nwindsteps counted
+   * steps; win holds closed windows as fmc_window records */
+  int st_nwind, st_accsample, nwin, capwin;
+  long long prodcount, rollcount;
+  double chain[FMC_MAX_SPECIES + 1][FMC_WIN_STATS];
+  fmc_window *win;
   /* pending trial (for accept/reject) */
   int pend_kind, pend_iguest, pend_choice, pend_valid;
   double pend_delrc, pend_engsictmp, pend_chgtmp;
@@ -695,10 +705,12 @@ static void zero_components(orc_state *s) {
 }
 
 /* per-site pair loop shared by insertion / guest_energy / deletion:
- * gathers the neighbour list, applies images and the overlap test
- * (surface_check, mc_moves.f:10-44), then ewald2 + srfrce. */
+ * gathers the neighbour list, applies images, then surface_check
+ * (mc_moves.f:10-44: *surf if a frozen neighbour lies closer than
+ * surftol + sigma of the pair, prmvdw(ivdw,2); the overlap test) and
+ * ewald2 + srfrce. */
 static int site_pairs(orc_state *s, int i, int mol, double chg, int sittyp, int stop_on_overlap,
-                      double *e2, double *ev, int *overlap) {
+                      double *e2, double *ev, int *overlap, int *surf) {
   int ik = 0;
   for (int j = 1; j <= s->gstlentry[i]; ++j) {
     ++ik;
@@ -712,6 +724,17 @@ static int site_pairs(orc_state *s, int i, int mol, double chg, int sittyp, int 
   images(s, ik, s->xdf, s->ydf, s->zdf);
   for (int l = 1; l <= s->gstlentry[i]; ++l) {
     s->rsqdf[l] = s->xdf[l] * s->xdf[l] + s->ydf[l] * s->ydf[l] + s->zdf[l] * s->zdf[l];
+    {
+      int jatm = s->ilist[l];
+      double aa = (double)sittyp, ab = (double)s->ltype[jatm], ak;
+      if (aa > ab)
+        ak = aa * (aa - 1.0) * 0.5 + ab + 0.5;
+      else
+        ak = ab * (ab - 1.0) * 0.5 + aa + 0.5;
+      double req = s->prm_sig[s->lstvdw[(int)ak]];
+      double tol = (s->surftol + req) * (s->surftol + req);
+      if (s->rsqdf[l] < tol && s->lfreezesite[jatm] == 1) *surf = 1;
+    }
     if (s->rsqdf[l] < s->overlap) *overlap = 1;
     if (*overlap && stop_on_overlap) return 1;
   }
@@ -741,7 +764,7 @@ static void get_guest(orc_state *s, int iguest, int choice) {
 /* insertion, mc_moves.f:150-276 */
 static int insertion(orc_state *s, int iguest, fmc_trial *out) {
   int mol = s->locguest[iguest], natms = s->numatoms[mol];
-  int loverlap = 0;
+  int loverlap = 0, lnewsurf = 0;
   double ewld2sum = 0.0, vdwsum = 0.0, estep = 0.0;
   for (int i = 0; i <= FMC_MAX_SITES; ++i) s->ind[i] = 0;
   zero_components(s);
@@ -749,9 +772,11 @@ static int insertion(orc_state *s, int iguest, fmc_trial *out) {
   double delrc = gstlrcorrect(s, iguest, 0);
   for (int i = 1; i <= natms; ++i) {
     double e2 = 0.0, ev = 0.0;
-    if (site_pairs(s, i, mol, s->atmchg[mol][i], s->ltpsit[mol][i], 1, &e2, &ev, &loverlap)) {
+    if (site_pairs(s, i, mol, s->atmchg[mol][i], s->ltpsit[mol][i], 1, &e2, &ev, &loverlap, &lnewsurf)) {
       memset(out, 0, sizeof *out);
       out->overlap = 1;
+      out->surface = lnewsurf;
+      s->lnewsurf = lnewsurf;
       s->pend_delrc = delrc;
       return 1;
     }
@@ -769,6 +794,8 @@ static int insertion(orc_state *s, int iguest, fmc_trial *out) {
   s->delE[mol] += estep;
   memset(out, 0, sizeof *out);
   out->estep = estep;
+  out->surface = lnewsurf;
+  s->lnewsurf = lnewsurf;
   out->recip = (ewld1eng - ewld3sum) / s->engunit;
   out->real = ewld2sum / s->engunit;
   out->vdw = vdwsum / s->engunit;
@@ -781,10 +808,11 @@ static int insertion(orc_state *s, int iguest, fmc_trial *out) {
 }
 
 /* guest_energy, mc_moves.f:277-374: returns estep, sets *ov on overlap */
-static double guest_energy(orc_state *s, int iguest, int lexisting, int *ov, double comps[3]) {
+static double guest_energy(orc_state *s, int iguest, int lexisting, int *ov, double comps[3], int *lsurf) {
   int mol = s->locguest[iguest], natms = s->numatoms[mol];
   double ewld2sum = 0.0, vdwsum = 0.0;
   *ov = 0;
+  *lsurf = 0;
   zero_components(s);
   guestlistgen(s, natms);
   double chgtmp, engsictmp;
@@ -792,7 +820,7 @@ static double guest_energy(orc_state *s, int iguest, int lexisting, int *ov, dou
   for (int i = 1; i <= natms; ++i) {
     int itatm = s->ind[i];
     double e2 = 0.0, ev = 0.0;
-    if (site_pairs(s, i, mol, s->atmcharge[itatm], s->ltype[itatm], 1, &e2, &ev, ov)) return 0.0;
+    if (site_pairs(s, i, mol, s->atmcharge[itatm], s->ltype[itatm], 1, &e2, &ev, ov, lsurf)) return 0.0;
     ewld2sum += e2;
     vdwsum += ev;
   }
@@ -809,7 +837,7 @@ static int deletion(orc_state *s, int iguest, int choice, fmc_trial *out) {
   double ewld2sum = 0.0, vdwsum = 0.0;
   get_guest(s, iguest, choice);
   int mol = s->locguest[iguest], natms = s->numatoms[mol];
-  int loverlap = 0;
+  int loverlap = 0, linitsurf = 0;
   zero_components(s);
   double delrc = gstlrcorrect(s, iguest, 1);
   guestlistgen(s, natms);
@@ -818,7 +846,7 @@ static int deletion(orc_state *s, int iguest, int choice, fmc_trial *out) {
   for (int i = 1; i <= natms; ++i) {
     int itatm = s->ind[i];
     double e2 = 0.0, ev = 0.0;
-    site_pairs(s, i, mol, s->atmcharge[itatm], s->ltype[itatm], 0, &e2, &ev, &loverlap);
+    site_pairs(s, i, mol, s->atmcharge[itatm], s->ltype[itatm], 0, &e2, &ev, &loverlap, &linitsurf);
     ewld2sum += e2;
     vdwsum += ev;
   }
@@ -833,6 +861,8 @@ static int deletion(orc_state *s, int iguest, int choice, fmc_trial *out) {
   memset(out, 0, sizeof *out);
   out->estep = estep;
   out->overlap = loverlap;
+  out->surface = linitsurf << 1;
+  s->linitsurf = linitsurf;
   out->recip = (-ewld1eng - ewld3sum) / s->engunit;
   out->real = ewld2sum / s->engunit;
   out->vdw = vdwsum / s->engunit;
@@ -938,9 +968,9 @@ static int displace_eval(orc_state *s, int iguest, int choice, fmc_trial *out, i
   int mol = s->locguest[iguest], natms = s->numatoms[mol];
   int maxmls = s->maxmls;
   get_guest(s, iguest, choice);
-  int ov = 0;
+  int ov = 0, linitsurf = 0, lnewsurf = 0;
   double c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
-  double enginit = guest_energy(s, iguest, 1, &ov, c0);
+  double enginit = guest_energy(s, iguest, 1, &ov, c0, &linitsurf);
   for (int ik = 1; ik <= s->mxcmls; ++ik) {
     s->ewald1entmp[ik] = s->ewald1en[ik];
     s->ewald2entmp[ik] = s->ewald2en[ik];
@@ -968,8 +998,10 @@ static int displace_eval(orc_state *s, int iguest, int choice, fmc_trial *out, i
     rotation(s->newx, s->newy, s->newz, comx, comy, comz, natms, q[0], q[1], q[2], q[3]);
   }
   int ov2 = 0;
-  double estep = guest_energy(s, iguest, 0, &ov2, c1);
+  double estep = guest_energy(s, iguest, 0, &ov2, c1, &lnewsurf);
   estep = estep - enginit;
+  s->linitsurf = linitsurf;
+  s->lnewsurf = lnewsurf;
   for (int i = 1; i <= maxmls; ++i) {
     int ik = loc2(mol, i);
     if (i != mol)
@@ -980,6 +1012,7 @@ static int displace_eval(orc_state *s, int iguest, int choice, fmc_trial *out, i
   memset(out, 0, sizeof *out);
   out->estep = estep;
   out->overlap = ov2;
+  out->surface = lnewsurf | (linitsurf << 1);
   out->recip = (c1[0] - c0[0]) / s->engunit;
   out->real = (c1[1] - c0[1]) / s->engunit;
   out->vdw = (c1[2] - c0[2]) / s->engunit;
@@ -1460,6 +1493,7 @@ orc_state *orc_create(const fmc_system *sys, const fmc_walker *wk) {
   s->epsq = sys->epsq;
   s->engunit = sys->engunit;
   s->overlap = sys->overlap * sys->overlap;
+  s->surftol = sys->surftol;
   s->sumchg = sys->sumchg;
   s->kmax1 = sys->kmax[0];
   s->kmax2 = sys->kmax[1];
@@ -1611,6 +1645,7 @@ orc_state *orc_create(const fmc_system *sys, const fmc_walker *wk) {
 
 void orc_destroy(orc_state *s) {
   if (!s) return;
+  free(s->win);
   free(s->erc); free(s->vvv); free(s->lstvdw); free(s->ltpvdw); free(s->prm_eps); free(s->prm_sig);
   free(s->steadd); free(s->kv);
   for (int m = 0; m <= FMC_MAX_SPECIES; ++m) {
@@ -1636,6 +1671,26 @@ int orc_info(const orc_state *s, int *newld, int *mxegrd, int *totatm, int *maxm
 }
 
 /* ----------------------------------------------------- public trials -- */
+/* surfacemols after an accepted move of species mol: accept_move counts an
+ * inserted molecule at the surface up and a deleted one down
+ * (mc_moves.f:526,568); a displacement or jump (chain = 1) moves the count
+ * by the change of the molecule's own flag, never below zero
+ * (gcmc.f:1906-1912,2021-2027). */
+static void surf_tally(orc_state *s, int kind, int mol, int chain) {
+  if (kind == FMC_MOVE_INSERT) {
+    if (s->lnewsurf) s->surfacemols[mol] += 1;
+  } else if (kind == FMC_MOVE_DELETE) {
+    if (s->linitsurf) s->surfacemols[mol] -= 1;
+  } else if (chain) {
+    if (s->linitsurf && !s->lnewsurf) {
+      s->surfacemols[mol] -= 1;
+      if (s->surfacemols[mol] < 0) s->surfacemols[mol] = 0;
+    } else if (!s->linitsurf && s->lnewsurf) {
+      s->surfacemols[mol] += 1;
+    }
+  }
+}
+
 int orc_trial_insert(orc_state *s, int iguest, const double *xyz, fmc_trial *out) {
   if (iguest < 1 || iguest > s->ntpguest) return FMC_ERR_ARG;
   int natms = s->numatoms[s->locguest[iguest]];
@@ -1678,6 +1733,7 @@ int orc_trial_displace(orc_state *s, int iguest, int choice, const double *xyz_n
 
 int orc_accept(orc_state *s) {
   if (!s->pend_kind || !s->pend_valid) return FMC_ERR_STATE;
+  surf_tally(s, s->pend_kind, s->locguest[s->pend_iguest], 0);
   accept_move(s, s->pend_kind, s->pend_iguest, s->pend_choice);
   s->pend_kind = 0;
   return 0;
@@ -1749,8 +1805,11 @@ static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, d
       SK(ckssorig, rows[r], ik) = SK(ckssum, rows[r], ik);
     }
   /* delete molecule ichoice of iguest */
+  int surf_orig[FMC_MAX_SPECIES + 1];
+  for (int i = 0; i <= FMC_MAX_SPECIES; ++i) surf_orig[i] = s->surfacemols[i];
   fmc_trial td;
   deletion(s, iguest, ichoice, &td);
+  surf_tally(s, FMC_MOVE_DELETE, imol, 1);
   accept_move(s, FMC_MOVE_DELETE, iguest, ichoice);
   const double estepi = -td.estep;
   /* insert a jguest molecule at its COM */
@@ -1770,6 +1829,7 @@ static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, d
   }
   fmc_trial ti;
   const int ov = insertion(s, jguest, &ti);
+  surf_tally(s, FMC_MOVE_INSERT, jmol, 1);
   accept_move(s, FMC_MOVE_INSERT, jguest, 0);
   const double estepj = ov ? 0.0 : ti.estep;
   int accepted = 0, drew = 0;
@@ -1834,6 +1894,8 @@ static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, d
       }
     s->chgsum_mol[imol] = chgsum_orig[imol];
     s->chgsum_mol[jmol] = chgsum_orig[jmol];
+    s->surfacemols[imol] = surf_orig[imol];
+    s->surfacemols[jmol] = surf_orig[jmol];
     s->elrc = elrc_orig;
     condense(s);
     for (int i = 0; i < FMC_MAX_SPECIES + 2; ++i) s->energy[i] = energy_orig[i];
@@ -1842,6 +1904,7 @@ static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, d
   memset(tr, 0, sizeof *tr);
   tr->estep = estepi + estepj;
   tr->overlap = ov;
+  tr->surface = s->lnewsurf | (s->linitsurf << 1);
   *choice_out = ichoice;
   *rande_out = rande;
   *drew_out = drew;
@@ -1857,7 +1920,76 @@ int orc_move_counts(const orc_state *s, long long *att, long long *acc) {
   return 0;
 }
 
-int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
+/* The production block of the main loop (gcmc.f:1050-1130) after a step:
+ * count it (every step, or only accepted ones with the 'sampling'
+ * directive), add N, E, E*N, N*N, E*E and the surface count of every guest
+ * type to the open window, and close the window when the production count
+ * reaches a multiple of nwindsteps or on the run's final step (lgchk false). */
+static void stats_step(orc_state *s, int accepted, int final_step) {
+  if (!s->st_accsample || accepted) {
+    s->prodcount += 1;
+    s->rollcount += 1;
+    for (int g = 1; g <= s->ntpguest; ++g) {
+      int mol = s->locguest[g];
+      double dm = (double)(float)s->nummols[mol];
+      double m2 = dm * dm, e = s->energy[mol], e2 = e * e, sf = (double)(float)s->surfacemols[mol];
+      double *c = s->chain[g];
+      c[0] += dm;
+      c[1] += e;
+      c[2] += e * dm;
+      c[3] += m2;
+      c[4] += e2;
+      c[5] += sf;
+    }
+  }
+  if ((s->st_nwind > 0 && s->prodcount % s->st_nwind == 0) || final_step) {
+    if (s->nwin == s->capwin) {
+      s->capwin = s->capwin * 2 + 8;
+      s->win = (fmc_window *)realloc(s->win, sizeof(fmc_window) * (size_t)s->capwin);
+    }
+    fmc_window *w = &s->win[s->nwin++];
+    memset(w, 0, sizeof *w);
+    w->count = (double)s->rollcount;
+    for (int g = 1; g <= s->ntpguest && g <= FMC_MAX_SPECIES; ++g)
+      for (int q = 0; q < FMC_WIN_STATS; ++q) {
+        w->sums[g - 1][q] = s->chain[g][q];
+        s->chain[g][q] = 0.0;
+      }
+    s->rollcount = 0;
+  }
+}
+
+int orc_set_sampling(orc_state *s, int nwindsteps, int accepted_only) {
+  if (nwindsteps < 0) return FMC_ERR_ARG;
+  s->st_nwind = nwindsteps;
+  s->st_accsample = accepted_only != 0;
+  s->prodcount = s->rollcount = 0;
+  s->nwin = 0;
+  memset(s->chain, 0, sizeof s->chain);
+  return 0;
+}
+
+int orc_get_windows(const orc_state *s, fmc_window *out, int cap, int *nwin, long long *prodcount) {
+  for (int i = 0; i < s->nwin && i < cap; ++i) out[i] = s->win[i];
+  if (nwin) *nwin = s->nwin;
+  if (prodcount) *prodcount = s->prodcount;
+  return 0;
+}
+
+int orc_get_surface(const orc_state *s, int *surf) {
+  for (int g = 1; g <= s->ntpguest; ++g) surf[g - 1] = s->surfacemols[s->locguest[g]];
+  return 0;
+}
+
+static int run_chain(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags);
+
+int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) { return run_chain(s, nsteps, trace, 0); }
+
+int orc_run_stats(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags) {
+  return run_chain(s, nsteps, trace, flags);
+}
+
+static int run_chain(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags) {
   for (int step = 1; step <= nsteps; ++step) {
     int iguest = 1;
     if (s->ntpguest > 1) iguest = (int)floor(mzt_next(&s->rng) * s->ntpguest) + 1;
@@ -1943,6 +2075,7 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       }
       if (accepted) {
         s->mv_acc[FMC_MOVE_INSERT] += 1;
+        surf_tally(s, FMC_MOVE_INSERT, mol, 1);
         accept_move(s, FMC_MOVE_INSERT, iguest, 0);
       } else {
         for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
@@ -1963,6 +2096,7 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       s->mv_att[FMC_MOVE_DELETE] += 1;
       if (accepted) {
         s->mv_acc[FMC_MOVE_DELETE] += 1;
+        surf_tally(s, FMC_MOVE_DELETE, mol, 1);
         accept_move(s, FMC_MOVE_DELETE, iguest, choice);
       } else {
         for (int i = 0; i < 64; ++i) s->engsic[i] = s->engsicorig[i];
@@ -2002,6 +2136,7 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       if (accepted) {
         s->mv_acc[FMC_MOVE_JUMP] += 1;
         accept_move(s, FMC_MOVE_DISPLACE, iguest, choice);
+        surf_tally(s, FMC_MOVE_DISPLACE, mol, 1);
       } else {
         restore_sk(s, mol);
         for (int i = 0; i < 64; ++i) s->delE[i] = 0.0;
@@ -2059,6 +2194,7 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       if (accepted) {
         s->mv_acc[kind] += 1;
         accept_move(s, FMC_MOVE_DISPLACE, iguest, choice);
+        surf_tally(s, FMC_MOVE_DISPLACE, mol, 1);
         if (kind == FMC_MOVE_TRANSLATE) s->accept_tran[iguest] += 1;
         else if (kind == FMC_MOVE_ROTATE) s->accept_rota[iguest] += 1;
         else s->accept_disp[iguest] += 1;
@@ -2088,7 +2224,10 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace) {
       r->delrdisp = s->delrdisp[iguest];
       r->rotangle = s->rotangle;
       r->rota_rotangle = s->rota_rotangle;
+      r->surface = tr.surface;
+      r->nsurface = s->surfacemols[mol];
     }
+    if (flags & FMC_STATS_COUNT) stats_step(s, accepted, (flags & FMC_STATS_CLOSE) && step == nsteps);
   }
   return 0;
 }

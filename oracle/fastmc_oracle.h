@@ -24,6 +24,13 @@ int orc_accept(orc_state *s);
 int orc_reject(orc_state *s);
 
 int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
+/* orc_run with the production block (gcmc.f:1050-1130); flags as fmc_run's
+ * count_stats (FMC_STATS_COUNT, FMC_STATS_CLOSE). */
+int orc_run_stats(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags);
+int orc_set_sampling(orc_state *s, int nwindsteps, int accepted_only);
+int orc_get_windows(const orc_state *s, fmc_window *out, int cap, int *nwin, long long *prodcount);
+/* surfacemols by guest type */
+int orc_get_surface(const orc_state *s, int *surf);
 /* attempts / acceptances by move kind (index FMC_MOVE_*), summed over guests */
 int orc_move_counts(const orc_state *s, long long *att, long long *acc);
 

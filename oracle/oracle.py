@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, TRACE_DTYPE, make_walker)
+from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, FmcWindow, TRACE_DTYPE, make_walker)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
@@ -35,7 +35,8 @@ def lib():
         for name in ("orc_info", "orc_single_point", "orc_trial_insert", "orc_trial_delete",
                      "orc_trial_displace", "orc_accept", "orc_reject", "orc_run", "orc_get_guests",
                      "orc_get_sk", "orc_get_table", "orc_get_kvec", "orc_get_energy",
-                     "orc_full_resum", "orc_wl_setup", "orc_wl_run", "orc_wl_get"):
+                     "orc_full_resum", "orc_wl_setup", "orc_wl_run", "orc_wl_get", "orc_run_stats",
+                     "orc_set_sampling", "orc_get_windows", "orc_get_surface"):
             getattr(L, name).restype = C.c_int
         L.orc_duni_seq.restype = None
         L.orc_wl_setup.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -111,12 +112,33 @@ class Oracle:
         lib().orc_single_point(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
         return out
 
-    def run(self, nsteps):
+    def run(self, nsteps, stats=0):
+        """nsteps steps of the chain; stats as fmc_run's count_stats flags
+        (1: production steps, 2: close the open window after the last)."""
         tr = np.zeros(nsteps, dtype=TRACE_DTYPE)
-        rc = lib().orc_run(C.c_void_p(self.h), C.c_int(nsteps), tr.ctypes.data_as(C.c_void_p))
+        rc = lib().orc_run_stats(C.c_void_p(self.h), C.c_int(nsteps), tr.ctypes.data_as(C.c_void_p),
+                                 C.c_int(stats))
         if rc:
             raise RuntimeError(f"orc_run rc={rc}")
         return tr
+
+    def set_sampling(self, nwindsteps, accepted_only=False):
+        rc = lib().orc_set_sampling(C.c_void_p(self.h), C.c_int(nwindsteps), C.c_int(int(accepted_only)))
+        if rc:
+            raise RuntimeError(f"orc_set_sampling rc={rc}")
+
+    def windows(self):
+        """(closed windows as an FmcWindow array, production count)."""
+        n, pc = C.c_int(), C.c_longlong()
+        lib().orc_get_windows(C.c_void_p(self.h), None, C.c_int(0), C.byref(n), C.byref(pc))
+        buf = (FmcWindow * max(n.value, 1))()
+        lib().orc_get_windows(C.c_void_p(self.h), buf, C.c_int(n.value), C.byref(n), C.byref(pc))
+        return buf, n.value, pc.value
+
+    def surface(self):
+        out = np.zeros(8, dtype=np.int32)
+        lib().orc_get_surface(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))
+        return out
 
     def _trial(self, fn, *args):
         t = FmcTrial()

@@ -1,0 +1,136 @@
+"""Production-statistics known answers from the compiled reference (run in
+the build container only; the reference never travels):
+
+    make -C oracle ref                   # builds oracle/_ref/gcmc_ser.x from the reference sources
+    python tests/golden/make_stats_ref.py
+
+Each case writes its inputs to a scratch directory and runs the reference's
+serial program there.  From OUTPUT it records the duni seeds, the step at
+which production started ("Starting production at", gcmc.f:755-757; step 1
+when equilibration is 0), the total step count ("Time elapsed for",
+gcmc.f:1461), the window length ("Each window will average over",
+utility_pack.f:472-475), and per guest the node report (gcmc.f:1276-1311:
+<N>, <E>, <E*N>, <N*N>, <E*E>, Multiplier, Q_st and its error, C_v and its
+error, <surface adsorbed N>) and the final report (gcmc.f:1517-1556: the
+five moments with +/- errors, average number of guests and its standard
+error, Q_st and error, surface adsorption and error, C_v and error, total
+steps counted), as printed (f20.9 / f15.6 / f12.3).
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, HERE)
+
+from fastmc_amd import inputs as I  # noqa: E402
+
+EXE = os.path.join(REPO, "oracle", "_ref", "gcmc_ser.x")
+NUM = r"([-+0-9.EeNaInf]+)"
+
+
+def set_steps(ctl, eq, steps, extra=""):
+    ctl = re.sub(r"(?m)^steps .*$", f"steps    {steps}", ctl)
+    ctl = re.sub(r"(?m)^equilibration .*$", f"equilibration    {eq}", ctl)
+    if extra:
+        ctl = ctl.replace("history 0\n", "history 0\n" + extra + "\n")
+    return ctl
+
+
+def write_cfg0(d, eq, steps, extra=""):
+    I.config0(d, pressure_bar=1.0)
+    p = os.path.join(d, "CONTROL")
+    ctl = set_steps(open(p).read(), eq, steps, extra)
+    open(p, "w").write(ctl)
+
+
+def write_calf20_moves6(d, eq, steps, extra=""):
+    import make_moves_ref
+    for f in ("FIELD", "CONFIG"):
+        shutil.copy(os.path.join(HERE, "calf20", f), os.path.join(d, f))
+    open(os.path.join(d, "CONTROL"), "w").write(set_steps(make_moves_ref.control(steps), eq, steps, extra))
+
+
+CASES = [
+    # name, writer, equilibration, steps, extra CONTROL lines
+    ("calf20_moves6_stats", write_calf20_moves6, 1000, 4000, ""),
+    ("cfg0_p1_stats", write_cfg0, 600, 3000, ""),
+    ("cfg0_p1_sampling", write_cfg0, 400, 400, "sampling"),
+]
+
+NODE_KEYS = [("<N>", "<N>: "), ("<E>", "<E>: "), ("<E*N>", r"<E\*N>: "), ("<N*N>", r"<N\*N>: "),
+             ("<E*E>", r"<E\*E>: "), ("Multiplier", "Multiplier: "),
+             ("Qst", r"Isosteric heat of adsorption \(kcal/mol\): "), ("Qst_err", "Isosteric heat error: "),
+             ("Cv", r"Heat capacity, Cv \(kcal/mol/K\): "), ("Cv_err", "Heat capacity error: "),
+             ("<NF>", "<surface adsorbed N>:")]
+FINAL_MOMENTS = [("<N>", "<N>: "), ("<E>", "<E>: "), ("<E*N>", r"<E\*N>: "), ("<N*N>", r"<N\*N>: "),
+                 ("<E*E>", r"<E\*E>: ")]
+FINAL_KEYS = [("avg_guests", "average number of guests: "), ("avg_guests_err", "standard error: "),
+              ("Qst", r"Isosteric heat of adsorption \(kcal/mol\): "), ("Qst_err", "Isosteric heat error: "),
+              ("surface", "average surface adsorption: "), ("surface_err", "standard error: "),
+              ("Cv", r"Heat capacity, Cv \(kcal/mol/K\): "), ("Cv_err", "Heat capacity error: "),
+              ("counted", "Total steps counted: ")]
+
+
+def parse_block(text, keys):
+    """Values of `keys` in order, each searched after the previous one."""
+    out, pos = {}, 0
+    for name, pat in keys:
+        m = re.compile(r"\s*" + pat + r"\s*" + NUM).search(text, pos)
+        out[name] = m.group(1)
+        pos = m.end()
+    return out
+
+
+def run_case(name, writer, eq, steps, extra):
+    d = tempfile.mkdtemp(prefix=f"fmc_stats_{name}_")
+    try:
+        writer(d, eq, steps, extra)
+        control = open(os.path.join(d, "CONTROL")).read()
+        subprocess.run([EXE], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=3600, check=False)
+        out = open(os.path.join(d, "OUTPUT")).read()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    seeds = [int(x) for x in re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]
+    total = int(re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out).group(1))
+    m = re.search(r"Starting production at\s+(\d+)", out)
+    start = int(m.group(1)) if m else 1
+    nwindsteps = int(re.search(r"Each window will average over\s+(\d+)", out).group(1))
+    node = out[out.index("Data reported from node"):]
+    node_guests = []
+    for part in re.split(r"\n\s+guest\s+\d+:", node)[1:]:
+        node_guests.append(parse_block(part, NODE_KEYS))
+    final = out[out.index("final stats"):]
+    final_guests = []
+    for part in final.split("final stats for guest")[1:]:
+        g = {}
+        pos = 0
+        for nm, pat in FINAL_MOMENTS:
+            mm = re.compile(r"\s*" + pat + r"\s*" + NUM + r"\s*\+/-\s*" + NUM).search(part, pos)
+            g[nm], g[nm + "_err"] = mm.group(1), mm.group(2)
+            pos = mm.end()
+        g.update(parse_block(part[pos:], FINAL_KEYS))
+        final_guests.append(g)
+    return {"name": name, "control": control, "seeds": seeds, "start": start, "total": total,
+            "nwindsteps": nwindsteps, "node": node_guests[: len(final_guests)], "final": final_guests}
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit(f"{EXE} missing: run `make -C oracle ref` first")
+    res = [run_case(*c) for c in CASES]
+    for r in res:
+        print({k: v for k, v in r.items() if k != "control"}, flush=True)
+    json.dump(res, open(os.path.join(HERE, "stats_ref.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()
