@@ -860,6 +860,8 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   else vvv = (const T *)P.vvv32;
   // NSITE > 0: trial sites in registers; padding sites (s >= natms) carry
   // charge 0, no LJ column and no overlap test.  NSITE == 0: read from LDS.
+  // The site values are the same on every lane: unid keeps them in scalar
+  // registers, leaving the vector registers to the pair terms and k-slots.
   constexpr int NSA = NSITE > 0 ? NSITE : 1;
   double xs[2][NSA][3], cq[NSA], ov2[NSA], qs[NSA];
   if constexpr (NSITE > 0) {
@@ -868,14 +870,14 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     for (int s = 0; s < NSITE; ++s) {
       const bool real = s < natms;
       const double c = sh.scq[mol][s], q = sh.sq[mol][s];
-      cq[s] = real ? c : 0.0;
-      ov2[s] = real ? ovl2 : -1.0;
-      qs[s] = real ? q : 0.0;
+      cq[s] = unid(real ? c : 0.0);
+      ov2[s] = unid(real ? ovl2 : -1.0);
+      qs[s] = unid(real ? q : 0.0);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const double a = sh.xn[s][d], b = sh.xo[s][d];
-        xs[0][s][d] = real ? a : 0.0;
-        xs[1][s][d] = real ? b : 0.0;
+        xs[0][s][d] = unid(real ? a : 0.0);
+        xs[1][s][d] = unid(real ? b : 0.0);
       }
     }
   }
@@ -1821,7 +1823,7 @@ __device__ inline void surf_displaced(Smem<NS> &sh) {
   }
 }
 
-// The production block after a step (gcmc.f:1050-1130), on wave 0: count the
+// The production block after a step (gcmc.f:1050-1130), on one wave: count the
 // step (only if accepted with the 'sampling' directive), add N, E, E*N, N*N,
 // E*E and the surface count of every guest kind to the open window (lane
 // 6(g-1)+q holds sum q of guest g), and close the window into W.win when the
@@ -1830,7 +1832,7 @@ __device__ inline void surf_displaced(Smem<NS> &sh) {
 template <int NS>
 __device__ inline void stats_step(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, bool close) {
   WState &ws = sh.ws;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int ng = P.ntpguest;
   const bool take = !P.accsample || sh.accepted;
   const long long pc = ws.prodcount + (take ? 1 : 0);
@@ -1943,9 +1945,12 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     }
     if (phase == 1) continue;
     if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
-    if (threadIdx.x < 64 && (count_stats & FMC_STATS_COUNT))
+    // the step's statistics on wave 1, idle while wave 0 proposes the next
+    // step (the fields written here are read nowhere before the next barrier)
+    const int stat_wave = blockDim.x > 64 ? 1 : 0;
+    if ((int)(threadIdx.x >> 6) == stat_wave && (count_stats & FMC_STATS_COUNT))
       stats_step<NS>(P, W, w, sh, (count_stats & FMC_STATS_CLOSE) && step == nsteps - 1);
-    if (threadIdx.x == 0 && count_stats) {
+    if ((int)threadIdx.x == 64 * stat_wave && count_stats) {
       WState &ws = sh.ws;
       double e = 0.0;
       for (int gg = 1; gg <= P.ntpguest; ++gg) {
