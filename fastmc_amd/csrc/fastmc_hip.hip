@@ -542,6 +542,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   P.nwindsteps = 0;
   P.accsample = 0;
   P.maxwin = 0;
+  P.nplots = 0;
   // tail constants, lrcorrect eadd (vdw_terms.f:581-645)
   P.ntri = ntri;
   {
@@ -1007,6 +1008,7 @@ int fmc_destroy(fmc_handle *h) {
   for (void *p : h->allocs) hipFree(p);
   if (h->dtrace) hipFree(h->dtrace);
   if (h->dw.win) hipFree(h->dw.win);
+  if (h->dw.prob) hipFree(h->dw.prob);
   for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
                   (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
                   (void *)h->g_vt})
@@ -1913,6 +1915,58 @@ int fmc_get_windows(fmc_handle *h, int w, fmc_window *out, int cap, int *nwin) {
   if (m > 0)
     HIPCHK(hipMemcpy(out, h->dw.win + (size_t)w * h->hp.maxwin, sizeof(fmc_window) * (size_t)m,
                      hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+int fmc_set_prob(fmc_handle *h, double griddim, const int gridfactor[3], int nplots, const fmc_prob_plot *plots,
+                 int ngrid_out[3]) {
+  if (!h || nplots < 0 || nplots > FMC_MAX_PLOTS || (nplots > 0 && (!plots || !(griddim > 0.0)))) return FMC_ERR_ARG;
+  DevParams &P = h->hp;
+  const int gf[3] = {gridfactor ? gridfactor[0] : 1, gridfactor ? gridfactor[1] : 1, gridfactor ? gridfactor[2] : 1};
+  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1) return FMC_ERR_ARG;
+  for (int p = 0; p < nplots; ++p)
+    if (plots[p].iguest < 1 || plots[p].iguest > P.ntpguest || plots[p].nsites < 0 ||
+        plots[p].nsites > FMC_MAX_SITES + 1)
+      return FMC_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  if (h->dw.prob) HIPCHK(hipFree(h->dw.prob));
+  h->dw.prob = nullptr;
+  P.nplots = 0;
+  if (nplots > 0) {
+    // grid points per axis (gcmc.f:258-281) and the unit cell the positions
+    // fold into (gcmc.f:299-306)
+    double celprp[10], ucell[9];
+    dcell(P.cell, celprp);
+    for (int a = 0; a < 3; ++a) {
+      const double gd = celprp[a] / std::ceil(celprp[a] / gf[a] / griddim) / gf[a];
+      P.pgrid[a] = (int)std::ceil(celprp[a] / (gd * gf[a]));
+    }
+    for (int i = 0; i < 9; ++i) ucell[i] = P.cell[i] / (double)gf[i / 3];
+    invert(ucell, P.prucell);
+    P.pgridsize = P.pgrid[0] * P.pgrid[1] * P.pgrid[2];
+    for (int p = 0; p < nplots; ++p) {
+      P.plot_g[p] = plots[p].iguest;
+      P.plot_ns[p] = plots[p].nsites;
+      for (int e = 0; e < plots[p].nsites; ++e) P.plot_site[p][e] = plots[p].sites[e];
+    }
+    const size_t n = (size_t)h->nwalkers * nplots * P.pgridsize;
+    HIPCHK(hipMalloc(&h->dw.prob, sizeof(double) * n));
+    HIPCHK(hipMemset(h->dw.prob, 0, sizeof(double) * n));
+    P.nplots = nplots;
+    if (ngrid_out)
+      for (int a = 0; a < 3; ++a) ngrid_out[a] = P.pgrid[a];
+  }
+  HIPCHK(hipMemcpy(h->dp, &P, sizeof(DevParams), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int fmc_get_prob(fmc_handle *h, int w, int plot, double *out) {
+  if (!h || w < 0 || w >= h->nwalkers || plot < 0 || plot >= h->hp.nplots || !out) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  HIPCHK(hipMemcpy(out, h->dw.prob + ((size_t)w * P.nplots + plot) * P.pgridsize, sizeof(double) * P.pgridsize,
+                   hipMemcpyDeviceToHost));
   return FMC_OK;
 }
 

@@ -87,6 +87,12 @@ struct DevParams {
   // production statistics (gcmc.f:1050-1130): window length in counted steps
   // (0: windows close only at the end), accepted steps only, window capacity
   int nwindsteps, accsample, maxwin;
+  // probability plots (storeprob, utility_pack.f:622-687): grid, its size,
+  // reciprocal of the unit cell used for folding, and per plot the guest
+  // kind and site list (0: centre of mass)
+  int nplots, pgrid[3], pgridsize;
+  double prucell[9];
+  int plot_g[FMC_MAX_PLOTS], plot_ns[FMC_MAX_PLOTS], plot_site[FMC_MAX_PLOTS][FMC_MAX_SITES + 1];
   // tables
   const double *erc;      // [mxegrd+1]
   const double *vvv;      // [npt][mxegrd+1]
@@ -161,6 +167,7 @@ struct DevWalkers {
   double *wl_lng;            // [w][wl_nbins] ln g(N) (dos_hist)
   int *wl_hist;              // [w][wl_nbins] visits (visit_hist)
   fmc_window *win;           // [w][maxwin] closed averaging windows
+  double *prob;              // [w][nplots][pgridsize] probability plots
 };
 
 __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872

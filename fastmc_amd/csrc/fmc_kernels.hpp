@@ -1823,6 +1823,113 @@ __device__ inline void surf_displaced(Smem<NS> &sh) {
   }
 }
 
+// equitable_bin (utility_pack.f:2407-2519): spread weight 1 over the 8 grid
+// points around fractional position (fa, fb, fc) of a grid of na x nb x nc
+// points (c fastest; the reference's 1-based bin b is grid[b - 1]), each
+// axis split by the position's offset from the nearest point.  The b axis
+// keeps the reference's offset of 1 (not 0) for its upper neighbour past the
+// last plane.  Bins outside the grid (only reachable with nb = 1) are
+// dropped.  Hardware FP64 atomics: the order of the additions is free.
+__device__ inline void equitable_bin(const DevParams &P, double *grid, double fa, double fb, double fc) {
+  const int na = P.pgrid[0], nb = P.pgrid[1], nc = P.pgrid[2];
+  const double gpta = na * fa, gptb = nb * fb, gptc = nc * fc;
+  const int aidx = (int)ceil(gpta), bidx = (int)ceil(gptb), cidx = (int)ceil(gptc);
+  const double apart = gpta - floor(gpta), bpart = gptb - floor(gptb), cpart = gptc - floor(gptc);
+  int alidx, aridx, blidx, bridx, clidx, cridx;
+  double al, ar, bl, br, cl, cr;
+  if (apart > 0.5) {
+    alidx = aidx >= na ? (na - 1) * nb * nc : (aidx - 1) * nb * nc;
+    aridx = aidx >= na ? 0 : aidx * nb * nc;
+    al = 1.5 - apart;
+    ar = apart - 0.5;
+  } else {
+    alidx = aidx <= 1 ? (na - 1) * nb * nc : (aidx - 2) * nb * nc;
+    aridx = aidx <= 1 ? 0 : (aidx - 1) * nb * nc;
+    al = 0.5 - apart;
+    ar = 0.5 + apart;
+  }
+  if (bpart > 0.5) {
+    blidx = bidx >= nb ? (nb - 1) * nc : (bidx - 1) * nc;
+    bridx = bidx >= nb ? 1 : bidx * nc;
+    bl = 1.5 - bpart;
+    br = bpart - 0.5;
+  } else {
+    blidx = bidx <= 1 ? (nb - 1) * nc : (bidx - 2) * nc;
+    bridx = bidx <= 1 ? 0 : (bidx - 1) * nc;
+    bl = 0.5 - bpart;
+    br = 0.5 + bpart;
+  }
+  if (cpart > 0.5) {
+    clidx = cidx >= nc ? nc : cidx;
+    cridx = cidx >= nc ? 1 : cidx + 1;
+    cl = 1.5 - cpart;
+    cr = cpart - 0.5;
+  } else {
+    clidx = cidx <= 1 ? nc : cidx - 1;
+    cridx = cidx <= 1 ? 1 : cidx;
+    cl = 0.5 - cpart;
+    cr = 0.5 + cpart;
+  }
+  const int bins[8] = {alidx + blidx + clidx, aridx + blidx + clidx, alidx + bridx + clidx, alidx + blidx + cridx,
+                       aridx + bridx + clidx, aridx + blidx + cridx, alidx + bridx + cridx, aridx + bridx + cridx};
+  const double wts[8] = {al * bl * cl, ar * bl * cl, al * br * cl, al * bl * cr,
+                         ar * br * cl, ar * bl * cr, al * br * cr, ar * br * cr};
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (bins[i] >= 1 && bins[i] <= P.pgridsize) unsafeAtomicAdd(grid + bins[i] - 1, wts[i]);
+}
+
+// storeprob (utility_pack.f:622-687) for one production step: every guest
+// molecule of a plotted kind, its centre of mass (com, utility_pack.f:
+// 3565-3596) and sites in fractional coordinates of the unit cell folded
+// into [0, 1) (frac / frac2, utility_pack.f:873-921), binned into each of
+// its kind's plots.  Threads t0, t0 + nthr, ... take one molecule each.
+template <bool LF>
+__device__ inline void prob_step(const DevParams &P, const DevWalkers &W, int w, const int *nmols,
+                                 const GuestXYZ<LF> &G, int t0, int nthr) {
+  const double *r = P.prucell;
+  auto frac = [&](double x, double y, double z, double &a, double &b, double &c) {
+    const double sx = x * r[0] + y * r[3] + z * r[6], sy = x * r[1] + y * r[4] + z * r[7],
+                 sz = x * r[2] + y * r[5] + z * r[8];
+    a = sx - floor(sx);
+    b = sy - floor(sy);
+    c = sz - floor(sz);
+  };
+  for (int g = 1; g <= P.ntpguest; ++g) {
+    bool any = false;
+    for (int p = 0; p < P.nplots; ++p) any |= P.plot_g[p] == g;
+    if (!any) continue;
+    const int mol = P.locguest[g], na = P.natoms[mol], nm = nmols[mol];
+    for (int m = t0; m < nm; m += nthr) {
+      const int base = P.gbase[g] + m * na;
+      double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int s = 0; s < na; ++s) {
+        const double ms = P.site_m[mol][s];
+        wsum += ms;
+        cx += G.x[base + s] * ms;
+        cy += G.y[base + s] * ms;
+        cz += G.z[base + s] * ms;
+      }
+      double ca, cb, cc;
+      frac(cx / wsum, cy / wsum, cz / wsum, ca, cb, cc);
+      for (int p = 0; p < P.nplots; ++p) {
+        if (P.plot_g[p] != g) continue;
+        double *grid = W.prob + ((size_t)w * P.nplots + p) * P.pgridsize;
+        for (int e = 0; e < P.plot_ns[p]; ++e) {
+          const int site = P.plot_site[p][e];
+          if (site == 0) {
+            equitable_bin(P, grid, ca, cb, cc);
+          } else if (site <= na) {
+            double a, b, c;
+            frac(G.x[base + site - 1], G.y[base + site - 1], G.z[base + site - 1], a, b, c);
+            equitable_bin(P, grid, a, b, c);
+          }
+        }
+      }
+    }
+  }
+}
+
 // The production block after a step (gcmc.f:1050-1130), on one wave: count the
 // step (only if accepted with the 'sampling' directive), add N, E, E*N, N*N,
 // E*E and the surface count of every guest kind to the open window (lane
@@ -1950,6 +2057,11 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     const int stat_wave = blockDim.x > 64 ? 1 : 0;
     if ((int)(threadIdx.x >> 6) == stat_wave && (count_stats & FMC_STATS_COUNT))
       stats_step<NS>(P, W, w, sh, (count_stats & FMC_STATS_CLOSE) && step == nsteps - 1);
+    // probability plots on waves 2 and up (every thread of a smaller block)
+    if (P.nplots > 0 && (count_stats & FMC_STATS_COUNT)) {
+      const int p0 = blockDim.x > 128 ? 128 : 0;
+      if ((int)threadIdx.x >= p0) prob_step<LF>(P, W, w, sh.ws.nmols, G, threadIdx.x - p0, blockDim.x - p0);
+    }
     if ((int)threadIdx.x == 64 * stat_wave && count_stats) {
       WState &ws = sh.ws;
       double e = 0.0;

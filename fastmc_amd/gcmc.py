@@ -122,16 +122,100 @@ def _f(x: float, w: int, d: int) -> str:
 
 
 def _e(x: float, w: int, d: int) -> str:
-    """Fortran Ew.d: 0.dddE+xx."""
-    if x == 0.0 or not math.isfinite(x):
-        return _f(x, w, d) if not math.isfinite(x) else f"0.{'0' * d}E+00".rjust(w)
-    e = math.floor(math.log10(abs(x))) + 1
-    m = abs(x) / 10.0 ** e
-    if round(m, d) >= 1.0:
-        m /= 10.0
-        e += 1
-    s = f"{'-' if x < 0 else ''}0.{int(round(m * 10 ** d)):0{d}d}E{'+' if e >= 0 else '-'}{abs(e):02d}"
-    return s.rjust(w)
+    """Fortran Ew.d: [-]0.ddd...E+xx, d significant digits."""
+    if not math.isfinite(x):
+        return _f(x, w, d)
+    if x == 0.0:
+        return f"0.{'0' * d}E+00".rjust(w)
+    m, e = f"{abs(x):.{d - 1}e}".split("e")
+    digits = m.replace(".", "")
+    ex = int(e) + 1
+    return f"{'-' if x < 0 else ''}0.{digits}E{'+' if ex >= 0 else '-'}{abs(ex):02d}".rjust(w)
+
+
+# ---------------------------------------------------- probability plots --
+ANGS2BOHR = 1.889725989                     # utility_pack.f:100
+
+# atmnumber (utility_pack.f:2521-2766): atomic number from a FIELD mass, the
+# first (low, high, Z) range holding it; bounds are single-precision literals
+# in the reference, so they are compared as float32 values.  A mass in no
+# range gives 0 (the reference returns an undefined value).
+_ATM_RANGES = (
+    (0.0, 1.5, 1), (3.9, 4.5, 2), (6.5, 7.1, 3), (8.9, 9.5, 4), (10.5, 11.1, 5), (11.9, 12.5, 6), (13.9, 14.5, 7),
+    (15.5, 16.1, 8), (18.5, 19.1, 9), (19.9, 20.5, 10), (22.5, 23.1, 11), (23.9, 24.5, 12), (26.5, 27.1, 13),
+    (27.9, 28.5, 14), (30.5, 31.1, 15), (31.9, 32.5, 16), (34.9, 36.1, 17), (39.5, 39.9999, 18), (38.9, 39.4, 19),
+    (40.0, 40.5, 20), (44.5, 45.1, 21), (47.5, 48.1, 22), (50.5, 51.1, 23), (51.5, 52.1, 24), (54.5, 55.1, 25),
+    (55.5, 56.1, 26), (58.76, 59.1, 27), (58.5, 59.75, 28), (62.9, 64.1, 29), (64.9, 66.1, 30), (69.5, 70.1, 31),
+    (72.5, 73.1, 32), (74.5, 75.1, 33), (78.5, 79.1, 34), (79.5, 80.1, 35), (83.5, 84.1, 36), (84.9, 86.1, 37),
+    (87.5, 88.1, 38), (88.5, 89.1, 39), (90.9, 91.5, 40), (92.5, 93.1, 41), (95.5, 96.1, 42), (97.9, 98.1, 43),
+    (109.9, 101.5, 44), (102.5, 103.1, 45), (105.9, 106.5, 46), (107.5, 108.1, 47), (111.9, 112.5, 48),
+    (114.5, 115.1, 49), (118.5, 119.1, 50), (121.5, 122.1, 51), (127.5, 128.1, 52), (126.5, 127.1, 53),
+    (130.9, 131.5, 54), (132.5, 133.1, 55), (136.9, 137.5, 56), (138.5, 139.1, 57), (139.9, 140.5, 58),
+    (140.6, 141.1, 59), (144.0, 144.5, 60), (144.9, 145.1, 61), (150.0, 150.6, 62), (151.5, 152.1, 63),
+    (156.9, 157.5, 64), (158.5, 159.1, 65), (162.0, 163.1, 66), (164.5, 165.1, 67), (166.5, 167.9, 68),
+    (168.0, 169.1, 69), (172.9, 173.5, 70), (174.0, 175.1, 71), (178.0, 179.1, 72), (180.0, 181.1, 73),
+    (183.0, 184.1, 74), (185.9, 186.5, 75), (189.9, 190.5, 76), (191.9, 192.5, 77), (194.9, 195.5, 78),
+    (196.5, 197.1, 79), (200.0, 201.1, 80), (203.9, 204.6, 81), (206.9, 207.6, 82), (208.5, 209.1, 83),
+    (209.9, 210.1, 85), (221.9, 222.1, 86), (222.9, 223.1, 87), (225.9, 226.1, 88), (226.9, 227.1, 89),
+    (231.9, 232.1, 90), (230.9, 231.1, 91), (237.9, 238.1, 92), (243.9, 244.1, 94), (242.9, 243.1, 95),
+    (246.9, 247.1, 96), (250.9, 251.1, 98), (251.9, 252.1, 99), (256.9, 257.1, 100), (257.9, 258.1, 101),
+    (258.9, 259.1, 102), (261.9, 262.1, 103), (266.9, 267.1, 104), (267.9, 268.1, 105), (270.9, 271.1, 106),
+    (269.9, 270.1, 107), (268.9, 269.1, 108), (277.9, 278.1, 109), (280.9, 281.1, 110), (284.9, 285.1, 112),
+    (285.9, 286.1, 113), (288.9, 289.1, 114), (292.9, 293.1, 116), (293.9, 294.1, 117))
+
+
+def atmnumber(mass: float) -> int:
+    import numpy as np
+    for lo, hi, z in _ATM_RANGES:
+        if float(np.float32(lo)) <= mass <= float(np.float32(hi)):
+            return z
+    return 0
+
+
+def prob_grid_size(cell, griddim: float, gridfactor=(1, 1, 1)):
+    """Grid points per axis of the probability plots and Widom grid
+    (gcmc.f:258-281)."""
+    from .system import dcell
+    celprp = dcell(cell)
+    out = []
+    for a in range(3):
+        gd = celprp[a] / math.ceil(celprp[a] / gridfactor[a] / griddim) / gridfactor[a]
+        out.append(math.ceil(celprp[a] / (gd * gridfactor[a])))
+    return tuple(out)
+
+
+def cube_text(sysm, grid, ngrid, gridfactor, steps: int) -> str:
+    """writeprob (utility_pack.f:773-871): a Gaussian cube file of one plot,
+    the grid divided by the voxel volume (bohr^3) and the production step
+    count; the framework atoms of one unit cell as the molecule list."""
+    na, nb, nc = ngrid
+    g = [0.0] + [float(sysm.cell.ravel()[i]) / float(gridfactor[i // 3]) * ANGS2BOHR for i in range(9)]
+    vol = ((g[2] * g[6] - g[3] * g[5]) / (na * nb) * g[7] / nc + (g[3] * g[4] - g[1] * g[6]) / (na * nb) * g[8] / nc
+           + (g[1] * g[5] - g[2] * g[4]) / (na * nb) * g[9] / nc)
+    scell = gridfactor[0] * gridfactor[1] * gridfactor[2]
+    fw = sysm.framework_species
+    nfram = 0
+    for sp in fw:
+        nfram = int(nfram + sp.nmols / float(scell) * len(sp.labels))
+    lines = ["probability cube file", "outer loop a, middle loop     b, inner loop c",
+             f"{nfram:6d}" + "".join(_f(0.0, 12, 6) for _ in range(3))]
+    for n, off in ((na, 1), (nb, 4), (nc, 7)):
+        lines.append(f"{n:6d}" + "".join(_f(g[off + k] / n, 12, 6) for k in range(3)))
+    for sp in fw:
+        nmol = int(sp.nmols / float(scell))
+        natms = len(sp.labels)
+        for a in range(nmol * natms):
+            x, y, z = sp.xyz[a]
+            lines.append(f"{atmnumber(float(sp.mass[a % natms])):6d}" + _f(0.0, 12, 6) + _f(x * ANGS2BOHR, 12, 6)
+                         + _f(y * ANGS2BOHR, 12, 6) + _f(z * ANGS2BOHR, 12, 6))
+    row, ip = [], 0
+    for j in range(1, na * nb * nc + 1):
+        ip += 1
+        row.append(_e(float(grid[j - 1]) / vol / float(steps), 15, 6))
+        if j % nc == 0 or ip == 6:
+            lines.append("".join(row))
+            row, ip = [], 0
+    return "\n".join(lines) + "\n"
 
 
 def species_name(sp) -> str:
@@ -232,6 +316,14 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         nwind, nwindsteps = window_steps(sysm.steps, sysm.nwind)
         minchk = check_interval(sysm.eqsteps, sysm.nnumg)
         h.set_sampling(nwindsteps, sysm.accsample, max_windows)
+        plots, cubes = [], []
+        for g, pl in enumerate(sysm.prob_plots, start=1):
+            for k, sites in enumerate(pl, start=1):
+                plots.append((g, sites))
+                cubes.append(f"prob_guest{g:02d}_prob_{k:02d}.cube")
+        if any(sysm.prob_energy):
+            sys.stderr.write("fastmc: energy probability plots ('e') are not provided; skipped\n")
+        ngrid = h.set_prob(sysm.griddim, sysm.gridfactor, plots) if plots else None
         eng = _Engine(h)
         t0 = time.perf_counter()
         r = run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount)
@@ -251,6 +343,10 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         lines += final_report(names, stats, int(st.prodcount), sysm.surftol)
         with open(os.path.join(path, out_name), "w") as fh:
             fh.write("\n".join(lines) + "\n")
+        if plots and st.prodcount > 0:
+            for p, name in enumerate(cubes):
+                with open(os.path.join(path, name), "w") as fh:
+                    fh.write(cube_text(sysm, h.prob(p), ngrid, sysm.gridfactor, int(st.prodcount)))
         return r, stats
     finally:
         h.close()

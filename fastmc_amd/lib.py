@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from .system import (CSystem, FmcGuestStats, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec, FmcWalker,
-                     FmcWalkerStats, FmcWindow, FmcWlParams, FmcWlStatus, TRACE_DTYPE, System, make_walker)
+                     FmcWalkerStats, FmcWindow, FmcWlParams, FmcWlStatus, TRACE_DTYPE, System, make_walker, prob_plots)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
@@ -28,7 +28,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
-           "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats")
+           "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob")
 
 _lib = None
 
@@ -219,6 +219,23 @@ class Handle:
         buf = (FmcWindow * max(n.value, 1))()
         _chk("fmc_get_windows", self.L.fmc_get_windows(self.h, C.c_int(w), buf, C.c_int(n.value), C.byref(n)))
         return buf, n.value
+
+    # ---- probability plots
+    def set_prob(self, griddim: float, gridfactor=(1, 1, 1), plots=()):
+        """Probability plots [(iguest, [sites])] (site 0: centre of mass) on a
+        grid of spacing griddim; returns the grid size (na, nb, nc)."""
+        arr = prob_plots(plots)
+        ng = (C.c_int * 3)()
+        _chk("fmc_set_prob", self.L.fmc_set_prob(self.h, C.c_double(griddim), (C.c_int * 3)(*gridfactor),
+                                                 C.c_int(len(plots)), arr, ng))
+        self._ngrid = tuple(ng)
+        return self._ngrid
+
+    def prob(self, plot: int, w: int = 0):
+        out = np.zeros(int(np.prod(self._ngrid)))
+        _chk("fmc_get_prob", self.L.fmc_get_prob(self.h, C.c_int(w), C.c_int(plot),
+                                                 out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def production_stats(self, w: int = 0):
         ng = len(self.sysm.guests)

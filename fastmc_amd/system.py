@@ -190,6 +190,10 @@ class System:
     nwind: int = 0                  # 'averaging window' (0: not given, initscan default)
     accsample: bool = False         # 'sampling': count accepted steps only
     nnumg: int = 0                  # 'numguests' interval (0: not given, readcontrol default)
+    griddim: float = 0.0            # 'grid spacing' (A) of probability plots / Widom grids
+    gridfactor: tuple = (1, 1, 1)   # 'grid factor': unit cells per supercell axis
+    prob_plots: list = field(default_factory=list)  # per guest: site lists (0 = COM) of 'probability'
+    prob_energy: list = field(default_factory=list) # per guest: an 'e' (energy) plot was requested
     pressures: list = field(default_factory=list)   # Pa per guest
     moves: list = field(default_factory=list)       # per guest dict key -> fraction
     guest_delr: list = field(default_factory=list)
@@ -426,7 +430,8 @@ def read_control(path: str, nguests: int):
         lines = f.read().split("\n")
     out = {"temperature": None, "rcut": None, "delr": None, "eps": None, "alpha": None,
            "kmax": None, "rvdw": None, "overlap": 1.5, "surftol": 0.0, "steps": 1, "eqsteps": 0,
-           "nwind": 0, "accsample": False, "nnumg": 0,
+           "nwind": 0, "accsample": False, "nnumg": 0, "griddim": 0.0, "gridfactor": (1, 1, 1),
+           "prob_plots": [[] for _ in range(nguests)], "prob_energy": [False] * nguests,
            "pressures": [-1.0] * nguests, "moves": [dict() for _ in range(nguests)],
            "guest_delr": [None] * nguests, "topmoves": {}}
     i = 1
@@ -442,7 +447,27 @@ def read_control(path: str, nguests: int):
                 i += 1
                 if r == "" or r.startswith("#"):
                     continue
-                if "press" in r and "move" not in r:
+                if "probability" in r:
+                    # readinputs.f:642-672: N lines, each 'n s1 .. sn' (site
+                    # 0: centre of mass) or starting with 'e' (energy plot)
+                    nplot, _ = intstr(r)
+                    got = 0
+                    while got < nplot and i < len(lines):
+                        pr = _lc(lines[i]).strip()
+                        i += 1
+                        if pr == "" or pr.startswith("#"):
+                            continue
+                        got += 1
+                        if pr.startswith("e"):
+                            out["prob_energy"][ig - 1] = True
+                            continue
+                        ns, rest = intstr(pr)
+                        sites = []
+                        for _ in range(ns):
+                            v, rest = intstr(rest)
+                            sites.append(v)
+                        out["prob_plots"][ig - 1].append(sites)
+                elif "press" in r and "move" not in r:
                     v, _ = dblstr(r)
                     out["pressures"][ig - 1] = v * 1.0e5
                 elif "move" in r:
@@ -459,6 +484,15 @@ def read_control(path: str, nguests: int):
             continue
         if "temp" in rec:
             out["temperature"], _ = dblstr(rec)
+        elif "grid" in rec and ("spa" in rec or "fac" in rec):
+            # utility_pack.f:436-442
+            if "spa" in rec:
+                out["griddim"], _ = dblstr(rec)
+            else:
+                a, rest = intstr(rec)
+                b, rest = intstr(rest)
+                c, rest = intstr(rest)
+                out["gridfactor"] = (a, b, c)
         elif "averaging window" in rec:
             out["nwind"], _ = intstr(rec)           # utility_pack.f:394-396
         elif "sampling" in rec:
@@ -556,6 +590,8 @@ def read_system(dirpath: str) -> System:
                   kmax=kmax, eps_precision=ctl["eps"] or 0.0, overlap=ctl["overlap"],
                   surftol=ctl["surftol"], steps=ctl["steps"], eqsteps=ctl["eqsteps"],
                   nwind=ctl["nwind"], accsample=ctl["accsample"], nnumg=ctl["nnumg"],
+                  griddim=ctl["griddim"], gridfactor=ctl["gridfactor"], prob_plots=ctl["prob_plots"],
+                  prob_energy=ctl["prob_energy"],
                   pressures=ctl["pressures"], moves=moves, guest_delr=gdelr)
 
 
@@ -616,6 +652,23 @@ FMC_WIN_STATS = 6
 
 class FmcWindow(C.Structure):
     _fields_ = [("count", C.c_double), ("sums", (C.c_double * FMC_WIN_STATS) * FMC_MAX_SPECIES)]
+
+
+FMC_MAX_PLOTS = 8
+
+
+class FmcProbPlot(C.Structure):
+    _fields_ = [("iguest", C.c_int), ("nsites", C.c_int), ("sites", C.c_int * (FMC_MAX_SITES + 1))]
+
+
+def prob_plots(plots):
+    """(iguest, site list) pairs -> an FmcProbPlot array."""
+    arr = (FmcProbPlot * max(len(plots), 1))()
+    for i, (g, sites) in enumerate(plots):
+        arr[i].iguest, arr[i].nsites = g, len(sites)
+        for e, v in enumerate(sites):
+            arr[i].sites[e] = v
+    return arr
 
 
 class FmcGuestStats(C.Structure):

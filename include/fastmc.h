@@ -365,6 +365,33 @@ int fmc_stats_from_windows(const fmc_window *win, int nwin, int ntpguest, double
  * out[ntpguest]. */
 int fmc_production_stats(fmc_handle *h, int w, fmc_guest_stats *out);
 
+/* ---- probability plots of guest positions (storeprob, utility_pack.f:622-687) */
+
+/* One plot: the guest type and its site list (site 0: the molecule's centre
+ * of mass; else a 1-based site), the CONTROL 'probability' lines
+ * (readinputs.f:642-672). */
+#define FMC_MAX_PLOTS 8
+typedef struct fmc_prob_plot {
+  int iguest;
+  int nsites;
+  int sites[FMC_MAX_SITES + 1];
+} fmc_prob_plot;
+
+/* Turn on the plots: every production step (fmc_run with FMC_STATS_COUNT)
+ * bins the centre of mass or sites of every guest molecule of each plot's
+ * type with equitable_bin's 8-corner weights (utility_pack.f:2407-2519) into
+ * that walker's grid.  Grid: spacing griddim (A) and 'grid factor' unit cells
+ * per axis (gcmc.f:258-281), positions folded into the unit cell
+ * (cell / gridfactor, gcmc.f:299-306).  ngrid_out receives the grid size.
+ * Clears every walker's grids.  The energy plot ('e') is not provided. */
+int fmc_set_prob(fmc_handle *h, double griddim, const int gridfactor[3], int nplots, const fmc_prob_plot *plots,
+                 int ngrid_out[3]);
+
+/* Walker w's grid of plot p (0-based): [ngrid a * ngrid b * ngrid c], c
+ * fastest, the reference's 1-based bin b at out[b-1]; weights summed over the
+ * production steps (the cube files divide by voxel volume and step count). */
+int fmc_get_prob(fmc_handle *h, int w, int plot, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,6 +149,12 @@ struct orc_state {
   long long prodcount, rollcount;
   double chain[FMC_MAX_SPECIES + 1][FMC_WIN_STATS];
   fmc_window *win;
+  /* probability plots (storeprob): grid, unit-cell reciprocal, plots, grids
+   * [nplots][gridsize + 1] with the reference's 1-based bins */
+  int nplots, ngrida, ngridb, ngridc, gridsize;
+  double rucell[10];
+  fmc_prob_plot plots[FMC_MAX_PLOTS];
+  double *grid;
   /* pending trial (for accept/reject) */
   int pend_kind, pend_iguest, pend_choice, pend_valid;
   double pend_delrc, pend_engsictmp, pend_chgtmp;
@@ -1646,6 +1652,7 @@ orc_state *orc_create(const fmc_system *sys, const fmc_walker *wk) {
 void orc_destroy(orc_state *s) {
   if (!s) return;
   free(s->win);
+  free(s->grid);
   free(s->erc); free(s->vvv); free(s->lstvdw); free(s->ltpvdw); free(s->prm_eps); free(s->prm_sig);
   free(s->steadd); free(s->kv);
   for (int m = 0; m <= FMC_MAX_SPECIES; ++m) {
@@ -1959,6 +1966,160 @@ static void stats_step(orc_state *s, int accepted, int final_step) {
   }
 }
 
+/* equitable_bin, utility_pack.f:2407-2519 */
+static void equitable_bin(orc_state *s, double fposa, double fposb, double fposc, int subgrid, double val) {
+  const int ngrida = s->ngrida, ngridb = s->ngridb, ngridc = s->ngridc;
+  double *grid = s->grid + (size_t)subgrid * (size_t)(s->gridsize + 1);
+  int alidx, aridx, blidx, bridx, clidx, cridx;
+  double al, ar, bl, br, cl, cr;
+  double gpta = ngrida * (fposa);
+  int aidx = (int)ceil(gpta);
+  double apart = gpta - floor(gpta / 1.0) * 1.0;
+  if (apart > 0.5) {
+    if (aidx >= ngrida) {
+      alidx = (ngrida - 1) * ngridb * ngridc;
+      aridx = 0;
+    } else {
+      alidx = (aidx - 1) * ngridb * ngridc;
+      aridx = (aidx)*ngridb * ngridc;
+    }
+    al = 1.5 - apart;
+    ar = apart - 0.5;
+  } else {
+    if (aidx <= 1) {
+      alidx = (ngrida - 1) * ngridb * ngridc;
+      aridx = 0;
+    } else {
+      alidx = (aidx - 2) * ngridb * ngridc;
+      aridx = (aidx - 1) * ngridb * ngridc;
+    }
+    al = 0.5 - apart;
+    ar = 0.5 + apart;
+  }
+  double gptb = ngridb * (fposb);
+  int bidx = (int)ceil(gptb);
+  double bpart = gptb - floor(gptb / 1.0) * 1.0;
+  if (bpart > 0.5) {
+    if (bidx >= ngridb) {
+      blidx = (ngridb - 1) * ngridc;
+      bridx = 1;
+    } else {
+      blidx = (bidx - 1) * ngridc;
+      bridx = (bidx)*ngridc;
+    }
+    bl = 1.5 - bpart;
+    br = bpart - 0.5;
+  } else {
+    if (bidx <= 1) {
+      blidx = (ngridb - 1) * ngridc;
+      bridx = 0;
+    } else {
+      blidx = (bidx - 2) * ngridc;
+      bridx = (bidx - 1) * ngridc;
+    }
+    bl = 0.5 - bpart;
+    br = 0.5 + bpart;
+  }
+  double gptc = ngridc * (fposc);
+  int cidx = (int)ceil(gptc);
+  double cpart = gptc - floor(gptc / 1.0) * 1.0;
+  if (cpart > 0.5) {
+    if (cidx >= ngridc) {
+      clidx = ngridc;
+      cridx = 1;
+    } else {
+      clidx = cidx;
+      cridx = cidx + 1;
+    }
+    cl = 1.5 - cpart;
+    cr = cpart - 0.5;
+  } else {
+    if (cidx <= 1) {
+      clidx = ngridc;
+      cridx = 1;
+    } else {
+      clidx = cidx - 1;
+      cridx = cidx;
+    }
+    cl = 0.5 - cpart;
+    cr = 0.5 + cpart;
+  }
+  const int bins[8] = {alidx + blidx + clidx, aridx + blidx + clidx, alidx + bridx + clidx, alidx + blidx + cridx,
+                       aridx + bridx + clidx, aridx + blidx + cridx, alidx + bridx + cridx, aridx + bridx + cridx};
+  const double wts[8] = {(al * bl * cl) * val, (ar * bl * cl) * val, (al * br * cl) * val, (al * bl * cr) * val,
+                         (ar * br * cl) * val, (ar * bl * cr) * val, (al * br * cr) * val, (ar * br * cr) * val};
+  for (int i = 0; i < 8; ++i)
+    if (bins[i] >= 1 && bins[i] <= s->gridsize) grid[bins[i]] = grid[bins[i]] + wts[i];
+}
+
+/* frac2 / frac, utility_pack.f:873-921: fractional coordinates in the unit
+ * cell, folded by modulo(x, 1.0) */
+static void frac1(const orc_state *s, double *x, double *y, double *z) {
+  const double *r = s->rucell;
+  double ssx = *x * r[1] + *y * r[4] + *z * r[7];
+  double ssy = *x * r[2] + *y * r[5] + *z * r[8];
+  double ssz = *x * r[3] + *y * r[6] + *z * r[9];
+  *x = ssx - floor(ssx);
+  *y = ssy - floor(ssy);
+  *z = ssz - floor(ssz);
+}
+
+/* storeprob, utility_pack.f:622-687: per guest type and molecule, the centre
+ * of mass and the sites in unit-cell fractional coordinates, binned into
+ * the type's plots (site 0: the centre of mass) */
+static void storeprob(orc_state *s) {
+  for (int iguest = 1; iguest <= s->ntpguest; ++iguest) {
+    int mol = s->locguest[iguest], nmols = s->nummols[mol], natms = s->numatoms[mol];
+    int jj = 0;
+    for (int itmols = 1; itmols <= nmols; ++itmols) {
+      for (int itatm = 1; itatm <= natms; ++itatm) {
+        ++jj;
+        s->newx[itatm] = s->molx[mol][jj];
+        s->newy[itatm] = s->moly[mol][jj];
+        s->newz[itatm] = s->molz[mol][jj];
+      }
+      double comx, comy, comz;
+      com(s, natms, mol, s->newx, s->newy, s->newz, &comx, &comy, &comz);
+      frac1(s, &comx, &comy, &comz);
+      for (int i = 1; i <= natms; ++i) frac1(s, &s->newx[i], &s->newy[i], &s->newz[i]);
+      for (int p = 0; p < s->nplots; ++p) {
+        const fmc_prob_plot *pl = &s->plots[p];
+        if (pl->iguest != iguest) continue;
+        for (int e = 0; e < pl->nsites; ++e) {
+          int jatm = pl->sites[e];
+          if (jatm == 0)
+            equitable_bin(s, comx, comy, comz, p, 1.0);
+          else if (jatm <= natms)
+            equitable_bin(s, s->newx[jatm], s->newy[jatm], s->newz[jatm], p, 1.0);
+        }
+      }
+    }
+  }
+}
+
+int orc_set_prob(orc_state *s, const int ngrid[3], const int gridfactor[3], int nplots, const fmc_prob_plot *plots) {
+  if (nplots < 0 || nplots > FMC_MAX_PLOTS) return FMC_ERR_ARG;
+  s->ngrida = ngrid[0];
+  s->ngridb = ngrid[1];
+  s->ngridc = ngrid[2];
+  s->gridsize = ngrid[0] * ngrid[1] * ngrid[2];
+  double ucell[10];
+  for (int i = 1; i <= 9; ++i) ucell[i] = s->cell[i] / (double)gridfactor[(i - 1) / 3];
+  invert(ucell, s->rucell);
+  s->nplots = nplots;
+  for (int p = 0; p < nplots; ++p) s->plots[p] = plots[p];
+  free(s->grid);
+  s->grid = (double *)calloc((size_t)(nplots > 0 ? nplots : 1) * (size_t)(s->gridsize + 1), sizeof(double));
+  return s->grid ? 0 : FMC_ERR_ALLOC;
+}
+
+int orc_get_prob(const orc_state *s, int plot, double *out) {
+  if (plot < 0 || plot >= s->nplots) return FMC_ERR_ARG;
+  const double *g = s->grid + (size_t)plot * (size_t)(s->gridsize + 1);
+  for (int b = 1; b <= s->gridsize; ++b) out[b - 1] = g[b];
+  return 0;
+}
+
 int orc_set_sampling(orc_state *s, int nwindsteps, int accepted_only) {
   if (nwindsteps < 0) return FMC_ERR_ARG;
   s->st_nwind = nwindsteps;
@@ -2227,7 +2388,10 @@ static int run_chain(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags) 
       r->surface = tr.surface;
       r->nsurface = s->surfacemols[mol];
     }
-    if (flags & FMC_STATS_COUNT) stats_step(s, accepted, (flags & FMC_STATS_CLOSE) && step == nsteps);
+    if (flags & FMC_STATS_COUNT) {
+      stats_step(s, accepted, (flags & FMC_STATS_CLOSE) && step == nsteps);
+      if (s->nplots > 0) storeprob(s);
+    }
   }
   return 0;
 }

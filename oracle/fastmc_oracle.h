@@ -29,6 +29,10 @@ int orc_run(orc_state *s, int nsteps, fmc_trace_rec *trace);
 int orc_run_stats(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags);
 int orc_set_sampling(orc_state *s, int nwindsteps, int accepted_only);
 int orc_get_windows(const orc_state *s, fmc_window *out, int cap, int *nwin, long long *prodcount);
+/* probability plots (storeprob, utility_pack.f:622-687) on production steps;
+ * orc_get_prob writes plot p's grid with bin b at out[b-1] */
+int orc_set_prob(orc_state *s, const int ngrid[3], const int gridfactor[3], int nplots, const fmc_prob_plot *plots);
+int orc_get_prob(const orc_state *s, int plot, double *out);
 /* surfacemols by guest type */
 int orc_get_surface(const orc_state *s, int *surf);
 /* attempts / acceptances by move kind (index FMC_MOVE_*), summed over guests */

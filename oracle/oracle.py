@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, FmcWindow, TRACE_DTYPE, make_walker)
+from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, FmcWindow, TRACE_DTYPE, make_walker, prob_plots)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
@@ -36,7 +36,7 @@ def lib():
                      "orc_trial_displace", "orc_accept", "orc_reject", "orc_run", "orc_get_guests",
                      "orc_get_sk", "orc_get_table", "orc_get_kvec", "orc_get_energy",
                      "orc_full_resum", "orc_wl_setup", "orc_wl_run", "orc_wl_get", "orc_run_stats",
-                     "orc_set_sampling", "orc_get_windows", "orc_get_surface"):
+                     "orc_set_sampling", "orc_get_windows", "orc_get_surface", "orc_set_prob", "orc_get_prob"):
             getattr(L, name).restype = C.c_int
         L.orc_duni_seq.restype = None
         L.orc_wl_setup.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -134,6 +134,20 @@ class Oracle:
         buf = (FmcWindow * max(n.value, 1))()
         lib().orc_get_windows(C.c_void_p(self.h), buf, C.c_int(n.value), C.byref(n), C.byref(pc))
         return buf, n.value, pc.value
+
+    def set_prob(self, ngrid, gridfactor, plots):
+        """Probability plots: plots = [(iguest, [sites])] (site 0: centre of mass)."""
+        arr = prob_plots(plots)
+        rc = lib().orc_set_prob(C.c_void_p(self.h), (C.c_int * 3)(*ngrid), (C.c_int * 3)(*gridfactor),
+                                C.c_int(len(plots)), arr)
+        if rc:
+            raise RuntimeError(f"orc_set_prob rc={rc}")
+        self._ngrid = tuple(ngrid)
+
+    def prob(self, plot):
+        out = np.zeros(int(np.prod(self._ngrid)))
+        lib().orc_get_prob(C.c_void_p(self.h), C.c_int(plot), out.ctypes.data_as(C.c_void_p))
+        return out
 
     def surface(self):
         out = np.zeros(8, dtype=np.int32)

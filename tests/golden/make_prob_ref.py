@@ -1,0 +1,97 @@
+"""Probability-grid known answers from the compiled reference (run in the
+build container only; the reference never travels):
+
+    make -C oracle ref                   # builds oracle/_ref/gcmc_ser.x from the reference sources
+    python tests/golden/make_prob_ref.py
+
+Each case adds the reference's probability-plot directives to CONTROL
+('grid spacing D' at the top, utility_pack.f:436-442; per guest
+'probability N' followed by N lines 'n s1 .. sn', site 0 = the centre of
+mass, readinputs.f:642-672), runs the reference's serial program and keeps,
+from OUTPUT, the duni seeds, the production start and total step count and
+the grid size (gcmc.f:278-281); from every prob_guestGG_prob_PP.cube file
+(writeprob, utility_pack.f:773-871) the whole text.  The values are
+grid / voxel volume (bohr^3) / production steps in E15.6.
+"""
+from __future__ import annotations
+
+import glob
+import gzip
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, HERE)
+
+EXE = os.path.join(REPO, "oracle", "_ref", "gcmc_ser.x")
+
+
+def add_prob(ctl, spacing, plots):
+    """plots: per guest a list of site lists (0 = centre of mass)."""
+    ctl = ctl.replace("history 0\n", f"history 0\ngrid spacing {spacing}\n")
+    blocks = ctl.split("&end")
+    out = []
+    for g, b in enumerate(blocks[:-1]):
+        lines = "".join(f"  {len(s)} " + " ".join(str(x) for x in s) + "\n" for s in plots[g])
+        out.append(b + (f"  probability {len(plots[g])}\n" + lines if plots[g] else ""))
+    out.append(blocks[-1])
+    return "&end".join(out)
+
+
+def write_cfg0(d):
+    import make_stats_ref
+    make_stats_ref.write_cfg0(d, 200, 1000)
+    p = os.path.join(d, "CONTROL")
+    ctl = add_prob(open(p).read(), 2.0, [[[0], [1, 3]]])
+    open(p, "w").write(ctl)
+
+
+def write_calf20(d):
+    import make_stats_ref
+    make_stats_ref.write_calf20_moves6(d, 400, 1200)
+    p = os.path.join(d, "CONTROL")
+    ctl = add_prob(open(p).read(), 1.5, [[[0]], [[3], [1, 2]]])
+    open(p, "w").write(ctl)
+
+
+CASES = [("cfg0_prob", write_cfg0), ("calf20_prob", write_calf20)]
+
+
+def run_case(name, writer):
+    d = tempfile.mkdtemp(prefix=f"fmc_prob_{name}_")
+    try:
+        writer(d)
+        control = open(os.path.join(d, "CONTROL")).read()
+        subprocess.run([EXE], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=3600, check=False)
+        out = open(os.path.join(d, "OUTPUT")).read()
+        cubes = {os.path.basename(p): open(p).read() for p in sorted(glob.glob(os.path.join(d, "prob_guest*.cube")))}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    seeds = [int(x) for x in re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]
+    total = int(re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out).group(1))
+    m = re.search(r"Starting production at\s+(\d+)", out)
+    start = int(m.group(1)) if m else 1
+    grid = [int(x) for x in re.search(r"Probability grid size\s+:\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]
+    return {"name": name, "control": control, "seeds": seeds, "start": start, "total": total, "grid": grid,
+            "cubes": cubes}
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit(f"{EXE} missing: run `make -C oracle ref` first")
+    res = [run_case(*c) for c in CASES]
+    for r in res:
+        print({k: (v if k != "cubes" else list(v)) for k, v in r.items() if k != "control"}, flush=True)
+    with gzip.open(os.path.join(HERE, "prob_ref.json.gz"), "wt") as fh:
+        json.dump(res, fh)
+
+
+if __name__ == "__main__":
+    main()
