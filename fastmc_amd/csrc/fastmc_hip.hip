@@ -543,6 +543,8 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   P.accsample = 0;
   P.maxwin = 0;
   P.nplots = 0;
+  P.nnumg = 0;
+  P.numg_cap = 0;
   // tail constants, lrcorrect eadd (vdw_terms.f:581-645)
   P.ntri = ntri;
   {
@@ -1009,6 +1011,7 @@ int fmc_destroy(fmc_handle *h) {
   if (h->dtrace) hipFree(h->dtrace);
   if (h->dw.win) hipFree(h->dw.win);
   if (h->dw.prob) hipFree(h->dw.prob);
+  if (h->dw.numg) hipFree(h->dw.numg);
   for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
                   (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
                   (void *)h->g_vt})
@@ -1959,6 +1962,45 @@ int fmc_set_prob(fmc_handle *h, double griddim, const int gridfactor[3], int npl
       for (int a = 0; a < 3; ++a) ngrid_out[a] = P.pgrid[a];
   }
   HIPCHK(hipMemcpy(h->dp, &P, sizeof(DevParams), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int fmc_set_numguests(fmc_handle *h, int nnumg, int cap) {
+  if (!h || nnumg < 0 || cap < 0 || (nnumg > 0 && cap < 1)) return FMC_ERR_ARG;
+  DevParams &P = h->hp;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  if (h->dw.numg) HIPCHK(hipFree(h->dw.numg));
+  h->dw.numg = nullptr;
+  P.nnumg = 0;
+  P.numg_cap = 0;
+  if (nnumg > 0) {
+    HIPCHK(hipMalloc(&h->dw.numg, sizeof(fmc_numguests_rec) * (size_t)h->nwalkers * cap));
+    HIPCHK(hipMemset(h->dw.numg, 0, sizeof(fmc_numguests_rec) * (size_t)h->nwalkers * cap));
+    P.nnumg = nnumg;
+    P.numg_cap = cap;
+  }
+  HIPCHK(hipMemcpy(h->dp, &P, sizeof(DevParams), hipMemcpyHostToDevice));
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  for (auto &s : ws) {
+    s.nnumg_n = 0;
+    s.overflow &= ~4;
+  }
+  HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int fmc_get_numguests(fmc_handle *h, int w, fmc_numguests_rec *out, int cap, int *n) {
+  if (!h || w < 0 || w >= h->nwalkers || cap < 0 || (cap > 0 && !out)) return FMC_ERR_ARG;
+  WState s;
+  HIPCHK(hipMemcpy(&s, h->dw.ws + w, sizeof s, hipMemcpyDeviceToHost));
+  const int total = (int)s.nnumg_n, m = std::min(total, cap);
+  if (n) *n = total;
+  if (m > 0)
+    HIPCHK(hipMemcpy(out, h->dw.numg + (size_t)w * h->hp.numg_cap, sizeof(fmc_numguests_rec) * (size_t)m,
+                     hipMemcpyDeviceToHost));
   return FMC_OK;
 }
 

@@ -91,6 +91,8 @@ struct DevParams {
   // reciprocal of the unit cell used for folding, and per plot the guest
   // kind and site list (0: centre of mass)
   int nplots, pgrid[3], pgridsize;
+  // guest-count trace every nnumg steps (0: off), numg_cap records per walker
+  int nnumg, numg_cap;
   double prucell[9];
   int plot_g[FMC_MAX_PLOTS], plot_ns[FMC_MAX_PLOTS], plot_site[FMC_MAX_PLOTS][FMC_MAX_SITES + 1];
   // tables
@@ -140,7 +142,8 @@ struct WState {
   // closed windows, and the open window's sums per guest g of N, E, E*N,
   // N*N, E*E and the surface count (chainstats, gcmc.f:1062-1075)
   int surfmols[KMAXS + 1];
-  long long prodcount, rollcount, nwin;
+  long long prodcount, rollcount, nwin, nnumg_n;
+  double delE_last[KMAXS + 1];    // delE after the last step that moved (a switch leaves it)
   double chain[KMAXS + 1][FMC_WIN_STATS];
   // Wang-Landau walker: window [wl_min, wl_max] of N, preload target,
   // incremental minimum of the window's visit histogram, f and ln f
@@ -168,6 +171,7 @@ struct DevWalkers {
   int *wl_hist;              // [w][wl_nbins] visits (visit_hist)
   fmc_window *win;           // [w][maxwin] closed averaging windows
   double *prob;              // [w][nplots][pgridsize] probability plots
+  fmc_numguests_rec *numg;   // [w][numg_cap] guest-count trace
 };
 
 __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872

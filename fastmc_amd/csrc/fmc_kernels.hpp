@@ -14,6 +14,9 @@ struct Acc {
   static constexpr int NACC = 6 * NS + 2;
   // layout: [0,NS) real_new, [NS,2NS) vdw_new, [2NS,3NS) real_old, [3NS,4NS) vdw_old,
   //         4NS rec_new, 4NS+1 rec_old, [4NS+2,5NS+2) mix_new, [5NS+2,6NS+2) mix_old
+  // reduced with two more rows, the flag counts of the new and the existing
+  // configuration (overlaps + 1024 x surface contacts, exact in FP64)
+  static constexpr int NRED = NACC + 2;
 };
 
 template <int NS>
@@ -32,9 +35,8 @@ struct Smem {
   int accepted, drew, ovl_new, ovl_old;
   int srf_new, srf_old;  // surface_check flags of the new / existing position
   double xn[KSITE][3], xo[KSITE][3];
-  double red[KWAVES][Acc<NS>::NACC];
-  double tot[Acc<NS>::NACC];
-  int ovlw[KWAVES][2];
+  double red[KWAVES][Acc<NS>::NRED];
+  double tot[Acc<NS>::NRED];
   double estep, rande, delrc;
   double delE[KMAXS + 1], delrc_mol[KMAXC + 1];
   double comp[4];
@@ -1078,39 +1080,35 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   }
   // ---- workgroup reduction: within each wave by recursive halving (the
   // accumulators of an absent configuration hold zeros), then across waves
-  wave_sum_rows<A::NACC>(acc, lane, sh.red[wave]);
-  // overlap (bit 0) and surface (bit 1) flags of the wave
-  unsigned long long bn = __ballot(ovl_new & 1), bo = __ballot(ovl_old & 1);
-  unsigned long long sn = __ballot(ovl_new & 2), so = __ballot(ovl_old & 2);
-  if (lane == 0) {
-    sh.ovlw[wave][0] = (int)(bn != 0ULL) | ((int)(sn != 0ULL) << 1);
-    sh.ovlw[wave][1] = (int)(bo != 0ULL) | ((int)(so != 0ULL) << 1);
+  {
+    double ax[A::NRED];
+#pragma unroll
+    for (int i = 0; i < A::NACC; ++i) ax[i] = acc[i];
+    // overlap (bit 0) and surface (bit 1) flags counted with the sums
+    ax[A::NACC] = (double)((ovl_new & 1) + 1024 * ((ovl_new >> 1) & 1));
+    ax[A::NACC + 1] = (double)((ovl_old & 1) + 1024 * ((ovl_old >> 1) & 1));
+    wave_sum_rows<A::NRED>(ax, lane, sh.red[wave]);
   }
   __syncthreads();
   // wave 0 finishes the sums; the decision that reads them runs on wave 0
   // too, so no further barrier is needed before it (callers that read
   // sh.tot or the overlap flags from other waves synchronise first)
-  if (wave == 0) {
-    if (lane < A::NACC) {
-      // pairwise over the waves (missing waves count as zero)
-      double v[KWAVES];
+  if (wave == 0 && lane < A::NRED) {
+    // pairwise over the waves (missing waves count as zero)
+    double v[KWAVES];
 #pragma unroll
-      for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
+    for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
 #pragma unroll
-      for (int st = KWAVES / 2; st >= 1; st >>= 1)
+    for (int st = KWAVES / 2; st >= 1; st >>= 1)
 #pragma unroll
-        for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
-      sh.tot[lane] = v[0];
-    } else if (lane == A::NACC) {
-      int a = 0, b = 0;
-      for (int wv = 0; wv < nwaves; ++wv) {
-        a |= sh.ovlw[wv][0];
-        b |= sh.ovlw[wv][1];
-      }
-      sh.ovl_new = a & 1;
-      sh.ovl_old = b & 1;
-      sh.srf_new = a >> 1;
-      sh.srf_old = b >> 1;
+      for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
+    sh.tot[lane] = v[0];
+    if (lane >= A::NACC) {
+      const long long n = (long long)v[0];
+      int &ovl = lane == A::NACC ? sh.ovl_new : sh.ovl_old;
+      int &srf = lane == A::NACC ? sh.srf_new : sh.srf_old;
+      ovl = (n % 1024) != 0;
+      srf = n >= 1024;
     }
   }
 }
@@ -1930,6 +1928,48 @@ __device__ inline void prob_step(const DevParams &P, const DevWalkers &W, int w,
   }
 }
 
+// The numguests.out record of a step (gcmc.f:1138-1163), lane g - 1 of one
+// wave for guest kind g: counts, energy, delE and the move flags of the
+// step.  delE is the reference's module array: zero after a rejection, the
+// accepted move's change after an acceptance, unchanged by a switch that
+// does not move; it is kept for every step and written every nnumg steps.
+template <int NS>
+__device__ inline void numg_step(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
+  WState &ws = sh.ws;
+  const int lane = threadIdx.x & 63;
+  if (lane < P.ntpguest && sh.mkind != FMC_MOVE_SWITCH) {
+    const int m = P.locguest[lane + 1];
+    ws.delE_last[m] = sh.accepted ? sh.delE[m] : 0.0;
+  }
+  if (ws.steps % P.nnumg != 0) return;
+  const long long n = ws.nnumg_n;
+  const bool room = n < P.numg_cap;
+  if (room && lane < P.ntpguest) {
+    fmc_numguests_rec *r = W.numg + (size_t)w * P.numg_cap + n;
+    const int g = lane + 1, m = P.locguest[g], mk = sh.mkind;
+    const bool swap = mk == FMC_MOVE_SWAP;
+    const int gm = swap ? sh.swap_ig : sh.g;
+    if (lane == 0) r->step = ws.steps;
+    r->nmols[lane] = ws.nmols[m];
+    r->energy[lane] = ws.energy[m];
+    r->delE[lane] = ws.delE_last[m];
+    r->moves[lane][0] = mk == FMC_MOVE_INSERT && gm == g;
+    r->moves[lane][1] = mk == FMC_MOVE_DELETE && gm == g;
+    r->moves[lane][2] = gm != g                       ? 0
+                        : mk == FMC_MOVE_DISPLACE  ? 1
+                        : mk == FMC_MOVE_TRANSLATE ? 2
+                        : mk == FMC_MOVE_ROTATE    ? 3
+                                                   : 0;
+    r->moves[lane][3] = mk == FMC_MOVE_JUMP && gm == g;
+    r->moves[lane][4] = swap && (sh.swap_ig == g || sh.swap_jg == g);
+    r->moves[lane][5] = 0;  // a switch does not move with fewer than three guest kinds
+  }
+  if (lane == 0) {
+    if (room) ws.nnumg_n = n + 1;
+    else ws.overflow |= 4;
+  }
+}
+
 // The production block after a step (gcmc.f:1050-1130), on one wave: count the
 // step (only if accepted with the 'sampling' directive), add N, E, E*N, N*N,
 // E*E and the surface count of every guest kind to the open window (lane
@@ -2057,6 +2097,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     const int stat_wave = blockDim.x > 64 ? 1 : 0;
     if ((int)(threadIdx.x >> 6) == stat_wave && (count_stats & FMC_STATS_COUNT))
       stats_step<NS>(P, W, w, sh, (count_stats & FMC_STATS_CLOSE) && step == nsteps - 1);
+    if ((int)(threadIdx.x >> 6) == stat_wave && P.nnumg > 0) numg_step<NS>(P, W, w, sh);
     // probability plots on waves 2 and up (every thread of a smaller block)
     if (P.nplots > 0 && (count_stats & FMC_STATS_COUNT)) {
       const int p0 = blockDim.x > 128 ? 128 : 0;

@@ -184,6 +184,24 @@ def prob_grid_size(cell, griddim: float, gridfactor=(1, 1, 1)):
     return tuple(out)
 
 
+def numguests_text(records, n: int, g: int) -> str:
+    """numguests.out lines of guest type g (1-based) from n trace records
+    (format (i9,i7,2f20.6,7i4), gcmc.f:1150-1152)."""
+    out = []
+    for r in records[:n]:
+        out.append(f"{r.step:9d}{r.nmols[g - 1]:7d}{_f(r.energy[g - 1], 20, 6)}{_f(r.delE[g - 1], 20, 6)}"
+                   + "".join(f"{r.moves[g - 1][k]:4d}" for k in range(6)))
+    return "".join(line + "\n" for line in out)
+
+
+def numguests_names(ntpguest: int):
+    """branch01/numguests.out for one guest type, numguestsGG.out for more
+    (gcmc.f:478-493)."""
+    if ntpguest == 1:
+        return ["numguests.out"]
+    return [f"numguests{g:02d}.out" for g in range(1, ntpguest + 1)]
+
+
 def cube_text(sysm, grid, ngrid, gridfactor, steps: int) -> str:
     """writeprob (utility_pack.f:773-871): a Gaussian cube file of one plot,
     the grid divided by the voxel volume (bohr^3) and the production step
@@ -324,6 +342,10 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         if any(sysm.prob_energy):
             sys.stderr.write("fastmc: energy probability plots ('e') are not provided; skipped\n")
         ngrid = h.set_prob(sysm.griddim, sysm.gridfactor, plots) if plots else None
+        # numguests.out every nnumg steps (readinputs.f:713-717,910-916)
+        nnumg = sysm.nnumg or (sysm.eqsteps // 1000 if sysm.eqsteps > 1000 else 1)
+        last = schedule(sysm.eqsteps, sysm.steps, minchk)[1] * (4 if sysm.accsample else 1)
+        h.set_numguests(nnumg, last // nnumg + 1)
         eng = _Engine(h)
         t0 = time.perf_counter()
         r = run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount)
@@ -343,6 +365,13 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         lines += final_report(names, stats, int(st.prodcount), sysm.surftol)
         with open(os.path.join(path, out_name), "w") as fh:
             fh.write("\n".join(lines) + "\n")
+        recs, nrec = h.numguests(0)
+        os.makedirs(os.path.join(path, "branch01"), exist_ok=True)
+        for g, name in enumerate(numguests_names(len(sysm.guests)), start=1):
+            with open(os.path.join(path, "branch01", name), "w") as fh:
+                fh.write(numguests_text(recs, nrec, g))
+        if st.overflow & 4:
+            sys.stderr.write("fastmc: numguests records truncated\n")
         if plots and st.prodcount > 0:
             for p, name in enumerate(cubes):
                 with open(os.path.join(path, name), "w") as fh:

@@ -10,8 +10,9 @@ import os
 
 import numpy as np
 
-from .system import (CSystem, FmcGuestStats, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec, FmcWalker,
-                     FmcWalkerStats, FmcWindow, FmcWlParams, FmcWlStatus, TRACE_DTYPE, System, make_walker, prob_plots)
+from .system import (CSystem, FmcGuestStats, FmcNumguestsRec, FmcTheta, FmcTrial, FmcWidomResult, FmcTraceRec,
+                     FmcWalker, FmcWalkerStats, FmcWindow, FmcWlParams, FmcWlStatus, TRACE_DTYPE, System, make_walker,
+                     prob_plots)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
@@ -28,7 +29,8 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
-           "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob")
+           "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob",
+           "fmc_set_numguests", "fmc_get_numguests")
 
 _lib = None
 
@@ -218,6 +220,18 @@ class Handle:
         _chk("fmc_get_windows", self.L.fmc_get_windows(self.h, C.c_int(w), None, C.c_int(0), C.byref(n)))
         buf = (FmcWindow * max(n.value, 1))()
         _chk("fmc_get_windows", self.L.fmc_get_windows(self.h, C.c_int(w), buf, C.c_int(n.value), C.byref(n)))
+        return buf, n.value
+
+    # ---- numguests.out trace
+    def set_numguests(self, nnumg: int, cap: int):
+        _chk("fmc_set_numguests", self.L.fmc_set_numguests(self.h, C.c_int(nnumg), C.c_int(cap)))
+
+    def numguests(self, w: int = 0):
+        """(records of walker w as an FmcNumguestsRec array, their count)."""
+        n = C.c_int()
+        _chk("fmc_get_numguests", self.L.fmc_get_numguests(self.h, C.c_int(w), None, C.c_int(0), C.byref(n)))
+        buf = (FmcNumguestsRec * max(n.value, 1))()
+        _chk("fmc_get_numguests", self.L.fmc_get_numguests(self.h, C.c_int(w), buf, C.c_int(n.value), C.byref(n)))
         return buf, n.value
 
     # ---- probability plots

@@ -657,6 +657,12 @@ class FmcWindow(C.Structure):
 FMC_MAX_PLOTS = 8
 
 
+class FmcNumguestsRec(C.Structure):
+    _fields_ = [("step", C.c_int64), ("nmols", C.c_int * FMC_MAX_SPECIES),
+                ("moves", (C.c_int * 6) * FMC_MAX_SPECIES), ("energy", C.c_double * FMC_MAX_SPECIES),
+                ("delE", C.c_double * FMC_MAX_SPECIES)]
+
+
 class FmcProbPlot(C.Structure):
     _fields_ = [("iguest", C.c_int), ("nsites", C.c_int), ("sites", C.c_int * (FMC_MAX_SITES + 1))]
 

@@ -128,7 +128,8 @@ typedef struct fmc_trace_rec {
 typedef struct fmc_walker_stats {
   int64_t steps, accepted[9], attempted[9];  /* index by move kind FMC_MOVE_* (1..8) */
   int nmols[FMC_MAX_SPECIES];
-  int overflow;                              /* bit 0: guest slots full; bit 1: window buffer full */
+  int overflow;                              /* bit 0: guest slots full; bit 1: window buffer full;
+                                                bit 2: numguests records full */
   double energy[FMC_MAX_SPECIES];            /* cumulative per-species energy (FP64) */
   double sum_n[FMC_MAX_SPECIES];             /* sum of N over counted steps */
   double sum_n2[FMC_MAX_SPECIES];
@@ -364,6 +365,26 @@ int fmc_stats_from_windows(const fmc_window *win, int nwin, int ntpguest, double
 /* fmc_get_windows + fmc_stats_from_windows at walker w's temperature;
  * out[ntpguest]. */
 int fmc_production_stats(fmc_handle *h, int w, fmc_guest_stats *out);
+
+/* ---- numguests.out (gcmc.f:1138-1163) ---------------------------------- */
+
+/* One record of the guest-count trace, written after every step whose total
+ * step count (gcmccount) is a multiple of nnumg: per guest type nummols,
+ * energy, delE (zero after a rejected move) and the step's move flags ins,
+ * del, dis (1 displace, 2 translate, 3 rotate), jmp, swp, swi as the
+ * reference writes them to numguests.out (format (i9,i7,2f20.6,7i4)). */
+typedef struct fmc_numguests_rec {
+  int64_t step;
+  int nmols[FMC_MAX_SPECIES];
+  int moves[FMC_MAX_SPECIES][6];
+  double energy[FMC_MAX_SPECIES];
+  double delE[FMC_MAX_SPECIES];
+} fmc_numguests_rec;
+
+/* Record every nnumg steps (0: off), up to cap records per walker (more set
+ * fmc_walker_stats.overflow bit 2).  Clears the records of every walker. */
+int fmc_set_numguests(fmc_handle *h, int nnumg, int cap);
+int fmc_get_numguests(fmc_handle *h, int w, fmc_numguests_rec *out, int cap, int *n);
 
 /* ---- probability plots of guest positions (storeprob, utility_pack.f:622-687) */
 

@@ -155,6 +155,10 @@ struct orc_state {
   double rucell[10];
   fmc_prob_plot plots[FMC_MAX_PLOTS];
   double *grid;
+  /* numguests.out records (gcmc.f:1138-1163) every nnumg steps */
+  long long gcmccount;
+  int nnumg, numg_n, numg_cap, swap_jguest;
+  fmc_numguests_rec *numg;
   /* pending trial (for accept/reject) */
   int pend_kind, pend_iguest, pend_choice, pend_valid;
   double pend_delrc, pend_engsictmp, pend_chgtmp;
@@ -1653,6 +1657,7 @@ void orc_destroy(orc_state *s) {
   if (!s) return;
   free(s->win);
   free(s->grid);
+  free(s->numg);
   free(s->erc); free(s->vvv); free(s->lstvdw); free(s->ltpvdw); free(s->prm_eps); free(s->prm_sig);
   free(s->steadd); free(s->kv);
   for (int m = 0; m <= FMC_MAX_SPECIES; ++m) {
@@ -1790,6 +1795,7 @@ static int swap_move(orc_state *s, int iguest, fmc_trial *tr, int *choice_out, d
   double comx, comy, comz;
   com(s, natms, imol, s->newx, s->newy, s->newz, &comx, &comy, &comz);
   const int jguest = chosen[(int)floor(mzt_next(&s->rng) * (s->ntpguest - 1) + 1)];
+  s->swap_jguest = jguest;
   const int jmol = s->locguest[jguest], jnatms = s->numatoms[jmol];
   /* back-ups of both species' coordinates and S(k) rows */
   const int ni = natms * s->nummols[imol], nj = jnatms * s->nummols[jmol];
@@ -2113,6 +2119,19 @@ int orc_set_prob(orc_state *s, const int ngrid[3], const int gridfactor[3], int 
   return s->grid ? 0 : FMC_ERR_ALLOC;
 }
 
+int orc_set_numguests(orc_state *s, int nnumg) {
+  if (nnumg < 0) return FMC_ERR_ARG;
+  s->nnumg = nnumg;
+  s->numg_n = 0;
+  return 0;
+}
+
+int orc_get_numguests(const orc_state *s, fmc_numguests_rec *out, int cap, int *n) {
+  for (int i = 0; i < s->numg_n && i < cap; ++i) out[i] = s->numg[i];
+  if (n) *n = s->numg_n;
+  return 0;
+}
+
 int orc_get_prob(const orc_state *s, int plot, double *out) {
   if (plot < 0 || plot >= s->nplots) return FMC_ERR_ARG;
   const double *g = s->grid + (size_t)plot * (size_t)(s->gridsize + 1);
@@ -2391,6 +2410,32 @@ static int run_chain(orc_state *s, int nsteps, fmc_trace_rec *trace, int flags) 
     if (flags & FMC_STATS_COUNT) {
       stats_step(s, accepted, (flags & FMC_STATS_CLOSE) && step == nsteps);
       if (s->nplots > 0) storeprob(s);
+    }
+    /* numguests.out: nummols, energy, delE and the step's move flags
+     * (ins, del, dis 1/2/3, jmp, swp, swi) per guest, gcmc.f:1138-1163 */
+    s->gcmccount += 1;
+    if (s->nnumg > 0 && s->gcmccount % s->nnumg == 0) {
+      if (s->numg_n == s->numg_cap) {
+        s->numg_cap = s->numg_cap * 2 + 16;
+        s->numg = (fmc_numguests_rec *)realloc(s->numg, sizeof(fmc_numguests_rec) * (size_t)s->numg_cap);
+      }
+      fmc_numguests_rec *q = &s->numg[s->numg_n++];
+      memset(q, 0, sizeof *q);
+      q->step = s->gcmccount;
+      for (int g = 1; g <= s->ntpguest && g <= FMC_MAX_SPECIES; ++g) {
+        int m = s->locguest[g];
+        q->nmols[g - 1] = s->nummols[m];
+        q->energy[g - 1] = s->energy[m];
+        q->delE[g - 1] = s->delE[m];
+        if (g == iguest) {
+          q->moves[g - 1][0] = kind == FMC_MOVE_INSERT;
+          q->moves[g - 1][1] = kind == FMC_MOVE_DELETE;
+          q->moves[g - 1][2] = kind == FMC_MOVE_DISPLACE ? 1 : kind == FMC_MOVE_TRANSLATE ? 2
+                               : kind == FMC_MOVE_ROTATE ? 3 : 0;
+          q->moves[g - 1][3] = kind == FMC_MOVE_JUMP;
+        }
+        q->moves[g - 1][4] = kind == FMC_MOVE_SWAP && (g == iguest || g == s->swap_jguest);
+      }
     }
   }
   return 0;

@@ -33,6 +33,9 @@ int orc_get_windows(const orc_state *s, fmc_window *out, int cap, int *nwin, lon
  * orc_get_prob writes plot p's grid with bin b at out[b-1] */
 int orc_set_prob(orc_state *s, const int ngrid[3], const int gridfactor[3], int nplots, const fmc_prob_plot *plots);
 int orc_get_prob(const orc_state *s, int plot, double *out);
+/* numguests.out records every nnumg steps of the chain (gcmc.f:1138-1163) */
+int orc_set_numguests(orc_state *s, int nnumg);
+int orc_get_numguests(const orc_state *s, fmc_numguests_rec *out, int cap, int *n);
 /* surfacemols by guest type */
 int orc_get_surface(const orc_state *s, int *surf);
 /* attempts / acceptances by move kind (index FMC_MOVE_*), summed over guests */

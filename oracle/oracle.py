@@ -11,7 +11,8 @@ import subprocess
 
 import numpy as np
 
-from fastmc_amd.system import (CSystem, FmcTrial, FmcTraceRec, FmcWindow, TRACE_DTYPE, make_walker, prob_plots)
+from fastmc_amd.system import (CSystem, FmcNumguestsRec, FmcTrial, FmcTraceRec, FmcWindow, TRACE_DTYPE, make_walker,
+                               prob_plots)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
@@ -36,7 +37,8 @@ def lib():
                      "orc_trial_displace", "orc_accept", "orc_reject", "orc_run", "orc_get_guests",
                      "orc_get_sk", "orc_get_table", "orc_get_kvec", "orc_get_energy",
                      "orc_full_resum", "orc_wl_setup", "orc_wl_run", "orc_wl_get", "orc_run_stats",
-                     "orc_set_sampling", "orc_get_windows", "orc_get_surface", "orc_set_prob", "orc_get_prob"):
+                     "orc_set_sampling", "orc_get_windows", "orc_get_surface", "orc_set_prob", "orc_get_prob", "orc_set_numguests",
+                     "orc_get_numguests"):
             getattr(L, name).restype = C.c_int
         L.orc_duni_seq.restype = None
         L.orc_wl_setup.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -148,6 +150,17 @@ class Oracle:
         out = np.zeros(int(np.prod(self._ngrid)))
         lib().orc_get_prob(C.c_void_p(self.h), C.c_int(plot), out.ctypes.data_as(C.c_void_p))
         return out
+
+    def set_numguests(self, nnumg):
+        if lib().orc_set_numguests(C.c_void_p(self.h), C.c_int(nnumg)):
+            raise RuntimeError("orc_set_numguests")
+
+    def numguests(self):
+        n = C.c_int()
+        lib().orc_get_numguests(C.c_void_p(self.h), None, C.c_int(0), C.byref(n))
+        buf = (FmcNumguestsRec * max(n.value, 1))()
+        lib().orc_get_numguests(C.c_void_p(self.h), buf, C.c_int(n.value), C.byref(n))
+        return buf, n.value
 
     def surface(self):
         out = np.zeros(8, dtype=np.int32)
