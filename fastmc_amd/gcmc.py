@@ -323,13 +323,49 @@ class _Engine:
         return int(self.h.stats(0).prodcount)
 
 
+def widom_report(names, henry, counted: int, attempts: int, seconds: float) -> list:
+    """The Widom run's report (gcmc.f:1313-1321, 1458-1466, 1545-1550):
+    Henry's constant per guest (E20.9 in the node report, E15.6 in the
+    final one), the attempt count as the step count, the non-overlapping
+    attempts as accepted and counted steps."""
+    out = ["", "*" * 100, "   Data reported from node   0", "*" * 100, ""]
+    for i, (nm, kh) in enumerate(zip(names, henry), start=1):
+        out += [f"     guest {i:2d}: {nm}", "", "     " + "Henry's Constant (mol /kg /bar): ".rjust(60) + _e(kh, 20, 9),
+                ""]
+    out += ["", f"Time elapsed for {attempts:9d} gcmc steps : {seconds:13.3f} seconds"]
+    out += ["", "Total accepted steps : ".rjust(30) + f"{counted:9d}"]
+    for i, (nm, kh) in enumerate(zip(names, henry), start=1):
+        out += ["", "=" * 100, f"final stats for guest {i:2d}   {nm}", "=" * 100, "", ""]
+        out += ["Henry's Constant (mol /kg /bar): ".rjust(60) + _e(kh, 15, 6),
+                "Total steps counted: ".rjust(60) + f"{counted:15d}", ""]
+    return out
+
+
+def run_widom(path: str, sysm: System, h, seed, out_name: str):
+    """The 'henry' run: widom_grid for every guest type (gcmc.f:605-635,
+    2454-2548) on the device, then its report."""
+    names = [species_name(s) for s in sysm.guests]
+    t0 = time.perf_counter()
+    res = [h.widom(g, sysm.nwidstep, sysm.griddim, sysm.gridfactor) for g in range(1, len(sysm.guests) + 1)]
+    seconds = time.perf_counter() - t0
+    counted = sum(int(r["accepted"]) for r in res)
+    lines = energy_report([0.0] * len(names), names, *h.single_point()[[3, 0]], seed)
+    lines += widom_report(names, [r["henry"] for r in res], counted, int(res[-1]["attempts"]), seconds)
+    with open(os.path.join(path, out_name), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return res
+
+
 def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78), out_name: str = "OUTPUT",
                   max_windows: int = 256):
-    """One GCMC run of the FIELD / CONFIG / CONTROL in `path` on the device;
-    writes the report to path/out_name and returns (Run, statistics)."""
+    """One GCMC run (or, with 'henry' in CONTROL, one Widom grid run) of the
+    FIELD / CONFIG / CONTROL in `path` on the device; writes the report to
+    path/out_name and returns (Run, statistics), or the Widom results."""
     sysm: System = read_system(path)
     h = L.Handle(sysm, precision=precision, seeds=[tuple(seed)])
     try:
+        if sysm.widom:
+            return run_widom(path, sysm, h, seed, out_name)
         sp = h.single_point()
         nwind, nwindsteps = window_steps(sysm.steps, sysm.nwind)
         minchk = check_interval(sysm.eqsteps, sysm.nnumg)

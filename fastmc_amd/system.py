@@ -194,6 +194,8 @@ class System:
     gridfactor: tuple = (1, 1, 1)   # 'grid factor': unit cells per supercell axis
     prob_plots: list = field(default_factory=list)  # per guest: site lists (0 = COM) of 'probability'
     prob_energy: list = field(default_factory=list) # per guest: an 'e' (energy) plot was requested
+    widom: bool = False             # 'henry': Widom insertions on the grid instead of GCMC
+    nwidstep: int = 0               # 'widom insertions': orientations per grid point
     pressures: list = field(default_factory=list)   # Pa per guest
     moves: list = field(default_factory=list)       # per guest dict key -> fraction
     guest_delr: list = field(default_factory=list)
@@ -431,7 +433,8 @@ def read_control(path: str, nguests: int):
     out = {"temperature": None, "rcut": None, "delr": None, "eps": None, "alpha": None,
            "kmax": None, "rvdw": None, "overlap": 1.5, "surftol": 0.0, "steps": 1, "eqsteps": 0,
            "nwind": 0, "accsample": False, "nnumg": 0, "griddim": 0.0, "gridfactor": (1, 1, 1),
-           "prob_plots": [[] for _ in range(nguests)], "prob_energy": [False] * nguests,
+           "prob_plots": [[] for _ in range(nguests)], "prob_energy": [False] * nguests, "widom": False,
+           "nwidstep": 0,
            "pressures": [-1.0] * nguests, "moves": [dict() for _ in range(nguests)],
            "guest_delr": [None] * nguests, "topmoves": {}}
     i = 1
@@ -495,6 +498,10 @@ def read_control(path: str, nguests: int):
                 out["gridfactor"] = (a, b, c)
         elif "averaging window" in rec:
             out["nwind"], _ = intstr(rec)           # utility_pack.f:394-396
+        elif "henry" in rec:
+            out["widom"] = True                     # utility_pack.f:389-390
+        elif "widom ins" in rec:
+            out["nwidstep"], _ = intstr(rec)        # readinputs.f:598-599
         elif "sampling" in rec:
             out["accsample"] = True                 # readinputs.f:608-613
         elif "numg" in rec:
@@ -591,7 +598,7 @@ def read_system(dirpath: str) -> System:
                   surftol=ctl["surftol"], steps=ctl["steps"], eqsteps=ctl["eqsteps"],
                   nwind=ctl["nwind"], accsample=ctl["accsample"], nnumg=ctl["nnumg"],
                   griddim=ctl["griddim"], gridfactor=ctl["gridfactor"], prob_plots=ctl["prob_plots"],
-                  prob_energy=ctl["prob_energy"],
+                  prob_energy=ctl["prob_energy"], widom=ctl["widom"], nwidstep=ctl["nwidstep"],
                   pressures=ctl["pressures"], moves=moves, guest_delr=gdelr)
 
 

@@ -78,3 +78,36 @@ def test_gpu_traced_run_keeps_gradient_workspace():
     for a, b in zip(first, again):
         np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
     h.close()
+
+
+@pytest.mark.parametrize("case", REF, ids=[c["name"] for c in REF])
+def test_gpu_run_directory_widom_mode(case, tmp_path):
+    """CONTROL with the reference's 'henry' directives through the run driver:
+    the OUTPUT it writes carries each guest's Henry's constant as the
+    reference printed it (final report, E15.6) and its counted steps."""
+    import re
+    import shutil
+    from fastmc_amd import gcmc
+    from fastmc_amd import inputs as I
+    from _helpers import GOLDEN
+    d = str(tmp_path)
+    if case["name"] == "calf20_co2n2":
+        for f in ("FIELD", "CONFIG"):
+            shutil.copy(os.path.join(GOLDEN, "calf20", f), os.path.join(d, f))
+        ctl = json.load(open(os.path.join(GOLDEN, "moves_ref.json")))[0]["control"]   # CALF-20 at 293 K
+    else:
+        I.config0(d, pressure_bar=1.0)
+        ctl = open(os.path.join(d, "CONTROL")).read()
+    ctl = ctl.replace("finish", f"henry\nwidom insertions {case['insertions_per_point']}\n"
+                                f"grid spacing {case['grid_spacing']}\nfinish")
+    open(os.path.join(d, "CONTROL"), "w").write(ctl)
+    res = gcmc.run_directory(d, seed=tuple(case["seeds"]))
+    out = open(os.path.join(d, "OUTPUT")).read()
+    final = out[out.index("final stats"):]
+    printed = [float(x) for x in re.findall(r"Henry's Constant \(mol /kg /bar\):\s+([-0-9.E+]+)", final)]
+    counted = [int(x) for x in re.findall(r"Total steps counted:\s+(\d+)", final)]
+    assert len(printed) == len(case["henry"])
+    for ours, ref in zip(printed, case["henry"]):
+        assert abs(ours - ref) <= 0.5 * 10.0 ** (math.floor(math.log10(abs(ref))) - 5) * 1.001, (ours, ref)
+    assert all(c == case["counted"][0] for c in counted)
+    assert [r["henry"] for r in res] == pytest.approx(case["henry"], rel=1e-8)
