@@ -1538,12 +1538,12 @@ int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, d
 // 605-610, attempts in the reference's loop order, orientations from the
 // walker's duni sequence (which advances by 4 draws per attempt).
 int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
-              double *estep_out, int *overlap_out, fmc_widom_result *res) {
+              double *estep_out, int *overlap_out, double *emap_out, double *cmap_out, fmc_widom_result *res) {
   if (!h || !res || w < 0 || w >= h->nwalkers || iguest < 1 || iguest > h->hp.ntpguest || nstep < 1 ||
       !(griddim > 0.0))
     return FMC_ERR_ARG;
   const int gf[3] = {gridfactor ? gridfactor[0] : 1, gridfactor ? gridfactor[1] : 1, gridfactor ? gridfactor[2] : 1};
-  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1) return FMC_ERR_ARG;
+  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1 || (!emap_out) != (!cmap_out)) return FMC_ERR_ARG;
   const DevParams &P = h->hp;
   double celprp[10];
   dcell(P.cell, celprp);
@@ -1559,8 +1559,9 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
   const long long chunk = std::min<long long>(total, 1LL << 22);
   const int nblocks = (int)std::min<long long>((chunk + WIDOM_WPB - 1) / WIDOM_WPB, 4096);
   float *duni = nullptr;
-  double *dest = nullptr, *dbsum = nullptr;
+  double *dest = nullptr, *dbsum = nullptr, *demap = nullptr, *dcmap = nullptr;
   int *dovl = nullptr;
+  const size_t mapsize = (size_t)ng[0] * ng[1] * ng[2];
   long long *dbcnt = nullptr;
   int rc = FMC_OK;
   auto fail = [&](int code) {
@@ -1571,8 +1572,16 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
       hipMalloc(&dbsum, sizeof(double) * nblocks) != hipSuccess ||
       hipMalloc(&dbcnt, sizeof(long long) * nblocks) != hipSuccess ||
       (estep_out && hipMalloc(&dest, sizeof(double) * (size_t)chunk) != hipSuccess) ||
-      (overlap_out && hipMalloc(&dovl, sizeof(int) * (size_t)chunk) != hipSuccess))
+      (overlap_out && hipMalloc(&dovl, sizeof(int) * (size_t)chunk) != hipSuccess) ||
+      (emap_out && (hipMalloc(&demap, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMalloc(&dcmap, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMemset(demap, 0, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMemset(dcmap, 0, sizeof(double) * mapsize) != hipSuccess)))
     fail(FMC_ERR_ALLOC);
+  // the unit cell the map folds into (gcmc.f:299-306)
+  double ucell[9], rucell[9];
+  for (int i = 0; i < 9; ++i) ucell[i] = P.cell[i] / (double)gf[i / 3];
+  invert(ucell, rucell);
   const WidomLds L = widom_lds(P);
   const bool sites3 = h->nsite == 3;
   if (rc == FMC_OK) {
@@ -1607,6 +1616,9 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
     A.bsum = dbsum;
     A.bcnt = dbcnt;
     A.kboltz = KBOLTZ;
+    A.emap = demap;
+    A.cmap = dcmap;
+    for (int i = 0; i < 9; ++i) A.rucell[i] = rucell[i];
     const int nb = (int)std::min<long long>((cnt + WIDOM_WPB - 1) / WIDOM_WPB, nblocks);
     if (h->precision == FMC_FP64) {
       if (sites3) hipLaunchKernelGGL((k_widom<double, 3>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
@@ -1631,7 +1643,11 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
       accepted += bcnt[b];
     }
   }
-  for (void *p : {(void *)duni, (void *)dest, (void *)dovl, (void *)dbsum, (void *)dbcnt})
+  if (rc == FMC_OK && emap_out &&
+      (hipMemcpy(emap_out, demap, sizeof(double) * mapsize, hipMemcpyDeviceToHost) != hipSuccess ||
+       hipMemcpy(cmap_out, dcmap, sizeof(double) * mapsize, hipMemcpyDeviceToHost) != hipSuccess))
+    fail(FMC_ERR_HIP);
+  for (void *p : {(void *)duni, (void *)dest, (void *)dovl, (void *)dbsum, (void *)dbcnt, (void *)demap, (void *)dcmap})
     if (p) hipFree(p);
   if (rc != FMC_OK) return rc;
   res->attempts = total;

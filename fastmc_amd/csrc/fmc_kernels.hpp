@@ -1827,9 +1827,11 @@ __device__ inline void surf_displaced(Smem<NS> &sh) {
 // axis split by the position's offset from the nearest point.  The b axis
 // keeps the reference's offset of 1 (not 0) for its upper neighbour past the
 // last plane.  Bins outside the grid (only reachable with nb = 1) are
-// dropped.  Hardware FP64 atomics: the order of the additions is free.
-__device__ inline void equitable_bin(const DevParams &P, double *grid, double fa, double fb, double fc) {
-  const int na = P.pgrid[0], nb = P.pgrid[1], nc = P.pgrid[2];
+// dropped.  Each point receives its weight times val.  Hardware FP64
+// atomics: the order of the additions is free.
+__device__ inline void equitable_bin(int na, int nb, int nc, double *grid, double fa, double fb, double fc,
+                                     double val) {
+  const int gsize = na * nb * nc;
   const double gpta = na * fa, gptb = nb * fb, gptc = nc * fc;
   const int aidx = (int)ceil(gpta), bidx = (int)ceil(gptb), cidx = (int)ceil(gptc);
   const double apart = gpta - floor(gpta), bpart = gptb - floor(gptb), cpart = gptc - floor(gptc);
@@ -1874,7 +1876,7 @@ __device__ inline void equitable_bin(const DevParams &P, double *grid, double fa
                          ar * br * cl, ar * bl * cr, al * br * cr, ar * br * cr};
 #pragma unroll
   for (int i = 0; i < 8; ++i)
-    if (bins[i] >= 1 && bins[i] <= P.pgridsize) unsafeAtomicAdd(grid + bins[i] - 1, wts[i]);
+    if (bins[i] >= 1 && bins[i] <= gsize) unsafeAtomicAdd(grid + bins[i] - 1, wts[i] * val);
 }
 
 // storeprob (utility_pack.f:622-687) for one production step: every guest
@@ -1916,11 +1918,11 @@ __device__ inline void prob_step(const DevParams &P, const DevWalkers &W, int w,
         for (int e = 0; e < P.plot_ns[p]; ++e) {
           const int site = P.plot_site[p][e];
           if (site == 0) {
-            equitable_bin(P, grid, ca, cb, cc);
+            equitable_bin(P.pgrid[0], P.pgrid[1], P.pgrid[2], grid, ca, cb, cc, 1.0);
           } else if (site <= na) {
             double a, b, c;
             frac(G.x[base + site - 1], G.y[base + site - 1], G.z[base + site - 1], a, b, c);
-            equitable_bin(P, grid, a, b, c);
+            equitable_bin(P.pgrid[0], P.pgrid[1], P.pgrid[2], grid, a, b, c, 1.0);
           }
         }
       }

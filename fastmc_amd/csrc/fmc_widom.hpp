@@ -20,6 +20,11 @@ struct WidomArgs {
   double *bsum;           // [gridDim] sum of exp(-estep / (kboltz T)) over non-overlapping attempts
   long long *bcnt;        // [gridDim] non-overlapping attempts
   double kboltz;         // the reference's exp(-1 * estep / kboltz / temp), temp the walker's
+  // energy map (storeenprob, utility_pack.f:550-620): estep and 1 of every
+  // non-overlapping attempt binned at the molecule's centre of mass, in the
+  // unit cell's fractional coordinates, on the ng grid; null: no map
+  double *emap, *cmap;
+  double rucell[9];
 };
 
 // n consecutive duni draws of walker w into out, advancing the walker's
@@ -294,6 +299,25 @@ __global__ void __launch_bounds__(256) k_widom(const DevParams *__restrict__ Pp,
         estep = (ewld1eng + er - c.ewld3 + ev + c.delrc) * c.reu;
         hsum += exp(-1.0 * estep / A.kboltz / ws.temp);
         hcnt += 1;
+        if (A.emap) {
+          double wsum = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+          for (int s = 0; s < natms; ++s) {
+            const double m = P.site_m[mol][s];
+            wsum += m;
+            mx += my_xyz[s * 3 + 0] * m;
+            my += my_xyz[s * 3 + 1] * m;
+            mz += my_xyz[s * 3 + 2] * m;
+          }
+          mx /= wsum;
+          my /= wsum;
+          mz /= wsum;
+          const double *r = A.rucell;
+          const double sx = mx * r[0] + my * r[3] + mz * r[6], sy = mx * r[1] + my * r[4] + mz * r[7],
+                       sz = mx * r[2] + my * r[5] + mz * r[8];
+          const double fa = sx - floor(sx), fb = sy - floor(sy), fc = sz - floor(sz);
+          equitable_bin(A.ng[0], A.ng[1], A.ng[2], A.emap, fa, fb, fc, estep);
+          equitable_bin(A.ng[0], A.ng[1], A.ng[2], A.cmap, fa, fb, fc, 1.0);
+        }
       }
       if (A.estep) A.estep[a] = estep;
       if (A.overlap) A.overlap[a] = overlap;

@@ -206,6 +206,19 @@ def cube_text(sysm, grid, ngrid, gridfactor, steps: int) -> str:
     """writeprob (utility_pack.f:773-871): a Gaussian cube file of one plot,
     the grid divided by the voxel volume (bohr^3) and the production step
     count; the framework atoms of one unit cell as the molecule list."""
+    return _cube(sysm, ngrid, gridfactor, ("probability cube file", "outer loop a, middle loop     b, inner loop c"),
+                 lambda vol, j: float(grid[j]) / vol / float(steps))
+
+
+def enprob_text(sysm, emap, cmap, ngrid, gridfactor) -> str:
+    """writeenprob (utility_pack.f:689-771): the energy plot, binned energy
+    over binned count per grid point (0 where nothing was binned)."""
+    return _cube(sysm, ngrid, gridfactor, ("Energy histogram cube file",
+                                           "outer loop a,            middle loop b, inner loop c"),
+                 lambda vol, j: float(emap[j]) / float(cmap[j]) if cmap[j] > 0.0 else 0.0)
+
+
+def _cube(sysm, ngrid, gridfactor, title, value) -> str:
     na, nb, nc = ngrid
     g = [0.0] + [float(sysm.cell.ravel()[i]) / float(gridfactor[i // 3]) * ANGS2BOHR for i in range(9)]
     vol = ((g[2] * g[6] - g[3] * g[5]) / (na * nb) * g[7] / nc + (g[3] * g[4] - g[1] * g[6]) / (na * nb) * g[8] / nc
@@ -215,8 +228,7 @@ def cube_text(sysm, grid, ngrid, gridfactor, steps: int) -> str:
     nfram = 0
     for sp in fw:
         nfram = int(nfram + sp.nmols / float(scell) * len(sp.labels))
-    lines = ["probability cube file", "outer loop a, middle loop     b, inner loop c",
-             f"{nfram:6d}" + "".join(_f(0.0, 12, 6) for _ in range(3))]
+    lines = [title[0], title[1], f"{nfram:6d}" + "".join(_f(0.0, 12, 6) for _ in range(3))]
     for n, off in ((na, 1), (nb, 4), (nc, 7)):
         lines.append(f"{n:6d}" + "".join(_f(g[off + k] / n, 12, 6) for k in range(3)))
     for sp in fw:
@@ -229,7 +241,7 @@ def cube_text(sysm, grid, ngrid, gridfactor, steps: int) -> str:
     row, ip = [], 0
     for j in range(1, na * nb * nc + 1):
         ip += 1
-        row.append(_e(float(grid[j - 1]) / vol / float(steps), 15, 6))
+        row.append(_e(value(vol, j - 1), 15, 6))
         if j % nc == 0 or ip == 6:
             lines.append("".join(row))
             row, ip = [], 0
@@ -346,8 +358,15 @@ def run_widom(path: str, sysm: System, h, seed, out_name: str):
     2454-2548) on the device, then its report."""
     names = [species_name(s) for s in sysm.guests]
     t0 = time.perf_counter()
-    res = [h.widom(g, sysm.nwidstep, sysm.griddim, sysm.gridfactor) for g in range(1, len(sysm.guests) + 1)]
+    res = [h.widom(g, sysm.nwidstep, sysm.griddim, sysm.gridfactor, energy_map=sysm.prob_energy[g - 1])
+           for g in range(1, len(sysm.guests) + 1)]
     seconds = time.perf_counter() - t0
+    # the energy plots of the Widom run (writeenprob after widom_grid)
+    for g, r in enumerate(res, start=1):
+        if sysm.prob_energy[g - 1] and sum(int(x["accepted"]) for x in res) > 0:
+            ng = prob_grid_size(sysm.cell, sysm.griddim, sysm.gridfactor)
+            with open(os.path.join(path, f"Ehist_guest{g:02d}.cube"), "w") as fh:
+                fh.write(enprob_text(sysm, r["emap"], r["cmap"], ng, sysm.gridfactor))
     counted = sum(int(r["accepted"]) for r in res)
     lines = energy_report([0.0] * len(names), names, *h.single_point()[[3, 0]], seed)
     lines += widom_report(names, [r["henry"] for r in res], counted, int(res[-1]["attempts"]), seconds)
@@ -375,8 +394,6 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
             for k, sites in enumerate(pl, start=1):
                 plots.append((g, sites))
                 cubes.append(f"prob_guest{g:02d}_prob_{k:02d}.cube")
-        if any(sysm.prob_energy):
-            sys.stderr.write("fastmc: energy probability plots ('e') are not provided; skipped\n")
         ngrid = h.set_prob(sysm.griddim, sysm.gridfactor, plots) if plots else None
         # numguests.out every nnumg steps (readinputs.f:713-717,910-916)
         nnumg = sysm.nnumg or (sysm.eqsteps // 1000 if sysm.eqsteps > 1000 else 1)
@@ -408,10 +425,19 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
                 fh.write(numguests_text(recs, nrec, g))
         if st.overflow & 4:
             sys.stderr.write("fastmc: numguests records truncated\n")
-        if plots and st.prodcount > 0:
+        if (plots or any(sysm.prob_energy)) and st.prodcount > 0:
             for p, name in enumerate(cubes):
                 with open(os.path.join(path, name), "w") as fh:
                     fh.write(cube_text(sysm, h.prob(p), ngrid, sysm.gridfactor, int(st.prodcount)))
+            # The energy plot of a GCMC run: the reference bins the main
+            # program's guest_toten, which stays 0 (mc_displace sets a local
+            # of the same name, gcmc.f:1840,1904), so every value is 0.
+            eg = prob_grid_size(sysm.cell, sysm.griddim, sysm.gridfactor)
+            zero = [0.0] * (eg[0] * eg[1] * eg[2])
+            for g, on in enumerate(sysm.prob_energy, start=1):
+                if on:
+                    with open(os.path.join(path, f"Ehist_guest{g:02d}.cube"), "w") as fh:
+                        fh.write(enprob_text(sysm, zero, zero, eg, sysm.gridfactor))
         return r, stats
     finally:
         h.close()

@@ -285,25 +285,34 @@ class Handle:
 
     # ---- Wang-Landau (wang_landau.f)
     def widom(self, iguest: int, nstep: int, griddim: float, gridfactor=(1, 1, 1), w: int = 0,
-              per_attempt: bool = False):
+              per_attempt: bool = False, energy_map: bool = False):
         """Widom insertions on a grid (fmc_widom): dict of <exp(-beta estep)>, attempts, non-overlapping
         attempts, Henry's constant as the reference prints it, grid sizes; with per_attempt also the
-        estep and overlap arrays in the reference's loop order."""
+        estep and overlap arrays in the reference's loop order; with energy_map the binned estep and
+        count grids ("emap", "cmap") of the 'e' probability plot."""
         gf = (C.c_int * 3)(*gridfactor)
         res = FmcWidomResult()
-        est = ovl = None
+        est = ovl = emap = cmap = None
         if per_attempt:
             # size from the same grid rule, so the arrays can be handed over
             n = self._widom_attempts(iguest, nstep, griddim, gridfactor)
             est = np.zeros(n)
             ovl = np.zeros(n, dtype=np.int32)
+        if energy_map:
+            from .gcmc import prob_grid_size
+            npt = int(np.prod(prob_grid_size(self.sysm.cell, griddim, gridfactor)))
+            emap, cmap = np.zeros(npt), np.zeros(npt)
+
+        def ptr(a):
+            return a.ctypes.data_as(C.c_void_p) if a is not None else None
         _chk("fmc_widom", self.L.fmc_widom(self.h, C.c_int(w), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim),
-                                           gf, est.ctypes.data_as(C.c_void_p) if per_attempt else None,
-                                           ovl.ctypes.data_as(C.c_void_p) if per_attempt else None, C.byref(res)))
+                                           gf, ptr(est), ptr(ovl), ptr(emap), ptr(cmap), C.byref(res)))
         out = {"avg_boltz": res.avg_boltz, "attempts": res.attempts, "accepted": res.accepted, "henry": res.henry,
                "ngrid": tuple(res.ngrid), "wngrid": tuple(res.wngrid)}
         if per_attempt:
             out["estep"], out["overlap"] = est, ovl
+        if energy_map:
+            out["emap"], out["cmap"] = emap, cmap
         return out
 
     def _widom_attempts(self, iguest, nstep, griddim, gridfactor):

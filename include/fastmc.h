@@ -311,7 +311,12 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
  * divided by all attempts (chainstats(istat+7)); henry = the printed Henry's
  * constant avg_boltz / avo / boltz / T * 1e5 (gcmc.f:1319-1321); per attempt,
  * in the reference's loop order, estep (0 on overlap) and the overlap flag if
- * the arrays are given (attempts = ngrida*ngridb*ngridc*nstep entries). */
+ * the arrays are given (attempts = ngrida*ngridb*ngridc*nstep entries).
+ * emap_out / cmap_out (both or neither, [ngrida*ngridb*ngridc], c fastest):
+ * the energy map of the 'e' probability plot (storeenprob,
+ * utility_pack.f:550-620): every non-overlapping attempt's estep and 1
+ * spread by equitable_bin at the molecule's centre of mass in the unit
+ * cell (cell / gridfactor); writeenprob prints emap / cmap. */
 typedef struct fmc_widom_result {
   double avg_boltz, henry;
   long long attempts, accepted;  /* accepted: attempts without overlap */
@@ -319,7 +324,7 @@ typedef struct fmc_widom_result {
 } fmc_widom_result;
 
 int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
-              double *estep_out, int *overlap_out, fmc_widom_result *res);
+              double *estep_out, int *overlap_out, double *emap_out, double *cmap_out, fmc_widom_result *res);
 
 /* ---- production statistics (gcmc.f:1050-1130, 1148-1258, 1458-1557) ---- */
 

@@ -1972,10 +1972,10 @@ static void stats_step(orc_state *s, int accepted, int final_step) {
   }
 }
 
-/* equitable_bin, utility_pack.f:2407-2519 */
-static void equitable_bin(orc_state *s, double fposa, double fposb, double fposc, int subgrid, double val) {
-  const int ngrida = s->ngrida, ngridb = s->ngridb, ngridc = s->ngridc;
-  double *grid = s->grid + (size_t)subgrid * (size_t)(s->gridsize + 1);
+/* equitable_bin, utility_pack.f:2407-2519, into grid[1..ngrida*ngridb*ngridc] */
+static void ebin(int ngrida, int ngridb, int ngridc, double *grid, double fposa, double fposb, double fposc,
+                 double val) {
+  const int gridsize = ngrida * ngridb * ngridc;
   int alidx, aridx, blidx, bridx, clidx, cridx;
   double al, ar, bl, br, cl, cr;
   double gpta = ngrida * (fposa);
@@ -2055,7 +2055,12 @@ static void equitable_bin(orc_state *s, double fposa, double fposb, double fposc
   const double wts[8] = {(al * bl * cl) * val, (ar * bl * cl) * val, (al * br * cl) * val, (al * bl * cr) * val,
                          (ar * br * cl) * val, (ar * bl * cr) * val, (al * br * cr) * val, (ar * br * cr) * val};
   for (int i = 0; i < 8; ++i)
-    if (bins[i] >= 1 && bins[i] <= s->gridsize) grid[bins[i]] = grid[bins[i]] + wts[i];
+    if (bins[i] >= 1 && bins[i] <= gridsize) grid[bins[i]] = grid[bins[i]] + wts[i];
+}
+
+static void equitable_bin(orc_state *s, double fposa, double fposb, double fposc, int subgrid, double val) {
+  ebin(s->ngrida, s->ngridb, s->ngridc, s->grid + (size_t)subgrid * (size_t)(s->gridsize + 1), fposa, fposb, fposc,
+       val);
 }
 
 /* frac2 / frac, utility_pack.f:873-921: fractional coordinates in the unit
@@ -2765,7 +2770,7 @@ int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst
  * ngrida, ngridb, ngridc, [7..9] wngrida, wngridb, wngridc.  estep / ovl (may
  * be NULL): per attempt in loop order, estep 0 on overlap. */
 int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[3], double out[10], double *estep,
-              int *ovl) {
+              int *ovl, double *emap, double *cmap) {
   if (iguest < 1 || iguest > s->ntpguest || nstep < 1 || griddim <= 0.0) return FMC_ERR_ARG;
   double celprp[11];
   dcell(s->cell, celprp);
@@ -2779,6 +2784,15 @@ int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[
   const int mol = s->locguest[iguest], natms = s->numatoms[mol];
   double hsum = 0.0;
   long long widcount = 0, prodcount = 0;
+  /* energy map: storeenprob (utility_pack.f:550-620) into 1-based grids */
+  const int npt = ng[0] * ng[1] * ng[2];
+  double *eg = NULL, *cg = NULL, ucell[10], rucell[10];
+  if (emap) {
+    eg = (double *)calloc((size_t)npt + 1, sizeof(double));
+    cg = (double *)calloc((size_t)npt + 1, sizeof(double));
+    for (int i = 1; i <= 9; ++i) ucell[i] = s->cell[i] / (double)gf[(i - 1) / 3];
+    invert(ucell, rucell);
+  }
   fmc_trial tr;
   const double *cl = s->cell;
   for (int i = 1; i <= ng[0]; ++i)
@@ -2809,12 +2823,32 @@ int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[
           if (estep) estep[widcount] = ov ? 0.0 : tr.estep;
           if (ovl) ovl[widcount] = ov;
           widcount += 1;
+          if (!ov && eg) {
+            double comx, comy, comz;
+            com(s, natms, mol, s->newx, s->newy, s->newz, &comx, &comy, &comz);
+            double sx = comx * rucell[1] + comy * rucell[4] + comz * rucell[7];
+            double sy = comx * rucell[2] + comy * rucell[5] + comz * rucell[8];
+            double sz = comx * rucell[3] + comy * rucell[6] + comz * rucell[9];
+            sx = sx - floor(sx);
+            sy = sy - floor(sy);
+            sz = sz - floor(sz);
+            ebin(ng[0], ng[1], ng[2], eg, sx, sy, sz, tr.estep);
+            ebin(ng[0], ng[1], ng[2], cg, sx, sy, sz, 1.0);
+          }
           if (!ov) {
             prodcount += 1;
             hsum += exp(-1.0 * tr.estep / KBOLTZ / s->temp);
           }
         }
       }
+  if (eg) {
+    for (int b = 1; b <= npt; ++b) {
+      emap[b - 1] = eg[b];
+      cmap[b - 1] = cg[b];
+    }
+    free(eg);
+    free(cg);
+  }
   const double avg = hsum / (double)widcount;
   out[0] = avg;
   out[1] = (double)widcount;

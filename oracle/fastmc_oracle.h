@@ -49,7 +49,7 @@ int orc_wl_get(orc_state *s, double *lng, int *hist, long long *ist, double *dst
 
 /* Widom insertions on a grid (gcmc.f:2454-2548); out[10] as documented in the source. */
 int orc_widom(orc_state *s, int iguest, int nstep, double griddim, const int gf[3], double out[10], double *estep,
-              int *ovl);
+              int *ovl, double *emap, double *cmap);
 
 int orc_get_guests(orc_state *s, int iguest, double *xyz, int cap_mols, int *nmols);
 int orc_get_sk(orc_state *s, double *sk);

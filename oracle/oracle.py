@@ -81,7 +81,7 @@ class Oracle:
         except Exception:
             pass
 
-    def widom(self, iguest, nstep, griddim, gridfactor=(1, 1, 1), max_attempts=0):
+    def widom(self, iguest, nstep, griddim, gridfactor=(1, 1, 1), max_attempts=0, energy_map=False):
         """widom_grid (gcmc.f:2454-2548): dict of <exp(-beta estep)>, attempts,
         non-overlapping attempts, Henry's constant as printed, grid sizes; with
         max_attempts > 0 (at least the attempt count) also per-attempt estep and
@@ -90,9 +90,16 @@ class Oracle:
         gf = (C.c_int * 3)(*gridfactor)
         est = np.zeros(max(max_attempts, 1))
         ovl = np.zeros(max(max_attempts, 1), dtype=np.int32)
+        emap = cmap = None
+        if energy_map:
+            from fastmc_amd.gcmc import prob_grid_size
+            npt = int(np.prod(prob_grid_size(self.sysm.cell, griddim, gridfactor)))
+            emap, cmap = np.zeros(npt), np.zeros(npt)
         rc = lib().orc_widom(C.c_void_p(self.h), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim), gf,
                              out.ctypes.data_as(C.c_void_p), est.ctypes.data_as(C.c_void_p) if max_attempts else None,
-                             ovl.ctypes.data_as(C.c_void_p) if max_attempts else None)
+                             ovl.ctypes.data_as(C.c_void_p) if max_attempts else None,
+                             emap.ctypes.data_as(C.c_void_p) if energy_map else None,
+                             cmap.ctypes.data_as(C.c_void_p) if energy_map else None)
         if rc:
             raise RuntimeError(f"orc_widom: {rc}")
         r = {"avg_boltz": out[0], "attempts": int(out[1]), "accepted": int(out[2]), "henry": out[3],
@@ -100,6 +107,8 @@ class Oracle:
         if max_attempts:
             assert max_attempts >= r["attempts"]
             r["estep"], r["overlap"] = est[: r["attempts"]], ovl[: r["attempts"]]
+        if energy_map:
+            r["emap"], r["cmap"] = emap, cmap
         return r
 
     def move_counts(self):

@@ -34,12 +34,13 @@ EXE = os.path.join(REPO, "oracle", "_ref", "gcmc_ser.x")
 
 
 def add_prob(ctl, spacing, plots):
-    """plots: per guest a list of site lists (0 = centre of mass)."""
+    """plots: per guest a list of site lists (0 = centre of mass) or "e" (energy plot)."""
     ctl = ctl.replace("history 0\n", f"history 0\ngrid spacing {spacing}\n")
     blocks = ctl.split("&end")
     out = []
     for g, b in enumerate(blocks[:-1]):
-        lines = "".join(f"  {len(s)} " + " ".join(str(x) for x in s) + "\n" for s in plots[g])
+        lines = "".join("  e\n" if s == "e" else f"  {len(s)} " + " ".join(str(x) for x in s) + "\n"
+                        for s in plots[g])
         out.append(b + (f"  probability {len(plots[g])}\n" + lines if plots[g] else ""))
     out.append(blocks[-1])
     return "&end".join(out)
@@ -49,7 +50,16 @@ def write_cfg0(d):
     import make_stats_ref
     make_stats_ref.write_cfg0(d, 200, 1000)
     p = os.path.join(d, "CONTROL")
-    ctl = add_prob(open(p).read(), 2.0, [[[0], [1, 3]]])
+    ctl = add_prob(open(p).read(), 2.0, [[[0], [1, 3], "e"]])
+    open(p, "w").write(ctl)
+
+
+def write_cfg0_widom(d):
+    from fastmc_amd import inputs as I
+    I.config0(d, pressure_bar=1.0)
+    p = os.path.join(d, "CONTROL")
+    ctl = add_prob(open(p).read(), 2.0, [["e"]]).replace("history 0\ngrid spacing 2.0\n", "history 0\n")
+    ctl = ctl.replace("finish", "henry\nwidom insertions 3\ngrid spacing 2.0\nfinish")
     open(p, "w").write(ctl)
 
 
@@ -61,7 +71,7 @@ def write_calf20(d):
     open(p, "w").write(ctl)
 
 
-CASES = [("cfg0_prob", write_cfg0), ("calf20_prob", write_calf20)]
+CASES = [("cfg0_prob", write_cfg0), ("calf20_prob", write_calf20), ("cfg0_widom_map", write_cfg0_widom)]
 
 
 def run_case(name, writer):
@@ -71,11 +81,13 @@ def run_case(name, writer):
         control = open(os.path.join(d, "CONTROL")).read()
         subprocess.run([EXE], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=3600, check=False)
         out = open(os.path.join(d, "OUTPUT")).read()
-        cubes = {os.path.basename(p): open(p).read() for p in sorted(glob.glob(os.path.join(d, "prob_guest*.cube")))}
+        cubes = {os.path.basename(p): open(p).read()
+                 for p in sorted(glob.glob(os.path.join(d, "prob_guest*.cube")) + glob.glob(os.path.join(d, "Ehist*.cube")))}
     finally:
         shutil.rmtree(d, ignore_errors=True)
     seeds = [int(x) for x in re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]
-    total = int(re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out).group(1))
+    m = re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out)
+    total = int(m.group(1)) if m else 0
     m = re.search(r"Starting production at\s+(\d+)", out)
     start = int(m.group(1)) if m else 1
     grid = [int(x) for x in re.search(r"Probability grid size\s+:\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]

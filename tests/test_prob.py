@@ -1,5 +1,6 @@
 """Probability plots (storeprob / equitable_bin, utility_pack.f:622-687,
-2407-2519; the cube files of writeprob, 773-871) in the C oracle against the
+2407-2519; the cube files of writeprob, 773-871; the energy plot of
+storeenprob / writeenprob, 550-620, 689-771) in the C oracle against the
 reference's own cube files (tests/golden/prob_ref.json.gz, made by
 make_prob_ref.py): the oracle replays each reference run from its printed
 seeds through the library's main-loop driver, bins every production step,
@@ -7,7 +8,10 @@ and the library's cube writer (fastmc_amd.gcmc.cube_text) must reproduce
 each file as printed (E15.6 values, header, framework atoms).  The config-0
 framework's synthetic masses match no element of atmnumber's table, for
 which the reference returns an undefined value: there the atom lines are
-compared without their first field.  Also the grid size (gcmc.f:258-281)."""
+compared without their first field.  The energy plot of a GCMC run is all
+zeros (the reference bins a variable that stays 0); that of a Widom run
+('henry') holds the mean insertion energy per grid point, from the
+oracle's widom_grid with the map.  Also the grid size (gcmc.f:258-281)."""
 import gzip
 import json
 import os
@@ -78,16 +82,40 @@ def same_cube(ours, ref, atom_z=True):
             assert x == y, (i, x, y)
 
 
-@pytest.mark.parametrize("case", REF, ids=[c["name"] for c in REF])
+GCMC = [c for c in REF if not c["name"].endswith("widom_map")]
+WIDOM = [c for c in REF if c["name"].endswith("widom_map")]
+
+
+@pytest.mark.parametrize("case", GCMC, ids=[c["name"] for c in GCMC])
 def test_oracle_probability_plots_match_reference_cubes(case):
     sysm, ngrid, names, grids, r = replay_oracle(case)
     assert list(ngrid) == case["grid"]
     assert (r.start, r.steps) == (case["start"], case["total"])
-    assert sorted(names) == sorted(case["cubes"])
+    enames = [f"Ehist_guest{g:02d}.cube" for g, on in enumerate(sysm.prob_energy, start=1) if on]
+    assert sorted(names + enames) == sorted(case["cubes"])
     for name, grid in zip(names, grids):
         assert grid.sum() > 0
         text = gcmc.cube_text(sysm, grid, ngrid, sysm.gridfactor, r.prodcount)
         same_cube(text, case["cubes"][name], atom_z=case["name"].startswith("calf20"))
+    zero = [0.0] * (ngrid[0] * ngrid[1] * ngrid[2])
+    for name in enames:
+        same_cube(gcmc.enprob_text(sysm, zero, zero, ngrid, sysm.gridfactor), case["cubes"][name],
+                  atom_z=case["name"].startswith("calf20"))
+
+
+@pytest.mark.parametrize("case", WIDOM, ids=[c["name"] for c in WIDOM])
+def test_oracle_widom_energy_map_matches_reference_cube(case):
+    sysm = prob_system(case)
+    assert sysm.widom and sysm.prob_energy == [True]
+    ngrid = gcmc.prob_grid_size(sysm.cell, sysm.griddim, sysm.gridfactor)
+    assert list(ngrid) == case["grid"]
+    o = Oracle(sysm, seed=tuple(case["seeds"]))
+    r = o.widom(1, sysm.nwidstep, sysm.griddim, sysm.gridfactor, energy_map=True)
+    o.close()
+    assert r["attempts"] == case["total"]
+    assert (r["cmap"] > 0).all()
+    same_cube(gcmc.enprob_text(sysm, r["emap"], r["cmap"], ngrid, sysm.gridfactor), case["cubes"]["Ehist_guest01.cube"],
+              atom_z=False)
 
 
 def test_grid_size_and_atomic_numbers():
