@@ -68,12 +68,17 @@ class Run:
 
 
 def run_main_loop(engine, eqsteps: int, mcsteps: int, minchk: int, accepted_only: bool = False,
-                  prodcount=None) -> Run:
+                  prodcount=None, nwindsteps: int = 0) -> Run:
     """Drive `engine.run(nsteps, flags)` (flags as fmc_run's count_stats)
     through the reference's main-loop schedule.  With every production step
     counted the schedule is known up front (three launches at most); with
     accepted_only the run ends on the production count, which `prodcount()`
-    reads at every check (gcmc.f:707-752)."""
+    reads at every check (gcmc.f:707-752).  Negative step counts are cycles
+    (gcmc.f:700-800, run_cycles); the engine then also needs
+    `guests()`, the total guest count, and `nwindsteps` the window length
+    in cycles."""
+    if eqsteps < 0 or mcsteps < 0:
+        return run_cycles(engine, eqsteps, mcsteps, minchk, accepted_only, prodcount, nwindsteps)
     r = Run()
     COUNT, CLOSE = L.FMC_STATS_COUNT, L.FMC_STATS_CLOSE
     if not accepted_only:
@@ -85,8 +90,6 @@ def run_main_loop(engine, eqsteps: int, mcsteps: int, minchk: int, accepted_only
         engine.run(1, COUNT | CLOSE)
         r.steps, r.start, r.prodcount = last, s0, last - s0 + 1
         return r
-    if eqsteps < 0 or mcsteps <= 0:
-        raise NotImplementedError("cycle-based equilibration or production (negative steps) is not supported")
     production = eqsteps == 0
     if production:
         r.start = 1
@@ -107,6 +110,74 @@ def run_main_loop(engine, eqsteps: int, mcsteps: int, minchk: int, accepted_only
         g += minchk
     r.steps = g
     r.prodcount = prodcount()
+    return r
+
+
+CYCLE_MULT = 20   # gcmc.f:141
+
+
+def run_cycles(engine, eqsteps: int, mcsteps: int, minchk: int, accepted_only: bool, prodcount,
+               nwindsteps: int) -> Run:
+    """The main loop with equilibration and / or production counted in
+    cycles (gcmc.f:700-800): every step advances cycle_step; at each check
+    (every minchk steps) a cycle ends once cycle_step reaches maxcycle, which
+    then becomes max(guests, 20) (initially max(20 x initial guests, 1000));
+    equilibration ends after -eqsteps cycles, production after -mcsteps.
+    With production in cycles, windows close when the cycle count is a
+    multiple of nwindsteps at the check step that ended a cycle
+    (gcmc.f:1085-1092), and on the last step; the device's own windows by
+    production count must be off (nwindsteps 0 in fmc_set_sampling)."""
+    COUNT, CLOSE = L.FMC_STATS_COUNT, L.FMC_STATS_CLOSE
+    r = Run()
+    production = eqsteps == 0
+    if production:
+        r.start = 1
+    maxcycle = max(engine.guests() * CYCLE_MULT, 1000)
+    cycle_count, cycle_step, g = 1, 0, 0
+    counted = 0
+    while True:
+        # steps up to the next check, then the check at the top of its step
+        seg = minchk - 1 - (g % minchk)
+        if seg > 0:
+            engine.run(seg, COUNT if production else 0)
+            counted += seg if production else 0
+            g += seg
+            cycle_step += seg
+        g += 1
+        cycle_step += 1
+        last = False
+        if production:
+            if mcsteps < 0:
+                if cycle_step >= maxcycle:
+                    cycle_step = 0
+                    maxcycle = max(engine.guests(), CYCLE_MULT)
+                    cycle_count += 1
+                last = cycle_count >= -mcsteps
+            else:
+                last = (prodcount() if accepted_only else counted) >= mcsteps
+        elif eqsteps < 0:
+            if cycle_step >= maxcycle:
+                cycle_step = 0
+                maxcycle = max(engine.guests(), CYCLE_MULT)
+                cycle_count += 1
+            if cycle_count >= -eqsteps:
+                cycle_count, cycle_step = 1, 0
+                maxcycle = max(engine.guests(), CYCLE_MULT)
+                production = True
+                r.start = g
+        elif g >= eqsteps:
+            production = True
+            r.start = g
+        flags = COUNT if production else 0
+        if production and (last or (mcsteps < 0 and nwindsteps > 0 and cycle_count % nwindsteps == 0
+                                    and cycle_step == 0)):
+            flags |= CLOSE
+        engine.run(1, flags)
+        counted += 1 if production else 0
+        if last:
+            break
+    r.steps = g
+    r.prodcount = prodcount() if prodcount else counted
     return r
 
 
@@ -388,7 +459,8 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         sp = h.single_point()
         nwind, nwindsteps = window_steps(sysm.steps, sysm.nwind)
         minchk = check_interval(sysm.eqsteps, sysm.nnumg)
-        h.set_sampling(nwindsteps, sysm.accsample, max_windows)
+        cycles = sysm.steps < 0 or sysm.eqsteps < 0
+        h.set_sampling(0 if sysm.steps < 0 else nwindsteps, sysm.accsample, max_windows)
         plots, cubes = [], []
         for g, pl in enumerate(sysm.prob_plots, start=1):
             for k, sites in enumerate(pl, start=1):
@@ -397,11 +469,11 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
         ngrid = h.set_prob(sysm.griddim, sysm.gridfactor, plots) if plots else None
         # numguests.out every nnumg steps (readinputs.f:713-717,910-916)
         nnumg = sysm.nnumg or (sysm.eqsteps // 1000 if sysm.eqsteps > 1000 else 1)
-        last = schedule(sysm.eqsteps, sysm.steps, minchk)[1] * (4 if sysm.accsample else 1)
+        last = 2000000 if cycles else schedule(sysm.eqsteps, sysm.steps, minchk)[1] * (4 if sysm.accsample else 1)
         h.set_numguests(nnumg, last // nnumg + 1)
         eng = _Engine(h)
         t0 = time.perf_counter()
-        r = run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount)
+        r = run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount, nwindsteps)
         seconds = time.perf_counter() - t0
         st = h.stats(0)
         stats = h.production_stats(0)
@@ -410,9 +482,14 @@ def run_directory(path: str, precision: int = L.FMC_FP64, seed=(12, 34, 56, 78),
                  f"Each window will average over {nwindsteps:10d} steps (or cycles)."]
         ens = [float(st.energy[i]) for i, s in enumerate(sysm.species) if s.is_guest]
         lines += energy_report(ens, names, sp[3], sp[0], seed)
-        if r.start > 1:
+        if sysm.eqsteps < 0:
+            lines += [f"Completed at least {-sysm.eqsteps:10d} equilibration cycles for each guest",
+                      f"Starting production at{r.start:10d} over all nodes"]
+        elif r.start > 1:
             lines += [f"Completed at least {sysm.eqsteps:10d} equlibration steps",
                       f"Starting production at {r.start:10d}"]
+        if sysm.steps < 0:
+            lines += [f"Completed at least {-sysm.steps:10d} production cycles for each guest"]
         lines += node_report(names, stats, int(st.prodcount))
         lines += move_report(r.steps, seconds, list(st.attempted), list(st.accepted))
         lines += final_report(names, stats, int(st.prodcount), sysm.surftol)

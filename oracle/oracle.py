@@ -209,6 +209,16 @@ class Oracle:
                              C.c_int(4096), C.byref(n))
         return buf[: 3 * nat * n.value].reshape(-1, 3)
 
+    def guest_total(self):
+        """Molecules of every guest type together."""
+        n, tot = C.c_int(), 0
+        buf = np.zeros(3)
+        for g in range(1, len(self.sysm.guests) + 1):
+            lib().orc_get_guests(C.c_void_p(self.h), C.c_int(g), buf.ctypes.data_as(C.c_void_p), C.c_int(0),
+                                 C.byref(n))
+            tot += n.value
+        return tot
+
     def sk(self):
         out = np.zeros(2 * self.newld)
         lib().orc_get_sk(C.c_void_p(self.h), out.ctypes.data_as(C.c_void_p))

@@ -64,6 +64,9 @@ CASES = [
     ("calf20_moves6_stats", write_calf20_moves6, 1000, 4000, ""),
     ("cfg0_p1_stats", write_cfg0, 600, 3000, ""),
     ("cfg0_p1_sampling", write_cfg0, 400, 400, "sampling"),
+    # run lengths in cycles (negative step counts, gcmc.f:700-800)
+    ("cfg0_p1_cycles", write_cfg0, -2, -12, ""),
+    ("calf20_moves6_cycles", write_calf20_moves6, 800, -20, ""),
 ]
 
 NODE_KEYS = [("<N>", "<N>: "), ("<E>", "<E>: "), ("<E*N>", r"<E\*N>: "), ("<N*N>", r"<N\*N>: "),
@@ -101,7 +104,7 @@ def run_case(name, writer, eq, steps, extra):
         shutil.rmtree(d, ignore_errors=True)
     seeds = [int(x) for x in re.search(r"i,j,k,l values for duni\(\)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", out).groups()]
     total = int(re.search(r"Time elapsed for\s+(\d+)\s+gcmc steps", out).group(1))
-    m = re.search(r"Starting production at\s+(\d+)", out)
+    m = re.search(r"Starting production at\s*(\d+)", out)
     start = int(m.group(1)) if m else 1
     nwindsteps = int(re.search(r"Each window will average over\s+(\d+)", out).group(1))
     node = out[out.index("Data reported from node"):]

@@ -44,6 +44,10 @@ class DeviceEngine:
     def prodcount(self):
         return int(self.h.stats(0).prodcount)
 
+    def guests(self):
+        st = self.h.stats(0)
+        return int(sum(st.nmols[g] for g in range(len(self.h.sysm.guests))))
+
 
 @pytest.mark.parametrize("case", REF, ids=[c["name"] for c in REF])
 def test_gpu_statistics_match_reference_and_oracle(case):
@@ -56,18 +60,18 @@ def test_gpu_statistics_match_reference_and_oracle(case):
     minchk = gcmc.check_interval(sysm.eqsteps, sysm.nnumg)
     h = Handle(sysm, precision=FMC_FP64, seeds=[tuple(case["seeds"])])
     try:
-        h.set_sampling(nwindsteps, sysm.accsample, 64)
+        h.set_sampling(0 if sysm.steps < 0 else nwindsteps, sysm.accsample, 64)
         eng = DeviceEngine(h)
-        r = gcmc.run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount)
+        r = gcmc.run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount, nwindsteps)
         win, nw = h.windows(0)
         st = h.stats(0)
         stats = h.production_stats(0)
     finally:
         h.close()
     o = Oracle(sysm, seed=tuple(case["seeds"]))
-    o.set_sampling(nwindsteps, sysm.accsample)
+    o.set_sampling(0 if sysm.steps < 0 else nwindsteps, sysm.accsample)
     oe = OracleEngine(o)
-    ro = gcmc.run_main_loop(oe, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, oe.prodcount)
+    ro = gcmc.run_main_loop(oe, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, oe.prodcount, nwindsteps)
     owin, onw, opc = o.windows()
     osurf = o.surface()
     o.close()

@@ -9,7 +9,9 @@ statistics (fmc_stats_from_windows: calc_Qst / calc_Cv, window-weighted
 means and deviations, avunion / stdunion).  Every value of the node report
 (f20.9) and of the final report (f15.6, errors f12.3 / f15.6) must match as
 printed, NaN included (the 'sampling' case closes an empty window, as the
-reference does)."""
+reference does).  Two runs count equilibration and / or production in
+cycles (negative step counts), whose ends and windows follow the guest
+count at each check."""
 import json
 import math
 import os
@@ -62,16 +64,20 @@ class OracleEngine:
     def prodcount(self):
         return self.o.windows()[2]
 
+    def guests(self):
+        return self.o.guest_total()
+
 
 def replay(case):
     sysm = stats_system(case)
     o = Oracle(sysm, seed=tuple(case["seeds"]))
     nwind, nwindsteps = gcmc.window_steps(sysm.steps, sysm.nwind)
     assert nwindsteps == case["nwindsteps"]
-    o.set_sampling(nwindsteps, sysm.accsample)
+    # in cycles the windows close by cycle count (the driver's CLOSE flags)
+    o.set_sampling(0 if sysm.steps < 0 else nwindsteps, sysm.accsample)
     eng = OracleEngine(o)
     minchk = gcmc.check_interval(sysm.eqsteps, sysm.nnumg)
-    r = gcmc.run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount)
+    r = gcmc.run_main_loop(eng, sysm.eqsteps, sysm.steps, minchk, sysm.accsample, eng.prodcount, nwindsteps)
     win, nw, pc = o.windows()
     o.close()
     return sysm, r, win, nw, pc
