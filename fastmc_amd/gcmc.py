@@ -53,7 +53,7 @@ def schedule(eqsteps: int, mcsteps: int, minchk: int):
     of the first later check that finds mcsteps production steps done
     (gcmc.f:707-752), and that step still counts."""
     if eqsteps < 0 or mcsteps <= 0:
-        raise NotImplementedError("cycle-based equilibration or production (negative steps) is not supported")
+        raise ValueError("cycle counts (negative steps) have no closed-form schedule: run_main_loop drives them")
     s0 = 1 if eqsteps == 0 else minchk * math.ceil(eqsteps / minchk)
     return s0, minchk * math.ceil((s0 + mcsteps) / minchk)
 
