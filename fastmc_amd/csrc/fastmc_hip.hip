@@ -771,6 +771,8 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
     off = al(off + (size_t)8 * (mxegrd + 3));
     P.off_ph = (int)off;
     off = al(off + (size_t)16 * 2 * P.nsmax * 3 * P.kcp);
+    P.off_ph32 = (int)off;
+    off = al(off + (size_t)8 * 2 * P.nsmax * 3 * P.kcp);
     P.off_cst = (int)off;
     off = al(off + (size_t)CST_BYTES);
     if (off > budget) {

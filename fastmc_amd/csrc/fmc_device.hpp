@@ -42,6 +42,7 @@ struct DevParams {
   // packed l m n), the guest coordinates (x, y, z over all guest slots) and,
   // in FP32 mode, the LJ columns of the guest-site types
   int off_erc, off_ph, off_cst, lds_bytes;
+  int off_ph32;                   // FP32 copy of the phase tables (mixed mode's low-weight k)
   int nsmax, kcp;
   int lds_all;
   int off_fw, off_k, off_g, off_vvv;
@@ -221,12 +222,6 @@ __device__ inline void mzt_advance(WState &w) {
 // tell): taken from the first active lane, so that what it indexes is read
 // through the scalar unit instead of one vector load per lane.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-// A double that is the same on every lane, moved to scalar registers.
-__device__ __forceinline__ double unid(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 // Lane k's value of v (k uniform).
 __device__ __forceinline__ double rdlane(double v, int k) {

@@ -659,6 +659,13 @@ constexpr int CST_BYTES = CST_SRF + 8 * (KMAXS + 1) * KSITE * (KTYPE + 1);
 static_assert(CST_SRF % 8 == 0, "srf alignment");
 static_assert((KMAXS + 1) * KSITE * (KTYPE + 1) <= CST_LSQ, "ltab overlaps lsq");
 
+// The same rows rounded to FP32 once when built (mixed mode's low-weight
+// k-vectors: the products are formed in FP32 from these, the same values
+// the FP64 rows convert to, without a conversion per use).
+__device__ inline float2 *ph_site32(unsigned char *lds, const DevParams &P, int cfg, int s) {
+  return (float2 *)(lds + P.off_ph32) + (size_t)((cfg * P.nsmax + s) * 3) * P.kcp;
+}
+
 __device__ inline double2 *ph_site(unsigned char *lds, const DevParams &P, int cfg, int s) {
   return (double2 *)(lds + P.off_ph) + (size_t)((cfg * P.nsmax + s) * 3) * P.kcp;
 }
@@ -686,6 +693,28 @@ __device__ inline void site_phase(const double2 *ph, int kcp, int ll, int mm, in
   } else {
     ckc = q * (clm * nc + slm * ns);
     cks = q * (slm * nc - clm * ns);
+  }
+}
+
+// The same product from FP32 rows (mixed mode's low-weight k-vectors).
+__device__ inline void site_phase(const float2 *ph, int kcp, int ll, int mm, int nn, float q, float &ckc,
+                                  float &cks) {
+  const int m = mm < 0 ? -mm : mm, n = nn < 0 ? -nn : nn;
+  const float2 L = ph[ll], M = ph[kcp + m], N = ph[2 * kcp + n];
+  float clm, slm;
+  if (mm >= 0) {
+    clm = L.x * M.x - L.y * M.y;
+    slm = L.y * M.x + M.y * L.x;
+  } else {
+    clm = L.x * M.x + L.y * M.y;
+    slm = L.y * M.x - M.y * L.x;
+  }
+  if (nn >= 0) {
+    ckc = q * (clm * N.x - slm * N.y);
+    cks = q * (slm * N.x + clm * N.y);
+  } else {
+    ckc = q * (clm * N.x + slm * N.y);
+    cks = q * (slm * N.x - clm * N.y);
   }
 }
 
@@ -834,8 +863,11 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         double s1, c1;
         sincospi(2.0 * fr, &s1, &c1);
         double2 *row = ph_site(lds, P, cfg, s) + ax * P.kcp;
+        float2 *row32 = ph_site32(lds, P, cfg, s) + ax * P.kcp;
         row[0] = make_double2(1.0, 0.0);
         row[1] = make_double2(c1, s1);
+        row32[0] = make_float2(1.0f, 0.0f);
+        row32[1] = make_float2((float)c1, (float)s1);
         double c = c1, sn = s1;
         for (int l = 2; l <= kmax; ++l) {
           double cn = c * c1 - sn * s1;
@@ -843,6 +875,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
           c = cn;
           sn = snn;
           row[l] = make_double2(c, sn);
+          row32[l] = make_float2((float)c, (float)sn);
         }
       }
     }
@@ -860,27 +893,22 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   if constexpr (LV) vvv = (const T *)(lds + P.off_vvv);
   else if constexpr (sizeof(T) == 8) vvv = (const T *)P.vvv;
   else vvv = (const T *)P.vvv32;
-  // NSITE > 0: trial sites in registers; padding sites (s >= natms) carry
-  // charge 0, no LJ column and no overlap test.  NSITE == 0: read from LDS.
-  // The site values are the same on every lane: unid keeps them in scalar
-  // registers, leaving the vector registers to the pair terms and k-slots.
+  // NSITE > 0: site charges and overlap thresholds in registers; padding
+  // sites (s >= natms) carry charge 0, no LJ column, no overlap or surface
+  // test, so whatever their LDS coordinates hold is selected away.  The site
+  // coordinates are read from LDS at each pair (same address on every lane,
+  // a broadcast): held in registers they spill the scalar file and cost more.
+  // NSITE == 0: everything from LDS.
   constexpr int NSA = NSITE > 0 ? NSITE : 1;
-  double xs[2][NSA][3], cq[NSA], ov2[NSA], qs[NSA];
+  double cq[NSA], ov2[NSA], qs[NSA];
   if constexpr (NSITE > 0) {
     const double ovl2 = sh.ovl2;
 #pragma unroll
     for (int s = 0; s < NSITE; ++s) {
       const bool real = s < natms;
-      const double c = sh.scq[mol][s], q = sh.sq[mol][s];
-      cq[s] = unid(real ? c : 0.0);
-      ov2[s] = unid(real ? ovl2 : -1.0);
-      qs[s] = unid(real ? q : 0.0);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double a = sh.xn[s][d], b = sh.xo[s][d];
-        xs[0][s][d] = unid(real ? a : 0.0);
-        xs[1][s][d] = unid(real ? b : 0.0);
-      }
+      cq[s] = real ? sh.scq[mol][s] : 0.0;
+      ov2[s] = real ? ovl2 : -1.0;
+      qs[s] = real ? sh.sq[mol][s] : 0.0;
     }
   }
   // atom index space: framework atoms, then the guests of each kind (per
@@ -967,7 +995,8 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         if (NSITE == 0 && s >= natms) break;
         double e2, ev;
         if constexpr (NSITE > 0)
-          pair_eval<T>(P, erc, vvv, xs[c][s], cq[s], col[s], ov2[s], sg2[s], xj, yj, zj, qj, e2, ev, ov);
+          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], cq[s], col[s], ov2[s], sg2[s], xj, yj, zj, qj, e2,
+                       ev, ov);
         else
           pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, sg2[s], xj, yj,
                        zj, qj, e2, ev, ov);
@@ -1008,7 +1037,12 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       for (int s = 0; s < NSR; ++s) {
         if (s >= natms) break;
         U ckc, cks;
-        site_phase<U>(ph_site(lds, P, 1, s), P.kcp, ll, mm, nn, (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
+        if constexpr (sizeof(U) == 4)
+          site_phase(ph_site32(lds, P, 1, s), P.kcp, ll, mm, nn,
+                        (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
+        else
+          site_phase<U>(ph_site(lds, P, 1, s), P.kcp, ll, mm, nn,
+                        (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
         c += ckc;
         s_ += cks;
       }
@@ -1029,7 +1063,12 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       for (int s = 0; s < NSR; ++s) {
         if (s >= natms) break;
         U ckc, cks;
-        site_phase<U>(ph_site(lds, P, 0, s), P.kcp, ll, mm, nn, (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
+        if constexpr (sizeof(U) == 4)
+          site_phase(ph_site32(lds, P, 0, s), P.kcp, ll, mm, nn,
+                        (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
+        else
+          site_phase<U>(ph_site(lds, P, 0, s), P.kcp, ll, mm, nn,
+                        (U)(NSITE > 0 ? qs[s < NSA ? s : 0] : lsq[mol * KSITE + s]), ckc, cks);
         c += ckc;
         s_ += cks;
       }
