@@ -906,9 +906,10 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
 #pragma unroll
     for (int s = 0; s < NSITE; ++s) {
       const bool real = s < natms;
-      cq[s] = real ? sh.scq[mol][s] : 0.0;
+      const double c = sh.scq[mol][s], q = sh.sq[mol][s];  // loaded unconditionally, then selected
+      cq[s] = real ? c : 0.0;
       ov2[s] = real ? ovl2 : -1.0;
-      qs[s] = real ? sh.sq[mol][s] : 0.0;
+      qs[s] = real ? q : 0.0;
     }
   }
   // atom index space: framework atoms, then the guests of each kind (per
