@@ -1,7 +1,8 @@
 """Guests with fewer sites than the three trial sites k_run keeps per pair
 (a one-site argon-like guest, and a two-site diatomic, beside CO2 on the
 config-0 framework): the padded site slots must contribute no energy and no
-overlap or surface flag.  The device FP64 chain runs against the oracle from
+overlap or surface flag.  A five-site methane beside CO2 takes the general
+trial-site path instead.  The device FP64 chain runs against the oracle from
 the same seeds: every move, molecule choice, decision and molecule count
 identical, and dU within 1e-10 * max(|dU|, kB T).  No pure rotation moves:
 rotating a one-site molecule changes nothing, its dU is rounding noise of
@@ -35,12 +36,23 @@ def diatomic():
                        lj={"Xa": (60.0 * KB_KCAL, 3.1), "Xb": (45.0 * KB_KCAL, 3.2)})
 
 
+def methane():
+    """Five sites (C and four H at 1.09 A, tetrahedral): more sites than the
+    register slots, so k_run takes its general trial-site path."""
+    h = 1.09 / 3 ** 0.5
+    return I.GuestSpec(name="CH4", sites=[("Cm", 12.0107, -0.24, 0.0, 0.0, 0.0)] +
+                       [("Hm", 1.00794, 0.06, sx * h, sy * h, sz * h)
+                        for sx, sy, sz in ((1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1))],
+                       lj={"Cm": (33.2 * KB_KCAL, 3.31), "Hm": (7.6 * KB_KCAL, 2.81)})
+
+
 MOVES = {"insert": 0.3, "delete": 0.3, "displace": 0.3, "jump": 0.0, "swap": 0.1, "translate": 0.0,
          "rotate": 0.0, "switch": 0.0}
 
 SYSTEMS = {
     "co2_ar": ([I.co2_guest(), argon()], [2.0, 5.0]),
     "co2_ar_x2": ([I.co2_guest(), argon(), diatomic()], [2.0, 5.0, 3.0]),
+    "co2_ch4": ([I.co2_guest(), methane()], [2.0, 5.0]),
 }
 
 
