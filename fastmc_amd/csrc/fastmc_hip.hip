@@ -1007,20 +1007,20 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
 
 int fmc_destroy(fmc_handle *h) {
   if (!h) return FMC_OK;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
-  for (void *p : h->allocs) hipFree(p);
-  if (h->dtrace) hipFree(h->dtrace);
-  if (h->dw.win) hipFree(h->dw.win);
-  if (h->dw.prob) hipFree(h->dw.prob);
-  if (h->dw.numg) hipFree(h->dw.numg);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->dtrace) (void)hipFree(h->dtrace);
+  if (h->dw.win) (void)hipFree(h->dw.win);
+  if (h->dw.prob) (void)hipFree(h->dw.prob);
+  if (h->dw.numg) (void)hipFree(h->dw.numg);
   for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
                   (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
                   (void *)h->g_vt})
-    if (p) hipFree(p);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->stream) hipStreamDestroy(h->stream);
+    if (p) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return FMC_OK;
 }
@@ -1206,7 +1206,7 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   DevWalkers dw = h->dw;
   if (trace) {
     if (h->dtrace_cap < nsteps) {
-      if (h->dtrace) hipFree(h->dtrace);
+      if (h->dtrace) (void)hipFree(h->dtrace);
       h->dtrace = nullptr;
       HIPCHK(hipMalloc(&h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps));
       h->dtrace_cap = nsteps;
@@ -1304,19 +1304,19 @@ int fmc_set_block(fmc_handle *h, int threads) {
       HIPCHK(hipMemcpy(h->dw.gz, bgz, ng, hipMemcpyDeviceToDevice));
       HIPCHK(hipMemcpy(h->dw.sk, bsk, nsk, hipMemcpyDeviceToDevice));
     }
-    hipFree(bws);
-    hipFree(bgx);
-    hipFree(bgy);
-    hipFree(bgz);
-    hipFree(bsk);
+    (void)hipFree(bws);
+    (void)hipFree(bgx);
+    (void)hipFree(bgy);
+    (void)hipFree(bgz);
+    (void)hipFree(bsk);
     h->block = best;
-    for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
     return FMC_OK;
   }
   if (threads < 64 || threads > 512 || threads % 64) return FMC_ERR_ARG;
   h->block = threads;
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   return FMC_OK;
 }
@@ -1427,7 +1427,7 @@ int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
   h->wl_maxn = p->maxn;
   h->wl_minn = p->minn;
   h->wl_nwin = p->nwindows;
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   return FMC_OK;
 }
@@ -1650,7 +1650,7 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
        hipMemcpy(cmap_out, dcmap, sizeof(double) * mapsize, hipMemcpyDeviceToHost) != hipSuccess))
     fail(FMC_ERR_HIP);
   for (void *p : {(void *)duni, (void *)dest, (void *)dovl, (void *)dbsum, (void *)dbcnt, (void *)demap, (void *)dcmap})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   if (rc != FMC_OK) return rc;
   res->attempts = total;
   res->accepted = accepted;
@@ -1726,10 +1726,10 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     HIPCHK(hipMemcpy(h->g_fwt, h->fw_type.data(), sizeof(int) * nfw, hipMemcpyHostToDevice));
   }
   if (nconf > h->g_cap || gstride > h->g_gcap) {
-    if (h->g_sc) hipFree(h->g_sc);
-    if (h->g_nmols) hipFree(h->g_nmols);
-    if (h->g_xyz) hipFree(h->g_xyz);
-    if (h->g_out) hipFree(h->g_out);
+    if (h->g_sc) (void)hipFree(h->g_sc);
+    if (h->g_nmols) (void)hipFree(h->g_nmols);
+    if (h->g_xyz) (void)hipFree(h->g_xyz);
+    if (h->g_out) (void)hipFree(h->g_out);
     h->g_cap = std::max(nconf, h->g_cap);
     h->g_gcap = std::max(gstride, h->g_gcap);
     HIPCHK(hipMalloc(&h->g_sc, sizeof(double2) * (size_t)h->g_cap * nk));
@@ -1738,7 +1738,7 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     HIPCHK(hipMalloc(&h->g_out, sizeof(double) * ((size_t)h->g_cap * (1 + nfw + 2 + 2 * NTT) + nfw)));
   }
   if (npt > h->g_nptcap) {
-    if (h->g_vt) hipFree(h->g_vt);
+    if (h->g_vt) (void)hipFree(h->g_vt);
     h->g_nptcap = npt;
     HIPCHK(hipMalloc(&h->g_vt, sizeof(double) * 3 * (size_t)vs * std::max(npt, 1)));
   }
@@ -1905,7 +1905,7 @@ int fmc_set_sampling(fmc_handle *h, int nwindsteps, int accepted_only, int max_w
   if (!h || nwindsteps < 0 || max_windows < 1) return FMC_ERR_ARG;
   HIPCHK(hipStreamSynchronize(h->stream));
   // captured graphs hold the old window buffer
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   if (h->dw.win) HIPCHK(hipFree(h->dw.win));
   h->dw.win = nullptr;
@@ -1950,7 +1950,7 @@ int fmc_set_prob(fmc_handle *h, double griddim, const int gridfactor[3], int npl
         plots[p].nsites > FMC_MAX_SITES + 1)
       return FMC_ERR_ARG;
   HIPCHK(hipStreamSynchronize(h->stream));
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   if (h->dw.prob) HIPCHK(hipFree(h->dw.prob));
   h->dw.prob = nullptr;
@@ -1987,7 +1987,7 @@ int fmc_set_numguests(fmc_handle *h, int nnumg, int cap) {
   if (!h || nnumg < 0 || cap < 0 || (nnumg > 0 && cap < 1)) return FMC_ERR_ARG;
   DevParams &P = h->hp;
   HIPCHK(hipStreamSynchronize(h->stream));
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   if (h->dw.numg) HIPCHK(hipFree(h->dw.numg));
   h->dw.numg = nullptr;
