@@ -92,3 +92,54 @@ def test_gpu_padded_sites_match_oracle(name):
                                                   for k in kinds}, "max scaled dU error", float(err.max()), flush=True)
     assert len(kinds) == len(SYSTEMS[name][0])   # every guest kind moved
     assert err.max() <= 1e-10
+
+
+NOSWAP = dict(MOVES, insert=1 / 3, delete=1 / 3, displace=1 / 3, swap=0.0)
+
+
+@pytest.mark.parametrize("name", ["co2_ar", "co2_ch4"])
+def test_gpu_mixed_precision_du_error_ragged(name):
+    """Mixed mode on the same guests (the general trial-site path and the
+    FP32 phase rows included): the FP64 device chain's trials replayed
+    through the one-trial seam on a mixed and an FP64 handle, |dU_mixed -
+    dU_fp64| <= 1e-6 * max(|dU_fp64|, kB T) on every trial without overlap."""
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    print(name, "mixed", flush=True)
+    guests, pressures = SYSTEMS[name]
+    d = tempfile.mkdtemp(prefix=f"fmc_ragged_mixed_{name}_")
+    try:
+        I.write_system(d, I.make_framework(), guests, 293.0, pressures, moves=NOSWAP)
+        sysm = read_system(d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    seeds = (21, 43, 65, 87)
+    natoms = [len(g.sites) for g in guests]
+    hd = Handle(sysm, precision=FMC_FP64, seeds=[seeds])
+    try:
+        tr = hd.run(1200, trace=True)
+    finally:
+        hd.close()
+    a_h = Handle(sysm, precision=FMC_FP64, seeds=[seeds])
+    b_h = Handle(sysm, precision=FMC_MIXED, seeds=[seeds])
+    errs = []
+    try:
+        for r in tr:
+            g, mv, c = int(r["iguest"]), int(r["move"]), int(r["choice"])
+            xyz = r["xyz"][:3 * natoms[g - 1]].reshape(natoms[g - 1], 3)
+            if mv == 1:
+                a, b = a_h.trial_insert(g, xyz), b_h.trial_insert(g, xyz)
+            elif mv == 2:
+                a, b = a_h.trial_delete(g, c), b_h.trial_delete(g, c)
+            else:
+                a, b = a_h.trial_displace(g, c, xyz), b_h.trial_displace(g, c, xyz)
+            assert a.overlap == b.overlap
+            if not a.overlap:
+                errs.append(abs(a.estep - b.estep) / max(abs(a.estep), KB_KCAL * 293.0))
+            for h in (a_h, b_h):
+                h.accept() if r["accepted"] else h.reject()
+    finally:
+        a_h.close()
+        b_h.close()
+    errs = np.array(errs)
+    print(name, "mixed dU scaled error: max", errs.max(), "n", len(errs), flush=True)
+    assert len(errs) > 500 and errs.max() <= 1e-6
