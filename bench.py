@@ -239,7 +239,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
-              "(state points shard across GPUs with no per-trial communication; see DESIGN.md)",
+              "(state points shard across GPUs as whole walkers, with no per-trial communication)",
               file=sys.stderr)
         sys.exit(2)
     from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle

@@ -1,8 +1,8 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy restatement of the single_point energy
 (gcmc.f:2549-2661) of one guest configuration as a function of theta (the
 framework charges and the LJ type parameters, Lorentz-Berthelot for every
-pair with a framework type), with its analytic gradient: the checker of
-fmc_energy_grad.  Only tests/ import it.
+pair with a framework type), with its analytic gradient: the oracle that
+fmc_energy_grad is compared with.  Only tests/ import it.
 
 Terms (FIELD units after / engunit):
   recip   C sum_k a_k |S_k|^2 + self + charged-system term   (ewald1, ewald.f:354-661)

@@ -1,8 +1,8 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- the oracle.  A plain-C FP64 restatement of the
  * reference FastMC GCMC trial-move energy path (FastMC v1.4.7, Fortran).  It is
- * the checker for the HIP library: tests/, smoke() and bench.py's cpu_baseline
- * leg may call it; the product never does.
+ * what the HIP library's results are compared with: tests/, smoke() and
+ * bench.py's cpu_baseline leg may call it; the product never does.
  *
  * Every routine names the reference routine it restates (file:line).  Arrays
  * that the reference indexes from 1 are kept 1-based here (index 0 unused) so

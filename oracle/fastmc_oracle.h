@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY.  The oracle is a plain-C FP64 restatement of the
  * reference FastMC trial-move energy path, used by tests/, smoke() and the
- * cpu_baseline leg of bench.py as the checker.  The product
+ * cpu_baseline leg of bench.py to compare results with.  The product
  * (libfastmc_hip.so) never links, loads or calls it.
  */
 #ifndef FASTMC_ORACLE_H

@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of the C oracle (liboracle.so).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, and only as the checker.  The product package never does.
+this module, and only to compare results with.  The product package never does.
 """
 from __future__ import annotations
 

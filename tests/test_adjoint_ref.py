@@ -1,4 +1,4 @@
-"""The FP64 numpy checker of the energy gradient (oracle/adjoint_ref.py),
+"""The FP64 numpy oracle of the energy gradient (oracle/adjoint_ref.py),
 pinned on the CPU: its energy equals the reference-faithful C oracle's
 single_point energy of the same configuration, and its analytic gradient
 equals central finite differences of that C-oracle energy (theta moved in the

@@ -1,5 +1,5 @@
 """GPU energy gradient with respect to theta (framework charges, LJ type
-parameters): fmc_energy_grad against the FP64 numpy checker (itself pinned
+parameters): fmc_energy_grad against the FP64 numpy oracle (itself pinned
 to the reference-faithful C oracle on the CPU), central finite differences
 (h = 1e-5) of its own FP64 energy, the mixed (FP32-product) mode, the
 torch.autograd.Function and the fluctuation estimator of d<N>/dtheta on
