@@ -1,5 +1,5 @@
-! Test infrastructure only (never shipped, never run on the GPU box's product
-! path): a trace recorder that drives the *reference* FastMC modules, compiled
+! Test infrastructure only (never shipped, never on the product's path): a
+! trace recorder that drives the *reference* FastMC modules, compiled
 ! from the read-only reference sources by oracle/Makefile, through the GCMC
 ! insert / delete / displace / translate / rotate sequence and writes every
 ! trial's outcome to the file TRACE in the working directory.

@@ -73,7 +73,7 @@ def replay(case):
     o = Oracle(sysm, seed=tuple(case["seeds"]))
     nwind, nwindsteps = gcmc.window_steps(sysm.steps, sysm.nwind)
     assert nwindsteps == case["nwindsteps"]
-    # in cycles the windows close by cycle count (the driver's CLOSE flags)
+    # in cycles the windows close by cycle count (run_main_loop's CLOSE flags)
     o.set_sampling(0 if sysm.steps < 0 else nwindsteps, sysm.accsample)
     eng = OracleEngine(o)
     minchk = gcmc.check_interval(sysm.eqsteps, sysm.nnumg)
