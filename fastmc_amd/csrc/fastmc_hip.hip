@@ -44,6 +44,7 @@ struct GraphKey {
 
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
+  int trace_w = 0;               // walker whose steps fmc_run records
   int nsite = 0;                 // 3: every guest has at most 3 sites (straight-line pair code)
   DevParams hp;                  // host copy
   DevParams *dp = nullptr;       // device copy
@@ -69,7 +70,7 @@ struct fmc_handle {
   size_t lds_base_end = 0;       // LDS bytes of the always-staged part of the plan
   int wl_maxn = 0, wl_minn = 0, wl_nwin = 0;  // Wang-Landau set-up (0: off)
   // energy-gradient entry: framework types and frozen flags, FIELD LJ rows
-  std::vector<int> fw_type, fw_frz;
+  std::vector<int> fw_type, fw_frz, fw_spec;
   std::vector<double> pair_eps, pair_sig;   // per FIELD row
   std::vector<int> pair_row;                // [KTYPE+1][KTYPE+1] -> FIELD row or -1
   double rcut = 0.0;
@@ -244,12 +245,14 @@ void accept_dispatch(fmc_handle *h, int w) {
   }
 }
 
-void resum_dispatch(fmc_handle *h, double *drift) {
-  dim3 grid((h->hp.nk + 255) / 256, h->nwalkers);
+// FP64 re-sum of S(k): every walker (only < 0) or walker `only`
+void resum_dispatch(fmc_handle *h, double *drift, int only = -1) {
+  dim3 grid((h->hp.nk + 255) / 256, only < 0 ? h->nwalkers : 1);
+  const int w0 = only < 0 ? 0 : only;
   switch (h->ns) {
-    case 2: hipLaunchKernelGGL((k_resum<2>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
-    case 3: hipLaunchKernelGGL((k_resum<3>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
-    default: hipLaunchKernelGGL((k_resum<4>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift); break;
+    case 2: hipLaunchKernelGGL((k_resum<2>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift, w0); break;
+    case 3: hipLaunchKernelGGL((k_resum<3>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift, w0); break;
+    default: hipLaunchKernelGGL((k_resum<4>), grid, dim3(256), 0, h->stream, h->dp, h->dw, drift, w0); break;
   }
 }
 
@@ -262,37 +265,50 @@ int walker_natoms(fmc_handle *h, int w, std::vector<WState> &ws) {
   return n;
 }
 
-// single_point for every walker: species-resolved recip, real and vdw sums.
-// out[w][0..4] as fmc_single_point; also refreshes ws[w].energy (gcmc.f:554-582).
-int single_points(fmc_handle *h, std::vector<WState> &ws, std::vector<double> &out, bool set_energy) {
+// The per-walker device arrays of walker w alone: a launch through this view
+// addresses walker w as walker 0.
+DevWalkers walker_view(const fmc_handle *h, int w) {
+  const DevParams &P = h->hp;
+  DevWalkers dw1 = h->dw;
+  dw1.ws = h->dw.ws + w;
+  dw1.gx = h->dw.gx + (size_t)w * P.gatoms_total;
+  dw1.gy = h->dw.gy + (size_t)w * P.gatoms_total;
+  dw1.gz = h->dw.gz + (size_t)w * P.gatoms_total;
+  dw1.sk = h->dw.sk + (size_t)w * (P.maxmls + 1) * P.nk;
+  if (P.fw_per_walker) {
+    dw1.fwx = h->dw.fwx + (size_t)w * P.nfw;
+    dw1.fwy = h->dw.fwy + (size_t)w * P.nfw;
+    dw1.fwz = h->dw.fwz + (size_t)w * P.nfw;
+  }
+  return dw1;
+}
+
+// single_point for every walker (only < 0) or walker `only`: species-resolved
+// recip, real and vdw sums.  out[w][0..4] as fmc_single_point; also
+// refreshes ws[w].energy (gcmc.f:554-582).
+int single_points(fmc_handle *h, std::vector<WState> &ws, std::vector<double> &out, bool set_energy,
+                  int only = -1) {
   const DevParams &P = h->hp;
   const int NW = h->nwalkers, NC = KMAXC + 1;
+  const int wlo = only < 0 ? 0 : only, whi = only < 0 ? NW : only + 1;
   std::vector<double> rec((size_t)NW * NC), pairs((size_t)NW * 2 * NC, 0.0);
   double *d = h->dscratch;
   HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * (size_t)NW * 3 * NC, h->stream));
-  switch (h->ns) {
-    case 2: hipLaunchKernelGGL((k_recip_sums<2>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
-    case 3: hipLaunchKernelGGL((k_recip_sums<3>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
-    default: hipLaunchKernelGGL((k_recip_sums<4>), dim3(NW), dim3(256), 0, h->stream, h->dp, h->dw, d); break;
+  {
+    const DevWalkers dv = only < 0 ? h->dw : walker_view(h, only);
+    double *o = d + (size_t)wlo * NC;
+    const dim3 grid(whi - wlo);
+    switch (h->ns) {
+      case 2: hipLaunchKernelGGL((k_recip_sums<2>), grid, dim3(256), 0, h->stream, h->dp, dv, o); break;
+      case 3: hipLaunchKernelGGL((k_recip_sums<3>), grid, dim3(256), 0, h->stream, h->dp, dv, o); break;
+      default: hipLaunchKernelGGL((k_recip_sums<4>), grid, dim3(256), 0, h->stream, h->dp, dv, o); break;
+    }
   }
-  int nmax = 0;
-  for (int w = 0; w < NW; ++w) nmax = std::max(nmax, walker_natoms(h, w, ws));
-  // k_pair_sums walks each walker's own atom count; pass the largest and let
-  // the kernel skip past the walker's end through the atom lookup
-  for (int w = 0; w < NW; ++w) {
+  // k_pair_sums walks one walker's own atoms per launch
+  for (int w = wlo; w < whi; ++w) {
     int n = walker_natoms(h, w, ws);
     dim3 grid((n + 255) / 256, 1);
-    DevWalkers dw1 = h->dw;
-    // shift the per-walker views so blockIdx.y == 0 addresses walker w
-    dw1.ws = h->dw.ws + w;
-    dw1.gx = h->dw.gx + (size_t)w * P.gatoms_total;
-    dw1.gy = h->dw.gy + (size_t)w * P.gatoms_total;
-    dw1.gz = h->dw.gz + (size_t)w * P.gatoms_total;
-    if (P.fw_per_walker) {
-      dw1.fwx = h->dw.fwx + (size_t)w * P.nfw;
-      dw1.fwy = h->dw.fwy + (size_t)w * P.nfw;
-      dw1.fwz = h->dw.fwz + (size_t)w * P.nfw;
-    }
+    const DevWalkers dw1 = walker_view(h, w);
     double *o = d + (size_t)NW * NC + (size_t)w * 2 * NC;
     switch (h->ns) {
       case 2: hipLaunchKernelGGL((k_pair_sums<2>), grid, dim3(256), 0, h->stream, h->dp, dw1, n, o); break;
@@ -308,7 +324,7 @@ int single_points(fmc_handle *h, std::vector<WState> &ws, std::vector<double> &o
   out.assign((size_t)NW * 5, 0.0);
   const int NS = h->ns;
   const double C = 2.0 * P.rvolm * R4PIE0 / P.epsq;
-  for (int w = 0; w < NW; ++w) {
+  for (int w = wlo; w < whi; ++w) {
     WState &s = ws[w];
     // ewald1 finalisation, ewald.f:626-643
     double q2 = h->selfq2_all_fw;
@@ -676,6 +692,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   P.nfw = (int)fwx.size();
   h->fw_type = fwtype;
   h->fw_frz = fwfrz;
+  h->fw_spec = fwspec;
   h->rcut = sys->rcut;
   // decision constants: the same expressions as the per-trial code of
   // ewald1_guest (ewald.f:96-105) and mc_insert (gcmc.f:1729)
@@ -1062,6 +1079,49 @@ int fmc_full_resum(fmc_handle *h, int w, double *max_drift) {
   return FMC_OK;
 }
 
+int fmc_load_guests(fmc_handle *h, int w, int iguest, const double *xyz, int nmols, const double *sumchg) {
+  if (!h || w < 0 || w >= h->nwalkers) return FMC_ERR_ARG;
+  const DevParams &P = h->hp;
+  if (iguest < 1 || iguest > P.ntpguest || nmols < 0 || nmols > P.gcap[iguest]) return FMC_ERR_ARG;
+  if (nmols > 0 && !xyz) return FMC_ERR_ARG;
+  if (P.wl_nbins > 0 && iguest == 1 && nmols + 1 > P.wl_nbins) return FMC_ERR_ARG;
+  const int mol = P.locguest[iguest], na = P.natoms[mol], n = nmols * na;
+  std::vector<WState> ws(h->nwalkers);
+  HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
+  WState &s = ws[w];
+  // the guest kind's slots: the given molecules, zeros past them up to the old count
+  const int nold = s.nmols[mol] * na, nput = std::max(n, nold);
+  if (nput > 0) {
+    std::vector<double> x(nput, 0.0), y(nput, 0.0), z(nput, 0.0);
+    for (int a = 0; a < n; ++a) {
+      x[a] = xyz[3 * a];
+      y[a] = xyz[3 * a + 1];
+      z[a] = xyz[3 * a + 2];
+    }
+    const size_t base = (size_t)w * P.gatoms_total + P.gbase[iguest];
+    HIPCHK(hipMemcpy(h->dw.gx + base, x.data(), 8 * (size_t)nput, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->dw.gy + base, y.data(), 8 * (size_t)nput, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->dw.gz + base, z.data(), 8 * (size_t)nput, hipMemcpyHostToDevice));
+  }
+  // counts and sums that follow from the molecule list, as at start-up
+  s.nmols[mol] = nmols;
+  s.chgsum_mol[mol] = (double)nmols * P.chgtmp[iguest];
+  s.engsic[loc2(mol, mol)] = (double)nmols * P.selfq2[iguest];
+  if (sumchg) s.sumchg = *sumchg;
+  lrcorrect_counts(P, nullptr, h->fw_type, h->fw_spec, h->fw_frz, s);
+  s.pend_valid = 0;
+  s.pend_kind = 0;
+  h->pend_state[w] = 0;
+  HIPCHK(hipMemcpy(h->dw.ws + w, &s, sizeof s, hipMemcpyHostToDevice));
+  // S(k) of every species and the total from the coordinates, then the energies
+  resum_dispatch(h, nullptr, w);
+  HIPCHK(hipGetLastError());
+  std::vector<double> sp;
+  if (int e = single_points(h, ws, sp, true, w)) return e;
+  HIPCHK(hipMemcpy(h->dw.ws + w, &s, sizeof s, hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
 static int seam_trial(fmc_handle *h, int w, int kind, int iguest, int choice, const double *xyz, fmc_trial *out) {
   if (!h || w < 0 || w >= h->nwalkers || !out) return FMC_ERR_ARG;
   const DevParams &P = h->hp;
@@ -1212,6 +1272,7 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
       h->dtrace_cap = nsteps;
     }
     dw.trace = h->dtrace;
+    dw.trace_w = h->trace_w;
   } else {
     dw.trace = nullptr;
   }
@@ -1260,6 +1321,12 @@ int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_st
   return FMC_OK;
 }
 
+int fmc_set_trace_walker(fmc_handle *h, int w) {
+  if (!h || w < 0 || w >= h->nwalkers) return FMC_ERR_ARG;
+  h->trace_w = w;
+  return FMC_OK;
+}
+
 int fmc_last_timing(fmc_handle *h, double *ms, int *launches) {
   if (!h) return FMC_ERR_ARG;
   if (ms) *ms = h->last_ms;
@@ -1271,48 +1338,55 @@ int fmc_set_block(fmc_handle *h, int threads) {
   if (!h) return FMC_ERR_ARG;
   if (threads == 0) {
     // autotune on a snapshot of the walkers: time a short run per block size
-    // and restore the chains afterwards
+    // and restore every array a run writes afterwards (state, guests, S(k),
+    // Wang-Landau ln g and visits; records past the snapshot's counts are
+    // rewritten by later runs)
     const DevParams &P = h->hp;
-    size_t nws = sizeof(WState) * (size_t)h->nwalkers;
-    size_t ng = 8 * (size_t)h->nwalkers * P.gatoms_total;
-    size_t nsk = sizeof(double2) * (size_t)h->nwalkers * (P.maxmls + 1) * P.nk;
-    void *bws, *bgx, *bgy, *bgz, *bsk;
-    HIPCHK(hipMalloc(&bws, nws));
-    HIPCHK(hipMalloc(&bgx, ng));
-    HIPCHK(hipMalloc(&bgy, ng));
-    HIPCHK(hipMalloc(&bgz, ng));
-    HIPCHK(hipMalloc(&bsk, nsk));
-    HIPCHK(hipMemcpy(bws, h->dw.ws, nws, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(bgx, h->dw.gx, ng, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(bgy, h->dw.gy, ng, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(bgz, h->dw.gz, ng, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(bsk, h->dw.sk, nsk, hipMemcpyDeviceToDevice));
-    int best = 512;
+    struct Snap { void *dev; size_t bytes; void *copy; };
+    std::vector<Snap> snap = {
+        {h->dw.ws, sizeof(WState) * (size_t)h->nwalkers, nullptr},
+        {h->dw.gx, 8 * (size_t)h->nwalkers * P.gatoms_total, nullptr},
+        {h->dw.gy, 8 * (size_t)h->nwalkers * P.gatoms_total, nullptr},
+        {h->dw.gz, 8 * (size_t)h->nwalkers * P.gatoms_total, nullptr},
+        {h->dw.sk, sizeof(double2) * (size_t)h->nwalkers * (P.maxmls + 1) * P.nk, nullptr}};
+    if (P.wl_nbins > 0) {
+      snap.push_back({h->dw.wl_lng, sizeof(double) * (size_t)h->nwalkers * P.wl_nbins, nullptr});
+      snap.push_back({h->dw.wl_hist, sizeof(int) * (size_t)h->nwalkers * P.wl_nbins, nullptr});
+    }
+    int rc = FMC_OK;
+    for (auto &sn : snap) {
+      if (hipMalloc(&sn.copy, sn.bytes) != hipSuccess) {
+        sn.copy = nullptr;
+        rc = FMC_ERR_ALLOC;
+        break;
+      }
+      if (hipMemcpy(sn.copy, sn.dev, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+        rc = FMC_ERR_HIP;
+        break;
+      }
+    }
+    const int prev = h->block;
+    int best = prev;
     double best_ms = 1e30;
     const int cand[3] = {128, 256, 512};
-    for (int c : cand) {
-      h->block = c;
-      fmc_run(h, 64, nullptr, 0);
-      fmc_run(h, 256, nullptr, 0);
-      if (h->last_ms < best_ms) {
+    for (int i = 0; i < 3 && rc == FMC_OK; ++i) {
+      h->block = cand[i];
+      int e = fmc_run(h, 64, nullptr, 0);
+      if (e == FMC_OK) e = fmc_run(h, 256, nullptr, 0);
+      if (e == FMC_OK && h->last_ms < best_ms) {
         best_ms = h->last_ms;
-        best = c;
+        best = cand[i];
       }
-      HIPCHK(hipMemcpy(h->dw.ws, bws, nws, hipMemcpyDeviceToDevice));
-      HIPCHK(hipMemcpy(h->dw.gx, bgx, ng, hipMemcpyDeviceToDevice));
-      HIPCHK(hipMemcpy(h->dw.gy, bgy, ng, hipMemcpyDeviceToDevice));
-      HIPCHK(hipMemcpy(h->dw.gz, bgz, ng, hipMemcpyDeviceToDevice));
-      HIPCHK(hipMemcpy(h->dw.sk, bsk, nsk, hipMemcpyDeviceToDevice));
+      for (auto &sn : snap)
+        if (hipMemcpy(sn.dev, sn.copy, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = FMC_ERR_HIP;
+      if (e != FMC_OK && rc == FMC_OK) rc = e;
     }
-    (void)hipFree(bws);
-    (void)hipFree(bgx);
-    (void)hipFree(bgy);
-    (void)hipFree(bgz);
-    (void)hipFree(bsk);
-    h->block = best;
+    for (auto &sn : snap)
+      if (sn.copy) (void)hipFree(sn.copy);
+    h->block = rc == FMC_OK ? best : prev;
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
-    return FMC_OK;
+    return rc;
   }
   if (threads < 64 || threads > 512 || threads % 64) return FMC_ERR_ARG;
   h->block = threads;
@@ -1359,43 +1433,35 @@ int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
     if (fmc_wl_window(win, p->nwindows, p->maxn, p->minn, &mn, &mx, nullptr) != FMC_OK || mn > mx)
       return FMC_ERR_ARG;
   }
-  // LDS: ln g and visits after the rest of the plan; without room the
-  // all-in-LDS variant gives way
-  const size_t budget = 160 * 1024 - 1024;
-  const int nbins = p->maxn + 2;
-  const size_t need = (size_t)12 * nbins + 32;
-  size_t base = (size_t)P.lds_bytes;
-  if (P.wl_nbins > 0) base = (size_t)P.wl_off;
-  if (base + need > budget && P.lds_all) {
-    P.lds_all = 0;
-    base = h->lds_base_end;
-  }
-  if (base + need > budget) return FMC_ERR_ARG;
-  P.wl_off = (int)((base + 15) & ~(size_t)15);
-  P.lds_bytes = (int)((P.wl_off + (size_t)12 * nbins + 15) & ~(size_t)15);
-  P.wl_nbins = nbins;
-  P.wl_flat = p->flatcoeff;
-  P.wl_visittol = p->visittol;
-  P.wl_prectol = p->prectol;
-  if (!h->dw.wl_lng) {
-    int rc = dalloc(h, &h->dw.wl_lng, (size_t)h->nwalkers * nbins);
-    rc |= dalloc(h, &h->dw.wl_hist, (size_t)h->nwalkers * nbins);
-    if (rc) return FMC_ERR_ALLOC;
-  } else if (nbins > h->wl_maxn + 2) {
-    return FMC_ERR_ARG;  // re-setup may not grow the range
-  }
-  HIPCHK(hipMemset(h->dw.wl_lng, 0, sizeof(double) * (size_t)h->nwalkers * nbins));
-  HIPCHK(hipMemset(h->dw.wl_hist, 0, sizeof(int) * (size_t)h->nwalkers * nbins));
-  HIPCHK(allow_lds_dispatch(h->ns, h->precision == FMC_MIXED, P.lds_bytes));
-  HIPCHK(hipMemcpy(h->dp, &P, sizeof P, hipMemcpyHostToDevice));
-  // guest-1 thermal wavelength, wang_landau.f:121-123 (molmass = sum of site masses, readinputs.f:208,247)
+  // every check first, on copies: the handle changes only once all pass
   const int mol1 = P.locguest[1];
-  double molmass = 0.0;
-  for (int i = 0; i < P.natoms[mol1]; ++i) molmass += P.site_m[mol1][i];
   std::vector<WState> ws(h->nwalkers);
   HIPCHK(hipMemcpy(ws.data(), h->dw.ws, sizeof(WState) * ws.size(), hipMemcpyDeviceToHost));
   for (int w = 0; w < h->nwalkers; ++w)
     if (ws[w].nmols[mol1] > p->maxn) return FMC_ERR_ARG;  // more guests than the sampled range
+  const int nbins = p->maxn + 2;
+  if (h->dw.wl_lng && nbins > h->wl_maxn + 2) return FMC_ERR_ARG;  // re-setup may not grow the range
+  // LDS: ln g and visits after the rest of the plan; without room the
+  // all-in-LDS variant gives way
+  DevParams Pn = P;
+  const size_t budget = 160 * 1024 - 1024;
+  const size_t need = (size_t)12 * nbins + 32;
+  size_t base = (size_t)Pn.lds_bytes;
+  if (Pn.wl_nbins > 0) base = (size_t)Pn.wl_off;
+  if (base + need > budget && Pn.lds_all) {
+    Pn.lds_all = 0;
+    base = h->lds_base_end;
+  }
+  if (base + need > budget) return FMC_ERR_ARG;
+  Pn.wl_off = (int)((base + 15) & ~(size_t)15);
+  Pn.lds_bytes = (int)((Pn.wl_off + (size_t)12 * nbins + 15) & ~(size_t)15);
+  Pn.wl_nbins = nbins;
+  Pn.wl_flat = p->flatcoeff;
+  Pn.wl_visittol = p->visittol;
+  Pn.wl_prectol = p->prectol;
+  // guest-1 thermal wavelength, wang_landau.f:121-123 (molmass = sum of site masses, readinputs.f:208,247)
+  double molmass = 0.0;
+  for (int i = 0; i < P.natoms[mol1]; ++i) molmass += P.site_m[mol1][i];
   for (int w = 0; w < h->nwalkers; ++w) {
     WState &s = ws[w];
     int win = (int)((long long)w * p->nwindows / h->nwalkers), mn, mx;
@@ -1423,12 +1489,23 @@ int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
       s.tran_delr[g] = s.delr[g];
     }
   }
+  // commit: the graphs captured with the old plan go first
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  if (!h->dw.wl_lng) {
+    int rc = dalloc(h, &h->dw.wl_lng, (size_t)h->nwalkers * nbins);
+    rc |= dalloc(h, &h->dw.wl_hist, (size_t)h->nwalkers * nbins);
+    if (rc) return FMC_ERR_ALLOC;
+  }
+  HIPCHK(hipMemset(h->dw.wl_lng, 0, sizeof(double) * (size_t)h->nwalkers * nbins));
+  HIPCHK(hipMemset(h->dw.wl_hist, 0, sizeof(int) * (size_t)h->nwalkers * nbins));
+  HIPCHK(allow_lds_dispatch(h->ns, h->precision == FMC_MIXED, Pn.lds_bytes));
+  P = Pn;
+  HIPCHK(hipMemcpy(h->dp, &P, sizeof P, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
   h->wl_maxn = p->maxn;
   h->wl_minn = p->minn;
   h->wl_nwin = p->nwindows;
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
   return FMC_OK;
 }
 

@@ -133,9 +133,12 @@ struct WState {
   double elrc, elrc_mol[KMAXC + 1];
   int numtyp[KTYPE + 1], numfrz[KTYPE + 1];
   int numtyp_mol[KMAXS + 2][KTYPE + 1], numfrz_mol[KMAXS + 2][KTYPE + 1];
-  // pending trial of the one-trial-at-a-time seam
+  // pending trial of the one-trial-at-a-time seam, and what its accept needs
+  // (kept per walker, so trials of several walkers may be pending at once)
   int pend_kind, pend_iguest, pend_choice, pend_valid;
   double pend_xyz[3 * KSITE];
+  double pend_delrc, pend_delE[KMAXS + 1], pend_delrc_mol[KMAXC + 1];
+  int pend_surface, pend_pad;
   // statistics
   long long steps, acc[9], att[9], counted;
   double sum_n[KMAXS + 1], sum_n2[KMAXS + 1], sum_e, sum_e2;
@@ -166,7 +169,8 @@ struct DevWalkers {
   double2 *sscr;             // [w][2][nk]: s_new, s_old of the current trial
   double2 *sbak;             // [w][maxmls+1][nk]: S(k) rows kept across a swap
   double *fwx, *fwy, *fwz;   // per-walker framework copies (if fw_per_walker)
-  fmc_trace_rec *trace;      // walker 0 trace or null
+  fmc_trace_rec *trace;      // trace of walker trace_w, or null
+  int trace_w;
   fmc_trial *trial_out;      // seam output
   double *wl_lng;            // [w][wl_nbins] ln g(N) (dos_hist)
   int *wl_hist;              // [w][wl_nbins] visits (visit_hist)

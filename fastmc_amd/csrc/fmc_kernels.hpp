@@ -2102,7 +2102,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
-      decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == 0);
+      decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w);
       if (sh.swap_phase != 1) {
         WState &ws = sh.ws;
         const int mk = sh.mkind;
@@ -2133,7 +2133,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       swap_restore<NS, LF>(P, W, w, sh, lds, ks);
     }
     if (phase == 1) continue;
-    if (threadIdx.x == 0 && W.trace && w == 0) record<NS>(P, W, sh, step);
+    if (threadIdx.x == 0 && W.trace && w == W.trace_w) record<NS>(P, W, sh, step);
     // the step's statistics on wave 1, idle while wave 0 proposes the next
     // step (the fields written here are read nowhere before the next barrier)
     const int stat_wave = blockDim.x > 64 ? 1 : 0;
@@ -2219,14 +2219,13 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     o->tail = sh.comp[3];
     o->overlap = sh.ovl_new;
     o->surface = sh.srf_new | (sh.srf_old << 1);
-    // keep what accept needs
+    // keep what accept needs in the walker's own state
     WState &ws = sh.ws;
     ws.pend_valid = 1;
-    double *keep = (double *)(W.trial_out + 1);
-    keep[0] = sh.delrc;
-    for (int i = 0; i <= KMAXS; ++i) keep[1 + i] = sh.delE[i];
-    for (int i = 0; i <= KMAXC; ++i) keep[2 + KMAXS + i] = sh.delrc_mol[i];
-    keep[3 + KMAXS + KMAXC] = (double)(sh.srf_new | (sh.srf_old << 1));
+    ws.pend_delrc = sh.delrc;
+    for (int i = 0; i <= KMAXS; ++i) ws.pend_delE[i] = sh.delE[i];
+    for (int i = 0; i <= KMAXC; ++i) ws.pend_delrc_mol[i] = sh.delrc_mol[i];
+    ws.pend_surface = sh.srf_new | (sh.srf_old << 1);
   }
   stage_out<NS>(W, w, sh);
 }
@@ -2247,11 +2246,10 @@ __global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp
     sh.choice = ws.pend_choice;
     for (int i = 0; i < sh.natms; ++i)
       for (int d = 0; d < 3; ++d) sh.xn[i][d] = ws.pend_xyz[3 * i + d];
-    const double *keep = (const double *)(W.trial_out + 1);
-    sh.delrc = keep[0];
-    for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = keep[1 + i];
-    for (int i = 0; i <= KMAXC; ++i) sh.delrc_mol[i] = keep[2 + KMAXS + i];
-    const int sf = (int)keep[3 + KMAXS + KMAXC];
+    sh.delrc = ws.pend_delrc;
+    for (int i = 0; i <= KMAXS; ++i) sh.delE[i] = ws.pend_delE[i];
+    for (int i = 0; i <= KMAXC; ++i) sh.delrc_mol[i] = ws.pend_delrc_mol[i];
+    const int sf = ws.pend_surface;
     sh.srf_new = sf & 1;
     sh.srf_old = sf >> 1;
     if (sh.kind == FMC_MOVE_TRANSLATE || sh.kind == FMC_MOVE_ROTATE) sh.kind = FMC_MOVE_DISPLACE;
@@ -2263,12 +2261,13 @@ __global__ void __launch_bounds__(512) k_accept(const DevParams *__restrict__ Pp
   stage_out<NS>(W, w, sh);
 }
 
-// Full FP64 structure factor of every walker (ewald1, ewald.f:354-661): one
-// thread per (walker, k); species rows plus the total row.
+// Full FP64 structure factor of walkers w0 .. w0 + gridDim.y - 1 (ewald1,
+// ewald.f:354-661): one thread per (walker, k); species rows plus the total row.
 template <int NS>
-__global__ void __launch_bounds__(256) k_resum(const DevParams *__restrict__ Pp, DevWalkers W, double *drift) {
+__global__ void __launch_bounds__(256) k_resum(const DevParams *__restrict__ Pp, DevWalkers W, double *drift,
+                                               int w0) {
   const DevParams &P = *Pp;
-  const int w = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = w0 + (int)blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.nk) return;
   int lmn = P.klmn[k];
   double l = (double)(lmn & 255), m = (double)(((lmn >> 8) & 255) - 128), n = (double)(((lmn >> 16) & 255) - 128);

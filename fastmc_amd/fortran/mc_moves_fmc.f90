@@ -16,8 +16,21 @@
 !   FMC_SHIM_TRACE=<file>      one line per trial and per decision
 !
 ! Not mirrored to the host: the trial structure factors ckcsnew/ckssnew and
-! the running ckcsum/ckssum (S(k) lives on the device; gcmc.f only backs it
-! up and restores it around displacements, mc_displace gcmc.f:1863-1930).
+! the running ckcsum/ckssum (S(k) lives on the device).  gcmc.f also writes
+! the host arrays itself: it backs up and restores S(k) around displacements
+! (mc_displace gcmc.f:1863-1930; the device trial leaves S(k) unchanged, so
+! nothing is needed), and after a rejected swap it restores the coordinates,
+! nummols and S(k) of both guest kinds (gcmc.f:2416-2437) after the deletion
+! and the insertion were accepted through this module.  The module therefore
+! keeps a shadow of the molecules the device holds; before each trial it
+! compares the host's molecule lists with it and reloads a guest kind whose
+! list changed behind its back (fmc_load_guests, which re-sums S(k) in FP64).
+! An accept of an overlapping insertion (mc_swap and mc_switch accept it and
+! then reject the whole move) leaves the device as it is.  mc_switch with
+! three or more guest kinds restores host arrays between its own trials, with
+! the stale-S(k) energies of the reference in between: such runs are refused
+! at start-up.  With one or two guest kinds mc_switch never moves
+! (gcmc.f:2098).
 ! The surface-atom flags (surface_check, mc_moves.f:10-44) come from the
 ! device trial; for the molecule in place of a displacement that takes one
 ! extra zero-displacement trial, whose flag bit 1 is the position's.
@@ -102,10 +115,21 @@ module mc_moves
       type(c_ptr), value :: h
       integer(c_int), value :: w
     end function fmc_reject
+    integer(c_int) function fmc_load_guests(h, w, iguest, xyz, nmols, sumchg) bind(C, name='fmc_load_guests')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: w, iguest, nmols
+      real(c_double), intent(in) :: xyz(*)
+      real(c_double), intent(in) :: sumchg
+    end function fmc_load_guests
   end interface
 
   type(c_ptr), save :: fmc_h = c_null_ptr
   integer, save :: fmc_pending = 0          ! 1 while a device trial awaits accept/reject
+  logical, save :: fmc_pend_overlap = .false. ! the pending trial is an overlapping insertion
+  ! the molecules the device holds, per species (guests only): count and sites
+  integer, allocatable, save :: fmc_shn(:)
+  real(8), allocatable, save :: fmc_shx(:, :), fmc_shy(:, :), fmc_shz(:, :)
   integer, save :: fmc_disp_iguest = 0      ! guest_energy(lexisting) seen: molecule to displace
   integer, save :: fmc_disp_choice = 0
   integer, save :: fmc_trace_unit = 0
@@ -238,8 +262,24 @@ contains
     if (envst == 0 .and. envlen > 0) then
       if (env(1:5) == 'mixed') prec = 1
     endif
+    ! mc_switch moves molecules only with three or more guest kinds, and then
+    ! restores the host arrays between its own trials (see the header)
+    if (ntpguest >= 3) then
+      do g = 1, ntpguest
+        if (mcswif(g) > mcrotf(g)) then
+          write(*, '(a)') 'fastmc: switch moves with three or more guest types are not supported'
+          call fmc_fail('fmc_shim_init', 9001_c_int)
+        endif
+      enddo
+    endif
     rc = fmc_create(sys, 1_c_int, wk, prec, fmc_h)
     if (rc /= 0) call fmc_fail('fmc_create', rc)
+    allocate(fmc_shn(maxmls))
+    allocate(fmc_shx(size(molxxx, 2), maxmls), fmc_shy(size(molxxx, 2), maxmls), fmc_shz(size(molxxx, 2), maxmls))
+    fmc_shn = 0
+    do g = 1, ntpguest
+      call fmc_shadow_take(locguest(g))
+    enddo
     call get_environment_variable('FMC_SHIM_TRACE', env, envlen, envst)
     if (envst == 0 .and. envlen > 0) then
       open(newunit=fmc_trace_unit, file=env(1:envlen), status='replace', action='write')
@@ -338,6 +378,47 @@ contains
     enddo
   end subroutine poreblock_check
 
+  ! the device now holds the host's molecule list of species mol
+  subroutine fmc_shadow_take(mol)
+    integer, intent(in) :: mol
+    integer :: n
+    n = numatoms(mol) * nummols(mol)
+    fmc_shn(mol) = nummols(mol)
+    fmc_shx(1:n, mol) = molxxx(mol, 1:n)
+    fmc_shy(1:n, mol) = molyyy(mol, 1:n)
+    fmc_shz(1:n, mol) = molzzz(mol, 1:n)
+  end subroutine fmc_shadow_take
+
+  ! Before a trial: reload every guest kind whose host molecule list is not
+  ! the one the device holds (gcmc.f restored it, e.g. after a rejected swap).
+  subroutine fmc_sync(sumchg)
+    real(8), intent(in) :: sumchg
+    integer :: g, mol, n, a
+    integer(c_int) :: rc
+    logical :: same
+    real(c_double), allocatable :: xyz(:)
+    do g = 1, size(guestx, 1)
+      mol = locguest(g)
+      n = numatoms(mol) * nummols(mol)
+      same = nummols(mol) == fmc_shn(mol)
+      if (same) same = all(molxxx(mol, 1:n) == fmc_shx(1:n, mol)) .and. &
+                       all(molyyy(mol, 1:n) == fmc_shy(1:n, mol)) .and. &
+                       all(molzzz(mol, 1:n) == fmc_shz(1:n, mol))
+      if (same) cycle
+      allocate(xyz(3 * max(n, 1)))
+      do a = 1, n
+        xyz(3 * a - 2) = molxxx(mol, a)
+        xyz(3 * a - 1) = molyyy(mol, a)
+        xyz(3 * a) = molzzz(mol, a)
+      enddo
+      rc = fmc_load_guests(fmc_h, 0_c_int, int(g, c_int), xyz, int(nummols(mol), c_int), sumchg)
+      deallocate(xyz)
+      if (rc /= 0) call fmc_fail('fmc_load_guests', rc)
+      call fmc_shadow_take(mol)
+      if (fmc_tracing) write(fmc_trace_unit, '(a,2i8)') 'L', g, nummols(mol)
+    enddo
+  end subroutine fmc_sync
+
   ! -------------------------------------------------------------- trials --
   ! Initial loading to nguests molecules with random non-overlapping
   ! insertions, accepted regardless of energy (mc_moves.f:84-149).
@@ -396,12 +477,14 @@ contains
     natms = numatoms(mol)
     lnewsurf = .false.
     loverlap = .false.
+    call fmc_sync(sumchg)
     call fmc_zero_terms(maxmls)
     call gstlrcorrect(imcon, iguest, keyfce, ntpatm, maxvdw, delrc, volm, maxmls, .false.)
     call fmc_pack(natms, xyz)
     rc = fmc_trial_insert(fmc_h, 0_c_int, int(iguest, c_int), xyz, t)
     if (rc /= 0) call fmc_fail('fmc_trial_insert', rc)
     fmc_pending = 1
+    fmc_pend_overlap = t%overlap /= 0
     call fmc_log_trial(1, iguest, 0, natms, xyz, t)
     loverlap = t%overlap /= 0
     lnewsurf = btest(t%surface, 0)
@@ -448,6 +531,7 @@ contains
     ewld1eng = 0.d0
     call fmc_selfterms(mol, natms, alpha, epsq, lexisting, chgtmp, engsictmp)
     if (lexisting) then
+      call fmc_sync(sumchg)
       atmadd = 0
       do g = 1, iguest - 1
         atmadd = atmadd + numatoms(locguest(g)) * nummols(locguest(g))
@@ -468,6 +552,7 @@ contains
     rc = fmc_trial_displace(fmc_h, 0_c_int, int(iguest, c_int), int(fmc_disp_choice, c_int), xyz, t)
     if (rc /= 0) call fmc_fail('fmc_trial_displace', rc)
     fmc_pending = 1
+    fmc_pend_overlap = .false.
     fmc_disp_iguest = 0
     call fmc_log_trial(3, iguest, fmc_disp_choice, natms, xyz, t)
     loverlap = t%overlap /= 0
@@ -499,6 +584,7 @@ contains
     mol = locguest(iguest)
     natms = numatoms(mol)
     linitsurf = .false.
+    call fmc_sync(sumchg)
     call fmc_zero_terms(maxmls)
     call gstlrcorrect(imcon, iguest, keyfce, ntpatm, maxvdw, delrc, volm, maxmls, .true.)
     do i = 1, natms
@@ -510,6 +596,7 @@ contains
     rc = fmc_trial_delete(fmc_h, 0_c_int, int(iguest, c_int), int(choice, c_int), t)
     if (rc /= 0) call fmc_fail('fmc_trial_delete', rc)
     fmc_pending = 1
+    fmc_pend_overlap = .false.
     call fmc_log_trial(2, iguest, choice, natms, xyz, t)
     linitsurf = btest(t%surface, 1)
     call fmc_selfterms(mol, natms, alpha, epsq, .true., chgtmp, engsictmp)
@@ -532,10 +619,22 @@ contains
     real(8) :: delrc, sumchg, engsictmp, chgtmp
     integer :: mol, natms, kk, first, last, i, s
     integer(c_int) :: rc
+    logical :: on_device
+    on_device = .false.
     if (fmc_pending /= 0) then
-      rc = fmc_accept(fmc_h, 0_c_int)
-      if (rc /= 0) call fmc_fail('fmc_accept', rc)
+      if (fmc_pend_overlap) then
+        ! an overlapping insertion accepted on the way to rejecting a swap or
+        ! switch: the device keeps its state, the host list is reloaded if
+        ! the move's restores leave it different
+        rc = fmc_reject(fmc_h, 0_c_int)
+        if (rc /= 0) call fmc_fail('fmc_reject', rc)
+      else
+        rc = fmc_accept(fmc_h, 0_c_int)
+        if (rc /= 0) call fmc_fail('fmc_accept', rc)
+        on_device = .true.
+      endif
       fmc_pending = 0
+      fmc_pend_overlap = .false.
       if (fmc_tracing) write(fmc_trace_unit, '(a)') 'D 1'
     endif
     mol = locguest(iguest)
@@ -590,6 +689,7 @@ contains
       enddo
       call condense(totatm, ntpfram, ntpguest)
     endif
+    if (on_device) call fmc_shadow_take(mol)
   end subroutine accept_move
 
   subroutine fmc_count_site(mol, s, dn)
@@ -614,8 +714,10 @@ contains
       rc = fmc_reject(fmc_h, 0_c_int)
       if (rc /= 0) call fmc_fail('fmc_reject', rc)
       fmc_pending = 0
+      fmc_pend_overlap = .false.
       if (fmc_tracing) write(fmc_trace_unit, '(a)') 'D 0'
     endif
+    if (swap .and. fmc_tracing) write(fmc_trace_unit, '(a)') 'R'
     delE(:) = 0.d0
     if (insert .or. delete) then
       engsic = engsicorig

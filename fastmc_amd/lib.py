@@ -19,6 +19,8 @@ LIB_PATH = os.path.join(HERE, "libfastmc_hip.so")
 
 FMC_FP64 = 0
 FMC_MIXED = 1
+FMC_MOVE_INSERT, FMC_MOVE_DELETE, FMC_MOVE_DISPLACE, FMC_MOVE_TRANSLATE = 1, 2, 3, 4
+FMC_MOVE_ROTATE, FMC_MOVE_JUMP, FMC_MOVE_SWAP, FMC_MOVE_SWITCH = 5, 6, 7, 8
 FMC_STATS_COUNT = 1
 FMC_STATS_CLOSE = 2
 STAT_NAMES = ("<N>", "<E>", "<E*N>", "<N*N>", "<E*E>", "<NF>", "Qst", "Cv")
@@ -30,7 +32,8 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
            "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob",
-           "fmc_set_numguests", "fmc_get_numguests")
+           "fmc_set_numguests", "fmc_get_numguests", "fmc_load_guests",
+           "fmc_set_trace_walker")
 
 _lib = None
 
@@ -128,6 +131,18 @@ class Handle:
         _chk("fmc_single_point", self.L.fmc_single_point(self.h, w, out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def load_guests(self, iguest: int, xyz, w: int = 0, sumchg=None):
+        """Replace walker w's molecules of guest type iguest by the sites xyz
+        ([nmols * natoms][3], molecule order) and rebuild counts, sums, S(k)
+        and energies from the coordinates (fmc_load_guests)."""
+        nat = self.sysm.guests[iguest - 1].natoms
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        if len(x) % nat:
+            raise ValueError("xyz is not a whole number of molecules")
+        q = C.byref(C.c_double(sumchg)) if sumchg is not None else None
+        _chk("fmc_load_guests", self.L.fmc_load_guests(self.h, C.c_int(w), C.c_int(iguest),
+                                                       x.ctypes.data_as(C.c_void_p), C.c_int(len(x) // nat), q))
+
     def full_resum(self, w: int = -1) -> float:
         d = C.c_double()
         _chk("fmc_full_resum", self.L.fmc_full_resum(self.h, w, C.byref(d)))
@@ -165,6 +180,10 @@ class Handle:
         ptr = tr.ctypes.data_as(C.c_void_p) if trace else None
         _chk("fmc_run", self.L.fmc_run(self.h, C.c_int(nsteps), ptr, C.c_int(int(count_stats))))
         return tr
+
+    def set_trace_walker(self, w: int):
+        """Walker whose steps run(trace=True) records."""
+        _chk("fmc_set_trace_walker", self.L.fmc_set_trace_walker(self.h, C.c_int(w)))
 
     def run_graph(self, nsweeps: int, nsteps_per_sweep: int, count_stats: bool = False):
         _chk("fmc_run_graph", self.L.fmc_run_graph(self.h, C.c_int(nsweeps), C.c_int(nsteps_per_sweep),

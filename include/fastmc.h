@@ -168,6 +168,19 @@ int fmc_single_point(fmc_handle *h, int w, double out[5]);
  * and return the largest |S_incremental - S_resum| found before overwriting. */
 int fmc_full_resum(fmc_handle *h, int w, double *max_drift);
 
+/* Replace the molecules of guest type iguest in walker w by nmols molecules
+ * whose sites are xyz [3*natoms*nmols] (molecule order), the way gcmc.f
+ * writes molxxx/y/z and nummols back after a rejected swap
+ * (gcmc.f:2416-2437) or switch (gcmc.f:2238-2262), then rebuild what follows
+ * from the coordinates as the start-up does (gcmc.f:542-582): molecule and
+ * atom-type counts, the guest kind's charge and self-interaction sums, the
+ * tail sums (lrcorrect, vdw_terms.f:468-684), every species row and the total
+ * of S(k) in FP64 (ewald1, ewald.f:354-661) and the per-species energies
+ * (single_point).  sumchg: the system charge to hold from now on, or NULL to
+ * keep the walker's (mc_swap does not restore it).  Surface counts are kept.
+ * A pending seam trial of walker w is discarded. */
+int fmc_load_guests(fmc_handle *h, int w, int iguest, const double *xyz, int nmols, const double *sumchg);
+
 /* ---- one trial at a time (the mc_moves.f seam) ------------------------- */
 
 /* insertion (mc_moves.f:150-276): xyz = [3*natoms] trial coordinates (newx/y/z). */
@@ -195,7 +208,8 @@ int fmc_reset_stats(fmc_handle *h);
 /* Advance every walker by nsteps GCMC steps on the device: move choice,
  * trial energy, Metropolis test (energy_eval, utility_pack.f:3637-3693) and
  * accept/reject, with the duni generator (utility_pack.f:2768-2857) per
- * walker.  If trace is non-NULL, walker 0 writes one record per step
+ * walker.  If trace is non-NULL, one walker (walker 0 unless
+ * fmc_set_trace_walker chose another) writes one record per step
  * (trace[nsteps]).  count_stats: bit 0 (FMC_STATS_COUNT) makes every step a
  * production step (gcmc.f:1050-1130: the N/E moments and the averaging
  * windows set by fmc_set_sampling); bit 1 (FMC_STATS_CLOSE) closes the open
@@ -208,6 +222,10 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats);
 /* Same, captured once into a HIP graph of `sweeps_per_graph` launches and
  * replayed; nsweeps launches of nsteps_per_sweep steps in total. */
 int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_stats);
+
+/* The walker whose steps a traced fmc_run records (no counterpart in the
+ * reference, which runs one chain per process). */
+int fmc_set_trace_walker(fmc_handle *h, int w);
 
 /* Time the last fmc_run / fmc_run_graph dispatches with HIP events on the
  * handle's stream: total milliseconds and launch count. */

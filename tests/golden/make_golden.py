@@ -30,6 +30,7 @@ import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
@@ -38,7 +39,6 @@ sys.path.insert(0, REPO)
 from fastmc_amd import inputs as I  # noqa: E402
 
 DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver.x")
-SCRATCH = os.path.join(REPO, "_work", "golden_runs")
 
 CALF_CONTROL = """GCMC Run
 temperature  293.000000
@@ -210,13 +210,18 @@ def main() -> None:
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-C", os.path.join(REPO, "oracle"), "ref"])
     ensure_calf20_inputs()
-    os.makedirs(SCRATCH, exist_ok=True)
-    only = set(sys.argv[1:])  # optional: regenerate just the named cases
+    scratch = tempfile.mkdtemp(prefix="fmc_golden_")
+    try:
+        run_cases(scratch, set(sys.argv[1:]))  # optional arguments: regenerate just the named cases
+    finally:
+        shutil.rmtree(scratch, ignore_errors=True)
+
+
+def run_cases(scratch: str, only: set) -> None:
     for name, writer, nsteps in CASES:
         if only and name not in only:
             continue
-        d = os.path.join(SCRATCH, name)
-        shutil.rmtree(d, ignore_errors=True)
+        d = os.path.join(scratch, name)
         if writer is None:
             write_calf20(d)
         else:

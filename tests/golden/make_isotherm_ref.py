@@ -4,8 +4,10 @@ travels):
 
     make -C oracle ref                   # builds oracle/_ref/gcmc_ser.x from the reference sources
     python tests/golden/make_isotherm_ref.py [--runs 8] [--eq 100000] [--prod 100000]
+    python tests/golden/make_isotherm_ref.py --runs 16 --temperature 273 --pressures 1.0 --append
 
-For each pressure the config-0 inputs (fastmc_amd.inputs.config0, T = 293 K)
+For each state point (T = 293 K at 0.1, 1 and 10 bar by default; --append adds
+points to the existing fixture) the config-0 inputs (fastmc_amd.inputs.config0)
 are written to a scratch directory per run and the reference's serial GCMC
 program runs there, `runs` times concurrently on the host's cores (the
 reference seeds duni from the clock, so the runs are independent chains; the
@@ -50,16 +52,24 @@ def main():
     ap.add_argument("--runs", type=int, default=8)
     ap.add_argument("--eq", type=int, default=100000)
     ap.add_argument("--prod", type=int, default=100000)
+    ap.add_argument("--temperature", type=float, default=293.0)
+    ap.add_argument("--pressures", type=float, nargs="*", default=list(PRESSURES_BAR))
+    ap.add_argument("--append", action="store_true", help="add the points to the existing fixture")
     args = ap.parse_args()
     if not os.path.exists(EXE):
         sys.exit(f"{EXE} missing: run `make -C oracle ref` first")
-    result = {"system": "config 0 (424-atom framework, CO2), T = 293 K", "equilibration": args.eq,
-              "production": args.prod, "points": []}
-    for p in PRESSURES_BAR:
+    path = os.path.join(HERE, "isotherm_ref_cfg0.json")
+    if args.append:
+        result = json.load(open(path))
+        assert (result["equilibration"], result["production"]) == (args.eq, args.prod)
+    else:
+        result = {"system": "config 0 (424-atom framework, CO2)", "equilibration": args.eq,
+                  "production": args.prod, "points": []}
+    for p in args.pressures:
         dirs, procs = [], []
         for r in range(args.runs):
             d = tempfile.mkdtemp(prefix=f"fmc_iso_{p}_{r}_")
-            I.config0(d, pressure_bar=p)
+            I.config0(d, pressure_bar=p, temperature=args.temperature)
             ctl = open(os.path.join(d, "CONTROL")).read()
             ctl = re.sub(r"(?m)^steps .*$", f"steps    {args.prod}", ctl)
             ctl = re.sub(r"(?m)^equilibration .*$", f"equilibration    {args.eq}", ctl)
@@ -76,11 +86,12 @@ def main():
         n = [x["mean_n"] for x in runs]
         mu = sum(n) / len(n)
         sd = (sum((x - mu) ** 2 for x in n) / max(1, len(n) - 1)) ** 0.5
-        result["points"].append({"pressure_bar": p, "runs": runs, "mean_n": mu, "sd_between_runs": sd,
+        result["points"].append({"temperature": args.temperature, "pressure_bar": p, "runs": runs, "mean_n": mu,
+                                 "sd_between_runs": sd,
                                  "mean_stderr_n": sum(x["stderr_n"] for x in runs) / len(runs)})
-        print(p, "bar:", mu, "+/-", sd, "between runs; per-run stderr", result["points"][-1]["mean_stderr_n"],
-              flush=True)
-    json.dump(result, open(os.path.join(HERE, "isotherm_ref_cfg0.json"), "w"), indent=1)
+        print(args.temperature, "K", p, "bar:", mu, "+/-", sd, "between runs; per-run stderr",
+              result["points"][-1]["mean_stderr_n"], flush=True)
+    json.dump(result, open(path, "w"), indent=1)
 
 
 if __name__ == "__main__":

@@ -233,3 +233,61 @@ def test_gpu_mixed_precision_du_error(name):
     assert errs.max() <= 1e-6
     for hh in (hd, hm, hm2, hd2):
         hh.close()
+
+
+def test_gpu_seam_interleaved_walkers():
+    """Trials of two walkers pending at once: each accept applies its own
+    walker's trial (the accept data are kept per walker)."""
+    Handle, FP64, _ = _lib()
+    fx = fixture("cfg0_p10")
+    sysm = case_system("cfg0_p10")
+    seeds = [tuple(fx["seeds"]), (7, 11, 13, 17)]
+    say("seam, two walkers interleaved")
+    orc = [Oracle(sysm, seed=s) for s in seeds]
+    for o in orc:
+        o.run(300)
+    h = Handle(sysm, precision=FP64, seeds=seeds, n=2)
+    h.run(300)
+    for w in range(2):
+        np.testing.assert_allclose(h.guests(1, w=w), orc[w].guests(1), rtol=0, atol=1e-9)
+    T = sysm.temperature
+    tmpl = sysm.guests[0].template
+    rng = np.random.default_rng(5)
+    naccept = 0
+    for it in range(60):
+        trials = []
+        for w in range(2):
+            kind = int(rng.integers(0, 3))
+            og = orc[w].guests(1)
+            nm = len(og) // 3
+            com = rng.uniform(0, sysm.cell[0], 3)
+            if kind == 0 or nm == 0:
+                xyz = tmpl @ _rot(rng.uniform()).T + com
+                a, b = orc[w].trial_insert(1, xyz), h.trial_insert(1, xyz, w=w)
+            elif kind == 1:
+                c = 1 + int(rng.integers(0, nm))
+                a, b = orc[w].trial_delete(1, c), h.trial_delete(1, c, w=w)
+            else:
+                c = 1 + int(rng.integers(0, nm))
+                mol = og[3 * (c - 1): 3 * c]
+                xyz = mol + rng.uniform(-0.5, 0.5, 3)
+                a, b = orc[w].trial_displace(1, c, xyz), h.trial_displace(1, c, xyz, w=w)
+            assert a.overlap == b.overlap
+            if not a.overlap or kind == 1:
+                assert abs(a.estep - b.estep) <= du_tol(a.estep, T), (it, w, kind, a.estep, b.estep)
+            trials.append(a.overlap)
+        # decide both after both trials ran: walker 1's trial must not leak into walker 0's accept
+        for w in range(2):
+            if not trials[w] and rng.uniform() < 0.6:
+                assert orc[w].accept() == 0
+                h.accept(w=w)
+                naccept += 1
+            else:
+                orc[w].reject()
+                h.reject(w=w)
+    assert naccept > 20
+    for w in range(2):
+        np.testing.assert_allclose(h.guests(1, w=w), orc[w].guests(1), rtol=0, atol=1e-9)
+        np.testing.assert_allclose(np.array(h.stats(w).energy[:2]), orc[w].energy(), rtol=1e-9, atol=1e-8)
+        np.testing.assert_allclose(h.sk(w), orc[w].sk(), rtol=0, atol=1e-9)
+    h.close()

@@ -43,13 +43,18 @@ def test_oracle_tables_and_sk_match_reference(name):
     for k, i, v in fx["vvv"]:
         col = o.table(1, k)
         assert abs(col[i - 1] - v) <= 1e-13 * max(abs(v), 1e-30)
+    # lrcorrect's steadd (vdw_terms.f:587-588,639-645), indexed by the type
+    # pair's triangular index; the fixture lists steadd(1..ntpvdw) beside the FIELD rows
     stead = o.table(2, n=len(fx["lstvdw"]))
-    for k, kind, eps, sig, st in fx["pairs"]:
-        pass  # steadd is indexed by type pair; checked through the tail energies of the traces
+    assert len(fx["pairs"]) > 0
+    for ll, kind, eps, sig, st in fx["pairs"]:
+        assert abs(stead[ll - 1] - st) <= 1e-14 * abs(st), (ll, stead[ll - 1], st)
+    # total S(k) right after single_point at the sampled reference k indices (ewald1)
     sk = o.sk()
+    assert len(fx["sk0"]) > 10
     for k, c, s, c2, s2 in fx["sk0"]:
-        # S(k) right after single_point (fixture rows are sampled)
-        pass
+        assert abs(sk[k - 1, 0] - c) <= 1e-12 * max(1.0, abs(c)), (k, sk[k - 1, 0], c)
+        assert abs(sk[k - 1, 1] - s) <= 1e-12 * max(1.0, abs(s)), (k, sk[k - 1, 1], s)
     assert o.newld == fx["sizes"]["newld"] and o.mxegrd == fx["sizes"]["mxegrd"]
 
 
