@@ -707,11 +707,13 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   // this many accepted moves; the FP64 mode follows the reference (never)
   P.resum_every = precision == FMC_MIXED ? 1000 : 0;
   if (getenv("FMC_RESUM_EVERY")) P.resum_every = std::max(0, atoi(getenv("FMC_RESUM_EVERY")));
-  // guest capacities: enough for a dense liquid in the cell
+  // guest capacities: one molecule per 40 A^3 of cell, denser than a liquid
+  // of small molecules (liquid CO2 holds one per ~66 A^3); a walker that
+  // fills its slots stops inserting and sets overflow bit 0
   int slot = 0;
   for (int g = 1; g <= ng; ++g) {
     int mol = P.locguest[g];
-    int cap = (int)std::ceil(P.volm / 30.0);
+    int cap = (int)std::ceil(P.volm / 40.0);
     cap = std::max(64, std::min(cap, 8192));
     cap = std::max(cap, sys->species[mol - 1].nmols + 64);
     if (const char *e = getenv("FMC_GUEST_CAP")) cap = std::max(sys->species[mol - 1].nmols + 1, atoi(e));
@@ -824,6 +826,29 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       }
     }
     P.nvcol = ncol;
+    // the LJ wall segment of mixed mode: from just below the overlap distance
+    // (closer pairs overlap) up to the last entry of any guest-site column
+    // with |V| above 0.5 kcal/mol, where FP32 storage alone moves a trial
+    // energy by more than 1e-7 kT
+    {
+      const double vwall = 0.5 * 418.4;  // internal energy units (10 J/mol)
+      P.lj_lo = std::max(0, (int)std::floor(std::sqrt(P.overlap2) * P.rdlrpot) - 1);
+      int hi = P.lj_lo + 1;
+      for (int g = 1; g <= ng; ++g)
+        for (int s2 = 0; s2 < P.natoms[P.locguest[g]]; ++s2)
+          for (int tj = 0; tj <= KTYPE; ++tj) {
+            const int c = P.pairidx[P.site_t[P.locguest[g]][s2]][tj];
+            if (c < 0) continue;
+            for (int i = P.vstride - 1; i > hi; --i)
+              if (std::fabs(vvv[(size_t)c * P.vstride + i]) > vwall) {
+                hi = i + 1;
+                break;
+              }
+          }
+      P.lj_hi = std::min(hi, mxegrd);
+      P.lj_n64 = P.lj_hi + 2 - P.lj_lo;
+      P.lj_n32 = (P.lj_lo + 2) + (P.vstride - P.lj_hi);
+    }
     P.lds_all = 0;
     if (!getenv("FMC_NO_LDS_ALL") && cols_ok) {
       size_t o = base_end;
@@ -833,21 +858,27 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       o = al(o + (size_t)12 * P.nk);
       int og = (int)o;
       o = al(o + (size_t)24 * P.gatoms_total);
-      int ov = (int)o;
-      if (precision == FMC_MIXED) o = al(o + (size_t)4 * ncol * P.vstride);
+      int ov64 = (int)o, ov = (int)o;
+      if (precision == FMC_MIXED) {
+        const size_t nc = (size_t)std::max(ncol, 1);
+        o = al(o + (size_t)8 * nc * P.lj_n64);
+        ov = (int)o;
+        o = al(o + (size_t)4 * nc * P.lj_n32);
+      }
       if (o <= budget) {
         P.lds_all = 1;
         P.off_fw = ofw;
         P.off_k = ok;
         P.off_g = og;
+        P.off_vvv64 = ov64;
         P.off_vvv = ov;
         off = o;
       }
     }
     P.lds_bytes = (int)al(off);
     if (getenv("FMC_VERBOSE"))
-      fprintf(stderr, "fastmc: LDS plan %d bytes, all-in-LDS %d, nsite %d, LJ columns %d, guest slots %d\n",
-              P.lds_bytes, P.lds_all, h->nsite, ncol, P.gatoms_total);
+      fprintf(stderr, "fastmc: LDS plan %d bytes, all-in-LDS %d, nsite %d, LJ columns %d (wall %d..%d), guest slots %d\n",
+              P.lds_bytes, P.lds_all, h->nsite, ncol, P.lj_lo, P.lj_hi, P.gatoms_total);
   }
   // ---- device buffers
   HIPCHK(hipStreamCreate(&h->stream));

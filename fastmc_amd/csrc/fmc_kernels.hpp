@@ -614,8 +614,56 @@ __device__ void swap_insert_setup(const DevParams &P, Smem<NS> &sh) {
 // are large and cancel); T is the precision of the LJ interpolation, whose
 // fractional grid position is formed in FP64 first.  col < 0: no LJ term;
 // ovl2 < 0: a padding site (never overlaps, chgea 0).
+// Where a pair's LJ table value comes from: FP64 mode reads v64 (column
+// stride s64) everywhere; mixed mode reads the wall entries [lo, hi) from v64
+// (table index b64 at v64[0]) and every other entry from v32, whose columns
+// either hold the whole table (packed false) or the entries 0 .. lo + 1 then
+// hi .. end (packed true, the LDS copy).
+struct LJTab {
+  const double *v64;
+  const float *v32;
+  int s64, s32, b64, lo, hi;
+  bool packed;
+};
+
+__device__ inline LJTab lj_global(const DevParams &P, bool mixed) {
+  LJTab t;
+  t.v64 = P.vvv;
+  t.v32 = P.vvv32;
+  t.s64 = t.s32 = P.vstride;
+  t.b64 = 0;
+  t.lo = mixed ? P.lj_lo : 0;
+  t.hi = mixed ? P.lj_hi : 0x7fffffff;
+  t.packed = false;
+  return t;
+}
+
+__device__ inline LJTab lj_lds(const DevParams &P, const unsigned char *lds) {
+  LJTab t;
+  t.v64 = (const double *)(lds + P.off_vvv64);
+  t.v32 = (const float *)(lds + P.off_vvv);
+  t.s64 = P.lj_n64;
+  t.s32 = P.lj_n32;
+  t.b64 = P.lj_lo;
+  t.lo = P.lj_lo;
+  t.hi = P.lj_hi;
+  t.packed = true;
+  return t;
+}
+
+// One guest-site / atom pair of insertion / guest_energy / deletion: minimum
+// image, surface and overlap tests (surface_check, mc_moves.f:10-44: bit 0
+// of ovl rsq < overlap^2, bit 1 rsq < sg2 = (surftol + sigma)^2 for a frozen
+// atom, sg2 < 0 otherwise), ewald2 and srfrce terms (ewald.f:689-726,
+// vdw_terms.f:418-460).  Branch-free, so that the
+// pairs of one atom with all trial sites form one straight-line block:
+// table indices are clamped and out-of-range terms selected away.  Geometry,
+// the real-space Coulomb term and the LJ interpolation are FP64 in both
+// modes (pair Coulomb terms are large and cancel); T = float takes the LJ
+// table values off the repulsive wall from FP32 storage.  col < 0: no LJ
+// term; ovl2 < 0: a padding site (never overlaps, chgea 0).
 template <typename T>
-__device__ inline void pair_eval(const DevParams &P, const double *erc, const T *vvv, const double *xs, double chgea,
+__device__ inline void pair_eval(const DevParams &P, const double *erc, const LJTab &lj, const double *xs, double chgea,
                                  int col, double ovl2, double sg2, double xj, double yj, double zj, double qj,
                                  double &e2, double &ev, int &ovl) {
   double dx = xs[0] - xj, dy = xs[1] - yj, dz = xs[2] - zj;
@@ -636,12 +684,30 @@ __device__ inline void pair_eval(const DevParams &P, const double *erc, const T 
   e2 = (in && fabs(chgprd) > 1e-10) ? ec : 0.0;
   double xv = r * P.rdlrpot;
   int lv = min((int)xv, lmax);
-  T pp = (T)(xv - (double)lv);
-  const T *vv = vvv + (size_t)(col < 0 ? 0 : col) * P.vstride + lv;
-  T w0 = vv[0], w1 = vv[1], w2 = vv[2];
-  T u1 = w0 + (w1 - w0) * pp;
-  T u2 = w1 + (w2 - w1) * (pp - (T)1);
-  double el = (double)(u1 + (u2 - u1) * pp * (T)0.5);
+  double pp = xv - (double)lv;
+  const int c = col < 0 ? 0 : col;
+  double w0, w1, w2;
+  if constexpr (sizeof(T) == 8) {
+    const double *vv = lj.v64 + (size_t)c * lj.s64 + lv;
+    w0 = vv[0];
+    w1 = vv[1];
+    w2 = vv[2];
+  } else {
+    // both stencils are read (indices clamped into their segments), one is kept
+    const bool wall = lv >= lj.lo && lv < lj.hi;
+    const int i64 = min(max(lv, lj.lo), lj.hi - 1) - lj.b64;
+    const int i32 = !lj.packed ? lv : (lv < lj.lo ? lv : max(lv, lj.hi) - lj.hi + lj.lo + 2);
+    const double *a = lj.v64 + (size_t)c * lj.s64 + i64;
+    const float *b = lj.v32 + (size_t)c * lj.s32 + i32;
+    const double a0 = a[0], a1 = a[1], a2 = a[2];
+    const float b0 = b[0], b1 = b[1], b2 = b[2];
+    w0 = wall ? a0 : (double)b0;
+    w1 = wall ? a1 : (double)b1;
+    w2 = wall ? a2 : (double)b2;
+  }
+  double u1 = w0 + (w1 - w0) * pp;
+  double u2 = w1 + (w2 - w1) * (pp - 1.0);
+  double el = u1 + (u2 - u1) * pp * 0.5;
   ev = (in && col >= 0) ? el : 0.0;
 }
 
@@ -773,11 +839,19 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
       kl[i] = P.klmn[i];
     }
     if constexpr (sizeof(T) == 4) {
-      float *dst = (float *)(lds + P.off_vvv);
-      const int n = P.nvcol * P.vstride;
-      for (int i = tid; i < n; i += nt) {
-        int c = i / P.vstride, r = i - c * P.vstride;
-        dst[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + r];
+      // LJ columns: the wall entries in FP64, the rest in FP32 (LJTab, packed)
+      double *d64 = (double *)(lds + P.off_vvv64);
+      const int n64 = P.nvcol * P.lj_n64;
+      for (int i = tid; i < n64; i += nt) {
+        int c = i / P.lj_n64, r = i - c * P.lj_n64;
+        d64[i] = P.vvv[(size_t)P.vcol_src[c] * P.vstride + P.lj_lo + r];
+      }
+      float *d32 = (float *)(lds + P.off_vvv);
+      const int n32 = P.nvcol * P.lj_n32, na = P.lj_lo + 2;
+      for (int i = tid; i < n32; i += nt) {
+        int c = i / P.lj_n32, r = i - c * P.lj_n32;
+        int e = r < na ? r : r - na + P.lj_hi;
+        d32[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + e];
       }
     }
     double *lx = (double *)(lds + P.off_g), *ly = lx + P.gatoms_total, *lz = ly + P.gatoms_total;
@@ -889,10 +963,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   const double *srf = (const double *)(lds + P.off_cst + CST_SRF) + g * KSITE * (KTYPE + 1);
   const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
   const int *lst = (const int *)(lds + P.off_cst + CST_LST);
-  const T *vvv;
-  if constexpr (LV) vvv = (const T *)(lds + P.off_vvv);
-  else if constexpr (sizeof(T) == 8) vvv = (const T *)P.vvv;
-  else vvv = (const T *)P.vvv32;
+  LJTab vvv;
+  if constexpr (LV) vvv = lj_lds(P, lds);
+  else vvv = lj_global(P, sizeof(T) == 4);
   // NSITE > 0: site charges and overlap thresholds in registers; padding
   // sites (s >= natms) carry charge 0, no LJ column, no overlap or surface
   // test, so whatever their LDS coordinates hold is selected away.  The site

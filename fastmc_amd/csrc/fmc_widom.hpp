@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256) k_widom(const DevParams *__restrict__ Pp,
   const WidomConst c = *K;
   // LJ from the FP64 tables in both modes: a grid visits every close contact,
   // where FP32 table entries alone move estep by ~1e-6 kcal/mol
-  const double *vvv = P.vvv;
+  const LJTab vvv = lj_global(P, false);
   const double *fx = P.fw_per_walker ? W.fwx + (size_t)A.w * P.nfw : P.fwx;
   const double *fy = P.fw_per_walker ? W.fwy + (size_t)A.w * P.nfw : P.fwy;
   const double *fz = P.fw_per_walker ? W.fwz + (size_t)A.w * P.nfw : P.fwz;

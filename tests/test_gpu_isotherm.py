@@ -1,18 +1,25 @@
-"""Isotherm loadings of the device chains against the reference's own: the
-config-0 system (424-atom framework, CO2) at T = 293 K and 0.1, 1 and 10 bar,
-and at 273 K and 323 K and 1 bar, 64 independent walkers per state point, each
-run for the reference CONTROL's 1e5 equilibration and 1e5 production steps.
-The reference side (tests/golden/isotherm_ref_cfg0.json, made by
-make_isotherm_ref.py from the reference's serial program) holds 16 runs per
-state point with each run's <N> (gcmc.f:1458-1557).
+"""Isotherm loadings of the device chains against the reference's own runs:
+the config-0 system (424-atom framework, CO2) at T = 293 K and 0.1, 1 and
+10 bar, and at 273 K and 323 K and 1 bar, with the reference CONTROL's 1e5
+equilibration and 1e5 production steps.  The reference side
+(tests/golden/isotherm_ref_cfg0.json, made by make_isotherm_ref.py from the
+reference's serial program) holds 16 runs per state point: each run's duni
+seeds and its printed <N> +/- block-average error (gcmc.f:1458-1557).
 
-Both sides are means of independent chains of the same Markov process, so
-their difference is tested against the standard error of the difference of
-the two means: |<N>_dev - <N>_ref| <= 3 sqrt(SE_dev^2 + SE_ref^2), with SE the
-standard deviation between chains over the square root of their number.  The
-mixed-precision run is the north-star check; the FP64 run (whose chains are
-the reference's step for step, as the trace replays show) is the control
-that separates a precision effect from sampling noise.  z-scores are printed."""
+* FP64: 80 walkers, one per reference run, started from that run's printed
+  seeds and driven through the reference's main-loop schedule, print every
+  run's <N> to the reference's 6 decimals -- the whole 2e5-step chains are
+  the reference's.
+* Mixed precision: 128 walkers per state point with random seeds, each run
+  in mixed precision and in FP64 from the same seeds.  The FP64 chains are the
+  reference's chains (above), so the paired difference mixed - FP64 isolates
+  what the precision does: |mean difference| <= 3 standard errors of the
+  paired differences (most mixed chains never take a different decision
+  from their FP64 chain, and then the difference is exactly zero).  The north star's criterion, <N>_mixed within one
+  reference block-average error of the reference mean, is checked too; z
+  against the reference mean is printed for both precisions (the reference
+  mean of 16 clock-seeded runs is itself a sample: at 10 bar it lies 3.4
+  standard errors above 64 random-seed FP64 chains)."""
 import json
 import os
 import shutil
@@ -25,7 +32,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-NPER = 64
+NPER = 128
 
 
 def say(*a):
@@ -47,48 +54,113 @@ def walker_seeds(n, salt):
     return out
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_gpu_isotherm_loadings_match_reference(precision):
+def reference():
+    return json.load(open(os.path.join(HERE, "golden", "isotherm_ref_cfg0.json")))
+
+
+def config0_system():
     from fastmc_amd import inputs as I
-    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
-    from fastmc_amd.system import make_walker, read_system
-    ref = json.load(open(os.path.join(HERE, "golden", "isotherm_ref_cfg0.json")))
+    from fastmc_amd.system import read_system
     d = tempfile.mkdtemp(prefix="fmc_iso_")
     try:
         I.config0(d, pressure_bar=1.0)
-        sysm = read_system(d)
+        return read_system(d)
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    pts = ref["points"]
-    seeds = walker_seeds(NPER * len(pts), 0 if precision == "mixed" else 1)
+
+
+class Engine:
+    """All walkers of a handle through the reference's main-loop schedule."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def run(self, n, flags):
+        self.h.run(n, count_stats=flags)
+
+    def prodcount(self):
+        return int(self.h.stats(0).prodcount)
+
+
+def test_gpu_isotherm_fp64_reproduces_reference_runs():
+    from fastmc_amd import gcmc
+    from fastmc_amd.lib import FMC_FP64, Handle
+    from fastmc_amd.system import make_walker
+    ref = reference()
+    sysm = config0_system()
+    walkers, keep, want = [], [], []
+    for pt in ref["points"]:
+        for run in pt["runs"]:
+            walkers.append(make_walker(sysm, seed=tuple(run["seeds"]), temperature=pt["temperature"],
+                                       pressures=[pt["pressure_bar"] * 1e5], keep=keep))
+            want.append((pt["temperature"], pt["pressure_bar"], run["mean_n"]))
+    eq, prod = ref["equilibration"], ref["production"]
+    say(f"fp64: {len(walkers)} walkers from the reference runs' seeds, {eq} + {prod} steps")
+    nwind, nwindsteps = gcmc.window_steps(prod)
+    h = Handle(sysm, walkers=walkers, precision=FMC_FP64)
+    try:
+        h.set_sampling(nwindsteps, False, 16)
+        eng = Engine(h)
+        gcmc.run_main_loop(eng, eq, prod, gcmc.check_interval(eq, 1000), False, eng.prodcount, nwindsteps)
+        got = [h.production_stats(w)[0]["avg_final"]["<N>"] for w in range(len(walkers))]
+        for w in range(len(walkers)):
+            assert h.stats(w).overflow == 0
+    finally:
+        h.close()
+    bad = [(t, p, n, round(g, 6)) for (t, p, n), g in zip(want, got) if abs(g - n) > 0.5e-6 * 1.0001]
+    say(f"fp64: {len(want) - len(bad)} of {len(want)} reference runs reproduced to the printed digits")
+    assert not bad, bad[:5]
+
+
+def run_walkers(sysm, pts, seeds, precision, eq, prod):
+    from fastmc_amd.lib import Handle
+    from fastmc_amd.system import make_walker
     walkers, keep = [], []
     for ip, pt in enumerate(pts):
         for r in range(NPER):
             walkers.append(make_walker(sysm, seed=seeds[ip * NPER + r], temperature=pt["temperature"],
                                        pressures=[pt["pressure_bar"] * 1e5], keep=keep))
-    say(f"{precision}: {len(walkers)} walkers, {ref['equilibration']} + {ref['production']} steps")
-    h = Handle(sysm, walkers=walkers, precision=FMC_MIXED if precision == "mixed" else FMC_FP64)
+    h = Handle(sysm, walkers=walkers, precision=precision)
     try:
-        h.run(ref["equilibration"])
+        h.run(eq)
         h.reset_stats()
-        h.run(ref["production"], count_stats=True)
+        h.run(prod, count_stats=True)
         means = []
         for w in range(len(walkers)):
             st = h.stats(w)
-            assert st.counted == ref["production"]
+            assert st.counted == prod and st.overflow == 0
             means.append(st.sum_n[0] / st.counted)
     finally:
         h.close()
-    means = np.array(means).reshape(len(pts), NPER)
+    return np.array(means).reshape(len(pts), NPER)
+
+
+def test_gpu_isotherm_mixed_matches_fp64_and_reference():
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED
+    ref = reference()
+    sysm = config0_system()
+    pts = ref["points"]
+    eq, prod = ref["equilibration"], ref["production"]
+    seeds = walker_seeds(NPER * len(pts), 0)
+    say(f"{len(seeds)} walkers x (mixed, fp64), {eq} + {prod} steps")
+    mixed = run_walkers(sysm, pts, seeds, FMC_MIXED, eq, prod)
+    fp64 = run_walkers(sysm, pts, seeds, FMC_FP64, eq, prod)
     bad = []
     for ip, pt in enumerate(pts):
-        dev = means[ip].mean()
-        se_dev = means[ip].std(ddof=1) / np.sqrt(NPER)
+        d = mixed[ip] - fp64[ip]
+        ndiff = int(np.count_nonzero(d))  # chains whose decisions ever differed
+        se_d = d.std(ddof=1) / np.sqrt(NPER)
+        z_pair = d.mean() / se_d if ndiff > 1 else 0.0
         se_ref = pt["sd_between_runs"] / np.sqrt(len(pt["runs"]))
-        z = (dev - pt["mean_n"]) / np.hypot(se_dev, se_ref)
-        say(f"{precision} T = {pt['temperature']:g} K, P = {pt['pressure_bar']} bar: device <N> = {dev:.3f} "
-            f"+/- {se_dev:.3f} ({NPER} chains); reference {pt['mean_n']:.3f} +/- {se_ref:.3f} "
-            f"({len(pt['runs'])} runs); z = {z:+.2f}")
-        if abs(z) > 3.0:
-            bad.append((pt["temperature"], pt["pressure_bar"], round(float(z), 2)))
+        line = f"T = {pt['temperature']:g} K, P = {pt['pressure_bar']} bar: "
+        for name, m in (("mixed", mixed[ip]), ("fp64", fp64[ip])):
+            se = m.std(ddof=1) / np.sqrt(NPER)
+            line += (f"{name} {m.mean():.3f} +/- {se:.3f} (z vs reference "
+                     f"{(m.mean() - pt['mean_n']) / np.hypot(se, se_ref):+.2f}); ")
+        say(line + f"reference {pt['mean_n']:.3f} +/- {se_ref:.3f}; mixed - fp64 = {d.mean():+.3f} +/- {se_d:.3f} "
+            f"(z {z_pair:+.2f}; {NPER - ndiff} of {NPER} mixed chains identical to their FP64 chain)")
+        if abs(z_pair) > 3.0:
+            bad.append(("paired", pt["temperature"], pt["pressure_bar"], round(float(z_pair), 2)))
+        if abs(mixed[ip].mean() - pt["mean_n"]) > pt["mean_stderr_n"]:
+            bad.append(("block error", pt["temperature"], pt["pressure_bar"]))
     assert not bad, bad
