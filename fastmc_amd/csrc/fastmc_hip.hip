@@ -847,7 +847,6 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
           }
       P.lj_hi = std::min(hi, mxegrd);
       P.lj_n64 = P.lj_hi + 2 - P.lj_lo;
-      P.lj_n32 = (P.lj_lo + 2) + (P.vstride - P.lj_hi);
     }
     P.lds_all = 0;
     if (!getenv("FMC_NO_LDS_ALL") && cols_ok) {
@@ -863,7 +862,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
         const size_t nc = (size_t)std::max(ncol, 1);
         o = al(o + (size_t)8 * nc * P.lj_n64);
         ov = (int)o;
-        o = al(o + (size_t)4 * nc * P.lj_n32);
+        o = al(o + (size_t)4 * nc * P.vstride);
       }
       if (o <= budget) {
         P.lds_all = 1;

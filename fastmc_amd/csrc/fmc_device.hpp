@@ -50,9 +50,8 @@ struct DevParams {
   // mixed-mode LJ table: entries [lj_lo, lj_hi) of a column, the repulsive
   // wall where |V| is large, are interpolated from FP64 values (entries up to
   // lj_hi + 1), all others from FP32 values.  LDS copy: off_vvv64 holds
-  // [nvcol][lj_n64] doubles (entries lj_lo ..), off_vvv [nvcol][lj_n32]
-  // floats: entries 0 .. lj_lo + 1, then lj_hi .. mxegrd + 2
-  int lj_lo, lj_hi, lj_n64, lj_n32, off_vvv64;
+  // [nvcol][lj_n64] doubles (entries lj_lo ..), off_vvv [nvcol][vstride] floats
+  int lj_lo, lj_hi, lj_n64, off_vvv64;
   int resum_every;                // full FP64 S(k) re-sum after this many accepted moves (0: never)
   // Wang-Landau sampling in N of guest 1 (wang_landau.f); wl_nbins = 0: off
   int wl_nbins;                   // bins per walker: N = 0 .. wl_nbins-1 (maxn + 2)

@@ -603,27 +603,14 @@ __device__ void swap_insert_setup(const DevParams &P, Smem<NS> &sh) {
 }
 
 // ----------------------------------------------------------- evaluation --
-// One guest-site / atom pair of insertion / guest_energy / deletion: minimum
-// image, surface and overlap tests (surface_check, mc_moves.f:10-44: bit 0
-// of ovl rsq < overlap^2, bit 1 rsq < sg2 = (surftol + sigma)^2 for a frozen
-// atom, sg2 < 0 otherwise), ewald2 and srfrce terms (ewald.f:689-726,
-// vdw_terms.f:418-460).  Branch-free, so that the
-// pairs of one atom with all trial sites form one straight-line block:
-// table indices are clamped and out-of-range terms selected away.  Geometry
-// and the real-space Coulomb term are FP64 in both modes (pair Coulomb terms
-// are large and cancel); T is the precision of the LJ interpolation, whose
-// fractional grid position is formed in FP64 first.  col < 0: no LJ term;
-// ovl2 < 0: a padding site (never overlaps, chgea 0).
 // Where a pair's LJ table value comes from: FP64 mode reads v64 (column
-// stride s64) everywhere; mixed mode reads the wall entries [lo, hi) from v64
-// (table index b64 at v64[0]) and every other entry from v32, whose columns
-// either hold the whole table (packed false) or the entries 0 .. lo + 1 then
-// hi .. end (packed true, the LDS copy).
+// stride s64) everywhere; mixed mode reads FP32 values from v32 (stride s32,
+// whole columns) and, for the wall entries [lo, hi), FP64 values from v64
+// (table index b64 at v64[0]).
 struct LJTab {
   const double *v64;
   const float *v32;
   int s64, s32, b64, lo, hi;
-  bool packed;
 };
 
 __device__ inline LJTab lj_global(const DevParams &P, bool mixed) {
@@ -634,7 +621,6 @@ __device__ inline LJTab lj_global(const DevParams &P, bool mixed) {
   t.b64 = 0;
   t.lo = mixed ? P.lj_lo : 0;
   t.hi = mixed ? P.lj_hi : 0x7fffffff;
-  t.packed = false;
   return t;
 }
 
@@ -643,11 +629,10 @@ __device__ inline LJTab lj_lds(const DevParams &P, const unsigned char *lds) {
   t.v64 = (const double *)(lds + P.off_vvv64);
   t.v32 = (const float *)(lds + P.off_vvv);
   t.s64 = P.lj_n64;
-  t.s32 = P.lj_n32;
+  t.s32 = P.vstride;
   t.b64 = P.lj_lo;
   t.lo = P.lj_lo;
   t.hi = P.lj_hi;
-  t.packed = true;
   return t;
 }
 
@@ -655,13 +640,15 @@ __device__ inline LJTab lj_lds(const DevParams &P, const unsigned char *lds) {
 // image, surface and overlap tests (surface_check, mc_moves.f:10-44: bit 0
 // of ovl rsq < overlap^2, bit 1 rsq < sg2 = (surftol + sigma)^2 for a frozen
 // atom, sg2 < 0 otherwise), ewald2 and srfrce terms (ewald.f:689-726,
-// vdw_terms.f:418-460).  Branch-free, so that the
-// pairs of one atom with all trial sites form one straight-line block:
-// table indices are clamped and out-of-range terms selected away.  Geometry,
-// the real-space Coulomb term and the LJ interpolation are FP64 in both
-// modes (pair Coulomb terms are large and cancel); T = float takes the LJ
-// table values off the repulsive wall from FP32 storage.  col < 0: no LJ
-// term; ovl2 < 0: a padding site (never overlaps, chgea 0).
+// vdw_terms.f:418-460).  Branch-free, so that the pairs of one atom with all
+// trial sites form one straight-line block: table indices are clamped and
+// out-of-range terms selected away.  Geometry and the real-space Coulomb term
+// are FP64 in both modes (pair Coulomb terms are large and cancel).  T is the
+// precision of the LJ interpolation away from the repulsive wall, whose
+// fractional grid position is formed in FP64 first; on the wall (mixed mode's
+// LJTab segment [lo, hi)) the LJ term is interpolated in FP64 from FP64
+// values.  col < 0: no LJ term; ovl2 < 0: a padding site (never overlaps,
+// chgea 0).
 template <typename T>
 __device__ inline void pair_eval(const DevParams &P, const double *erc, const LJTab &lj, const double *xs, double chgea,
                                  int col, double ovl2, double sg2, double xj, double yj, double zj, double qj,
@@ -684,30 +671,35 @@ __device__ inline void pair_eval(const DevParams &P, const double *erc, const LJ
   e2 = (in && fabs(chgprd) > 1e-10) ? ec : 0.0;
   double xv = r * P.rdlrpot;
   int lv = min((int)xv, lmax);
-  double pp = xv - (double)lv;
   const int c = col < 0 ? 0 : col;
-  double w0, w1, w2;
+  double el;
   if constexpr (sizeof(T) == 8) {
+    const double pp = xv - (double)lv;
     const double *vv = lj.v64 + (size_t)c * lj.s64 + lv;
-    w0 = vv[0];
-    w1 = vv[1];
-    w2 = vv[2];
+    const double w0 = vv[0], w1 = vv[1], w2 = vv[2];
+    const double u1 = w0 + (w1 - w0) * pp;
+    const double u2 = w1 + (w2 - w1) * (pp - 1.0);
+    el = u1 + (u2 - u1) * pp * 0.5;
   } else {
-    // both stencils are read (indices clamped into their segments), one is kept
+    const float pp = (float)(xv - (double)lv);
+    const float *vv = lj.v32 + (size_t)c * lj.s32 + lv;
+    const float w0 = vv[0], w1 = vv[1], w2 = vv[2];
+    const float u1 = w0 + (w1 - w0) * pp;
+    const float u2 = w1 + (w2 - w1) * (pp - 1.0f);
+    el = (double)(u1 + (u2 - u1) * pp * 0.5f);
+    // the repulsive wall in FP64, read only by a wave with a lane on it
+    // (close contacts are rare, so most waves skip it)
     const bool wall = lv >= lj.lo && lv < lj.hi;
-    const int i64 = min(max(lv, lj.lo), lj.hi - 1) - lj.b64;
-    const int i32 = !lj.packed ? lv : (lv < lj.lo ? lv : max(lv, lj.hi) - lj.hi + lj.lo + 2);
-    const double *a = lj.v64 + (size_t)c * lj.s64 + i64;
-    const float *b = lj.v32 + (size_t)c * lj.s32 + i32;
-    const double a0 = a[0], a1 = a[1], a2 = a[2];
-    const float b0 = b[0], b1 = b[1], b2 = b[2];
-    w0 = wall ? a0 : (double)b0;
-    w1 = wall ? a1 : (double)b1;
-    w2 = wall ? a2 : (double)b2;
+    if (__any(wall)) {
+      const double qq = xv - (double)lv;
+      const double *a = lj.v64 + (size_t)c * lj.s64 + (min(max(lv, lj.lo), lj.hi - 1) - lj.b64);
+      const double a0 = a[0], a1 = a[1], a2 = a[2];
+      const double t1 = a0 + (a1 - a0) * qq;
+      const double t2 = a1 + (a2 - a1) * (qq - 1.0);
+      const double e64 = t1 + (t2 - t1) * qq * 0.5;
+      el = wall ? e64 : el;
+    }
   }
-  double u1 = w0 + (w1 - w0) * pp;
-  double u2 = w1 + (w2 - w1) * (pp - 1.0);
-  double el = u1 + (u2 - u1) * pp * 0.5;
   ev = (in && col >= 0) ? el : 0.0;
 }
 
@@ -839,7 +831,7 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
       kl[i] = P.klmn[i];
     }
     if constexpr (sizeof(T) == 4) {
-      // LJ columns: the wall entries in FP64, the rest in FP32 (LJTab, packed)
+      // LJ columns: whole in FP32, and the wall entries again in FP64 (LJTab)
       double *d64 = (double *)(lds + P.off_vvv64);
       const int n64 = P.nvcol * P.lj_n64;
       for (int i = tid; i < n64; i += nt) {
@@ -847,11 +839,10 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
         d64[i] = P.vvv[(size_t)P.vcol_src[c] * P.vstride + P.lj_lo + r];
       }
       float *d32 = (float *)(lds + P.off_vvv);
-      const int n32 = P.nvcol * P.lj_n32, na = P.lj_lo + 2;
+      const int n32 = P.nvcol * P.vstride;
       for (int i = tid; i < n32; i += nt) {
-        int c = i / P.lj_n32, r = i - c * P.lj_n32;
-        int e = r < na ? r : r - na + P.lj_hi;
-        d32[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + e];
+        int c = i / P.vstride, r = i - c * P.vstride;
+        d32[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + r];
       }
     }
     double *lx = (double *)(lds + P.off_g), *ly = lx + P.gatoms_total, *lz = ly + P.gatoms_total;
