@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "fmc_grad.hpp"
@@ -31,6 +32,11 @@ namespace {
     }                                                                                  \
   } while (0)
 
+__global__ void k_fill_zero(double *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 struct GraphKey {
   int nsweeps, nsteps, count;
   bool operator<(const GraphKey &o) const {
@@ -42,8 +48,40 @@ struct GraphKey {
 
 }  // namespace
 
+// Fixed device scratch of fmc_widom_noalloc: one copy on every GPU the
+// library runs on, held until the process ends.  On the host the name is only
+// a stand-in for the device symbol: host code never passes it to a kernel or a
+// copy, it uses the address that scratch_address resolves per device.
+static __device__ __align__(256) unsigned char scratch[FMC_WIDOM_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_addr[64] = {};
+
+// Device address of `scratch` on the current device, resolved once per device.
+int scratch_address(void **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return FMC_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "fastmc_hip: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return FMC_ERR_HIP;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return FMC_OK;
+}
+
+}  // namespace
+
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
+  void *scratch = nullptr;       // this device's address of `scratch` (fmc_widom_noalloc)
   int trace_w = 0;               // walker whose steps fmc_run records
   int nsite = 0;                 // 3: every guest has at most 3 sites (straight-line pair code)
   DevParams hp;                  // host copy
@@ -881,6 +919,9 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   }
   // ---- device buffers
   HIPCHK(hipStreamCreate(&h->stream));
+  // the fixed scratch of fmc_widom_noalloc, resolved once per device (a
+  // handle without it only refuses that entry point)
+  if (scratch_address(&h->scratch) != FMC_OK) h->scratch = nullptr;
   HIPCHK(allow_lds_dispatch(h->ns, precision == FMC_MIXED, P.lds_bytes));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
@@ -1643,72 +1684,61 @@ int fmc_wl_isotherm(const double *lng, int maxn, int minn, double temperature, d
   return FMC_OK;
 }
 
-// widom_grid (gcmc.f:2454-2548) on walker w: grid set-up of gcmc.f:258-281 and
-// 605-610, attempts in the reference's loop order, orientations from the
-// walker's duni sequence (which advances by 4 draws per attempt).
-int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
-              double *estep_out, int *overlap_out, double *emap_out, double *cmap_out, fmc_widom_result *res) {
-  if (!h || !res || w < 0 || w >= h->nwalkers || iguest < 1 || iguest > h->hp.ntpguest || nstep < 1 ||
-      !(griddim > 0.0))
-    return FMC_ERR_ARG;
-  const int gf[3] = {gridfactor ? gridfactor[0] : 1, gridfactor ? gridfactor[1] : 1, gridfactor ? gridfactor[2] : 1};
-  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1 || (!emap_out) != (!cmap_out)) return FMC_ERR_ARG;
-  const DevParams &P = h->hp;
+namespace {
+
+// Device buffers of one fmc_widom call: 4 draws per attempt and the staged
+// per-attempt outputs for `chunk` attempts, per-block partial sums, maps.
+struct WidomBufs {
+  float *uni = nullptr;
+  double *est = nullptr, *bsum = nullptr, *emap = nullptr, *cmap = nullptr;
+  int *ovl = nullptr;
+  long long *bcnt = nullptr;
+  long long chunk = 0;
+  int nblocks = 0;
+};
+
+// Grid of widom_grid: spacing griddim, gridfactor unit cells per axis
+// (gcmc.f:258-281, 605-610).
+void widom_grid_size(const DevParams &P, const int *gf, double griddim, int *ng, int *wng) {
   double celprp[10];
   dcell(P.cell, celprp);
-  int ng[3], wng[3];
   for (int a = 0; a < 3; ++a) {
     const double gd = celprp[a] / std::ceil(celprp[a] / gf[a] / griddim) / gf[a];
     ng[a] = (int)std::ceil(celprp[a] / (gd * gf[a]));
     wng[a] = (int)std::ceil(celprp[a] / gd);
   }
+}
+
+// widom_grid (gcmc.f:2454-2548) on walker w with the given buffers (maps
+// already cleared): attempts in the reference's loop order, `chunk` at a
+// time, orientations from the walker's duni sequence (which advances by 4
+// draws per attempt).
+int widom_run(fmc_handle *h, int w, int iguest, int nstep, const int *gf, const int *ng, const int *wng,
+              const WidomBufs &B, double *estep_out, int *overlap_out, double *emap_out, double *cmap_out,
+              fmc_widom_result *res) {
+  const DevParams &P = h->hp;
   const long long total = (long long)ng[0] * ng[1] * ng[2] * nstep;
+  const size_t mapsize = (size_t)ng[0] * ng[1] * ng[2];
   WState ws;
   HIPCHK(hipMemcpy(&ws, h->dw.ws + w, sizeof(WState), hipMemcpyDeviceToHost));
-  const long long chunk = std::min<long long>(total, 1LL << 22);
-  const int nblocks = (int)std::min<long long>((chunk + WIDOM_WPB - 1) / WIDOM_WPB, 4096);
-  float *duni = nullptr;
-  double *dest = nullptr, *dbsum = nullptr, *demap = nullptr, *dcmap = nullptr;
-  int *dovl = nullptr;
-  const size_t mapsize = (size_t)ng[0] * ng[1] * ng[2];
-  long long *dbcnt = nullptr;
-  int rc = FMC_OK;
-  auto fail = [&](int code) {
-    rc = code;
-    return code;
-  };
-  if (hipMalloc(&duni, sizeof(float) * 4 * (size_t)chunk) != hipSuccess ||
-      hipMalloc(&dbsum, sizeof(double) * nblocks) != hipSuccess ||
-      hipMalloc(&dbcnt, sizeof(long long) * nblocks) != hipSuccess ||
-      (estep_out && hipMalloc(&dest, sizeof(double) * (size_t)chunk) != hipSuccess) ||
-      (overlap_out && hipMalloc(&dovl, sizeof(int) * (size_t)chunk) != hipSuccess) ||
-      (emap_out && (hipMalloc(&demap, sizeof(double) * mapsize) != hipSuccess ||
-                    hipMalloc(&dcmap, sizeof(double) * mapsize) != hipSuccess ||
-                    hipMemset(demap, 0, sizeof(double) * mapsize) != hipSuccess ||
-                    hipMemset(dcmap, 0, sizeof(double) * mapsize) != hipSuccess)))
-    fail(FMC_ERR_ALLOC);
   // the unit cell the map folds into (gcmc.f:299-306)
   double ucell[9], rucell[9];
   for (int i = 0; i < 9; ++i) ucell[i] = P.cell[i] / (double)gf[i / 3];
   invert(ucell, rucell);
   const WidomLds L = widom_lds(P);
   const bool sites3 = h->nsite == 3;
-  if (rc == FMC_OK) {
-    auto allow = [&](const void *f) {
-      return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total) == hipSuccess;
-    };
-    const void *kf = h->precision == FMC_FP64
-                         ? (sites3 ? (const void *)k_widom<double, 3> : (const void *)k_widom<double, 0>)
-                         : (sites3 ? (const void *)k_widom<float, 3> : (const void *)k_widom<float, 0>);
-    if (L.total > 160 * 1024 || !allow(kf)) fail(FMC_ERR_ARG);
-  }
-  std::vector<double> bsum(nblocks);
-  std::vector<long long> bcnt(nblocks);
+  const void *kf = h->precision == FMC_FP64
+                       ? (sites3 ? (const void *)k_widom<double, 3> : (const void *)k_widom<double, 0>)
+                       : (sites3 ? (const void *)k_widom<float, 3> : (const void *)k_widom<float, 0>);
+  if (L.total > 160 * 1024) return FMC_ERR_ARG;
+  HIPCHK(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total));
+  std::vector<double> bsum(B.nblocks);
+  std::vector<long long> bcnt(B.nblocks);
   double hsum = 0.0;
   long long accepted = 0;
-  for (long long a0 = 0; rc == FMC_OK && a0 < total; a0 += chunk) {
-    const long long cnt = std::min(chunk, total - a0);
-    hipLaunchKernelGGL(k_draws, dim3(1), dim3(64), 0, h->stream, h->dw, w, 4 * cnt, duni);
+  for (long long a0 = 0; a0 < total; a0 += B.chunk) {
+    const long long cnt = std::min(B.chunk, total - a0);
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(64), 0, h->stream, h->dw, w, 4 * cnt, B.uni);
     WidomArgs A{};
     A.w = w;
     A.g = iguest;
@@ -1719,16 +1749,16 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
     }
     A.a0 = a0;
     A.count = cnt;
-    A.uni = duni;
-    A.estep = dest;
-    A.overlap = dovl;
-    A.bsum = dbsum;
-    A.bcnt = dbcnt;
+    A.uni = B.uni;
+    A.estep = estep_out ? B.est : nullptr;
+    A.overlap = overlap_out ? B.ovl : nullptr;
+    A.bsum = B.bsum;
+    A.bcnt = B.bcnt;
     A.kboltz = KBOLTZ;
-    A.emap = demap;
-    A.cmap = dcmap;
+    A.emap = emap_out ? B.emap : nullptr;
+    A.cmap = emap_out ? B.cmap : nullptr;
     for (int i = 0; i < 9; ++i) A.rucell[i] = rucell[i];
-    const int nb = (int)std::min<long long>((cnt + WIDOM_WPB - 1) / WIDOM_WPB, nblocks);
+    const int nb = (int)std::min<long long>((cnt + WIDOM_WPB - 1) / WIDOM_WPB, B.nblocks);
     if (h->precision == FMC_FP64) {
       if (sites3) hipLaunchKernelGGL((k_widom<double, 3>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
       else hipLaunchKernelGGL((k_widom<double, 0>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
@@ -1736,29 +1766,21 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
       if (sites3) hipLaunchKernelGGL((k_widom<float, 3>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
       else hipLaunchKernelGGL((k_widom<float, 0>), dim3(nb), dim3(64 * WIDOM_WPB), L.total, h->stream, h->dp, h->dw, A);
     }
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
-      fail(FMC_ERR_HIP);
-      break;
-    }
-    if (hipMemcpy(bsum.data(), dbsum, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(bcnt.data(), dbcnt, sizeof(long long) * nb, hipMemcpyDeviceToHost) != hipSuccess ||
-        (estep_out && hipMemcpy(estep_out + a0, dest, sizeof(double) * cnt, hipMemcpyDeviceToHost) != hipSuccess) ||
-        (overlap_out && hipMemcpy(overlap_out + a0, dovl, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess)) {
-      fail(FMC_ERR_HIP);
-      break;
-    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(bsum.data(), B.bsum, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bcnt.data(), B.bcnt, sizeof(long long) * nb, hipMemcpyDeviceToHost));
+    if (estep_out) HIPCHK(hipMemcpy(estep_out + a0, B.est, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    if (overlap_out) HIPCHK(hipMemcpy(overlap_out + a0, B.ovl, sizeof(int) * cnt, hipMemcpyDeviceToHost));
     for (int b = 0; b < nb; ++b) {
       hsum += bsum[b];
       accepted += bcnt[b];
     }
   }
-  if (rc == FMC_OK && emap_out &&
-      (hipMemcpy(emap_out, demap, sizeof(double) * mapsize, hipMemcpyDeviceToHost) != hipSuccess ||
-       hipMemcpy(cmap_out, dcmap, sizeof(double) * mapsize, hipMemcpyDeviceToHost) != hipSuccess))
-    fail(FMC_ERR_HIP);
-  for (void *p : {(void *)duni, (void *)dest, (void *)dovl, (void *)dbsum, (void *)dbcnt, (void *)demap, (void *)dcmap})
-    if (p) (void)hipFree(p);
-  if (rc != FMC_OK) return rc;
+  if (emap_out) {
+    HIPCHK(hipMemcpy(emap_out, B.emap, sizeof(double) * mapsize, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cmap_out, B.cmap, sizeof(double) * mapsize, hipMemcpyDeviceToHost));
+  }
   res->attempts = total;
   res->accepted = accepted;
   res->avg_boltz = hsum / (double)total;
@@ -1768,6 +1790,95 @@ int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const
     res->wngrid[a] = wng[a];
   }
   return FMC_OK;
+}
+
+int widom_args(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+               const double *emap_out, const double *cmap_out, fmc_widom_result *res, int *gf) {
+  if (!h || !res || w < 0 || w >= h->nwalkers || iguest < 1 || iguest > h->hp.ntpguest || nstep < 1 ||
+      !(griddim > 0.0))
+    return FMC_ERR_ARG;
+  for (int a = 0; a < 3; ++a) gf[a] = gridfactor ? gridfactor[a] : 1;
+  if (gf[0] < 1 || gf[1] < 1 || gf[2] < 1 || (!emap_out) != (!cmap_out)) return FMC_ERR_ARG;
+  return FMC_OK;
+}
+
+}  // namespace
+
+// widom_grid (gcmc.f:2454-2548) on walker w: grid set-up of gcmc.f:258-281 and
+// 605-610, attempts in the reference's loop order, orientations from the
+// walker's duni sequence (which advances by 4 draws per attempt).
+int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+              double *estep_out, int *overlap_out, double *emap_out, double *cmap_out, fmc_widom_result *res) {
+  int gf[3];
+  if (int e = widom_args(h, w, iguest, nstep, griddim, gridfactor, emap_out, cmap_out, res, gf)) return e;
+  int ng[3], wng[3];
+  widom_grid_size(h->hp, gf, griddim, ng, wng);
+  const long long total = (long long)ng[0] * ng[1] * ng[2] * nstep;
+  const size_t mapsize = (size_t)ng[0] * ng[1] * ng[2];
+  WidomBufs B;
+  B.chunk = std::min<long long>(total, 1LL << 22);
+  B.nblocks = (int)std::min<long long>((B.chunk + WIDOM_WPB - 1) / WIDOM_WPB, 4096);
+  int rc = FMC_OK;
+  if (hipMalloc(&B.uni, sizeof(float) * 4 * (size_t)B.chunk) != hipSuccess ||
+      hipMalloc(&B.bsum, sizeof(double) * B.nblocks) != hipSuccess ||
+      hipMalloc(&B.bcnt, sizeof(long long) * B.nblocks) != hipSuccess ||
+      (estep_out && hipMalloc(&B.est, sizeof(double) * (size_t)B.chunk) != hipSuccess) ||
+      (overlap_out && hipMalloc(&B.ovl, sizeof(int) * (size_t)B.chunk) != hipSuccess) ||
+      (emap_out && (hipMalloc(&B.emap, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMalloc(&B.cmap, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMemset(B.emap, 0, sizeof(double) * mapsize) != hipSuccess ||
+                    hipMemset(B.cmap, 0, sizeof(double) * mapsize) != hipSuccess)))
+    rc = FMC_ERR_ALLOC;
+  if (rc == FMC_OK)
+    rc = widom_run(h, w, iguest, nstep, gf, ng, wng, B, estep_out, overlap_out, emap_out, cmap_out, res);
+  for (void *p : {(void *)B.uni, (void *)B.est, (void *)B.ovl, (void *)B.bsum, (void *)B.bcnt, (void *)B.emap,
+                  (void *)B.cmap})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int fmc_widom_noalloc(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+                      double *estep_out, int *overlap_out, double *emap_out, double *cmap_out,
+                      fmc_widom_result *res) {
+  int gf[3];
+  if (int e = widom_args(h, w, iguest, nstep, griddim, gridfactor, emap_out, cmap_out, res, gf)) return e;
+  if (!h->scratch) return FMC_ERR_STATE;  // fmc_create could not resolve the scratch array
+  int ng[3], wng[3];
+  widom_grid_size(h->hp, gf, griddim, ng, wng);
+  const size_t mapsize = (size_t)ng[0] * ng[1] * ng[2];
+  if (emap_out && mapsize > (size_t)FMC_WIDOM_SCRATCH_MAP) {
+    fprintf(stderr, "fastmc: fmc_widom_noalloc: an energy map of %zu grid points exceeds the fixed scratch (%d)\n",
+            mapsize, FMC_WIDOM_SCRATCH_MAP);
+    return FMC_ERR_SCRATCH;
+  }
+  // carve the fixed array: draws, per-attempt staging, block sums, maps
+  unsigned char *base = (unsigned char *)h->scratch;
+  const long long CH = FMC_WIDOM_SCRATCH_CHUNK;
+  WidomBufs B;
+  B.chunk = CH;
+  B.nblocks = 4096;
+  size_t off = 0;
+  B.uni = (float *)(base + off);
+  off += sizeof(float) * 4 * CH;
+  B.est = (double *)(base + off);
+  off += sizeof(double) * CH;
+  B.ovl = (int *)(base + off);
+  off += sizeof(int) * CH;
+  B.bsum = (double *)(base + off);
+  off += sizeof(double) * B.nblocks;
+  B.bcnt = (long long *)(base + off);
+  off += sizeof(long long) * B.nblocks;
+  B.emap = (double *)(base + off);
+  off += sizeof(double) * FMC_WIDOM_SCRATCH_MAP;
+  B.cmap = (double *)(base + off);
+  if (emap_out) {
+    // the array keeps the previous call's maps: clear them with a fill kernel
+    const int n = (int)mapsize;
+    hipLaunchKernelGGL(k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, h->stream, B.emap, n);
+    hipLaunchKernelGGL(k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, h->stream, B.cmap, n);
+    HIPCHK(hipGetLastError());
+  }
+  return widom_run(h, w, iguest, nstep, gf, ng, wng, B, estep_out, overlap_out, emap_out, cmap_out, res);
 }
 
 int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int *nmols, const double *xyz,

@@ -33,7 +33,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
            "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob",
            "fmc_set_numguests", "fmc_get_numguests", "fmc_load_guests",
-           "fmc_set_trace_walker")
+           "fmc_set_trace_walker", "fmc_widom_noalloc")
 
 _lib = None
 
@@ -304,11 +304,12 @@ class Handle:
 
     # ---- Wang-Landau (wang_landau.f)
     def widom(self, iguest: int, nstep: int, griddim: float, gridfactor=(1, 1, 1), w: int = 0,
-              per_attempt: bool = False, energy_map: bool = False):
+              per_attempt: bool = False, energy_map: bool = False, noalloc: bool = False):
         """Widom insertions on a grid (fmc_widom): dict of <exp(-beta estep)>, attempts, non-overlapping
         attempts, Henry's constant as the reference prints it, grid sizes; with per_attempt also the
         estep and overlap arrays in the reference's loop order; with energy_map the binned estep and
-        count grids ("emap", "cmap") of the 'e' probability plot."""
+        count grids ("emap", "cmap") of the 'e' probability plot.  noalloc: the variant that allocates
+        no device memory (fmc_widom_noalloc)."""
         gf = (C.c_int * 3)(*gridfactor)
         res = FmcWidomResult()
         est = ovl = emap = cmap = None
@@ -324,8 +325,9 @@ class Handle:
 
         def ptr(a):
             return a.ctypes.data_as(C.c_void_p) if a is not None else None
-        _chk("fmc_widom", self.L.fmc_widom(self.h, C.c_int(w), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim),
-                                           gf, ptr(est), ptr(ovl), ptr(emap), ptr(cmap), C.byref(res)))
+        fn = "fmc_widom_noalloc" if noalloc else "fmc_widom"
+        _chk(fn, getattr(self.L, fn)(self.h, C.c_int(w), C.c_int(iguest), C.c_int(nstep), C.c_double(griddim),
+                                     gf, ptr(est), ptr(ovl), ptr(emap), ptr(cmap), C.byref(res)))
         out = {"avg_boltz": res.avg_boltz, "attempts": res.attempts, "accepted": res.accepted, "henry": res.henry,
                "ngrid": tuple(res.ngrid), "wngrid": tuple(res.wngrid)}
         if per_attempt:

@@ -39,6 +39,7 @@ extern "C" {
 #define FMC_ERR_STATE 9002           /* call out of sequence (e.g. accept without a trial) */
 #define FMC_ERR_HIP 9003             /* HIP runtime failure */
 #define FMC_ERR_IMCON 9004           /* boundary condition other than imcon 1, 2, 3 */
+#define FMC_ERR_SCRATCH 9005         /* request larger than a fixed scratch buffer */
 
 /* precision modes */
 #define FMC_FP64 0   /* every term in FP64 (parity mode) */
@@ -343,6 +344,23 @@ typedef struct fmc_widom_result {
 
 int fmc_widom(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
               double *estep_out, int *overlap_out, double *emap_out, double *cmap_out, fmc_widom_result *res);
+
+/* fmc_widom that allocates no device memory: no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments
+ * and results.  Its draws, per-attempt staging, block sums and maps live in
+ * one fixed device array of FMC_WIDOM_SCRATCH_BYTES on each GPU the library
+ * runs on (held until the process ends; fmc_create resolves its address);
+ * attempts go FMC_WIDOM_SCRATCH_CHUNK per launch, and an energy map of more
+ * than FMC_WIDOM_SCRATCH_MAP grid points returns FMC_ERR_SCRATCH.  Every call
+ * on one GPU uses the same array: calls that overlap on one GPU, from any
+ * handles or threads, race. */
+#define FMC_WIDOM_SCRATCH_BYTES (4 << 20)
+#define FMC_WIDOM_SCRATCH_CHUNK 65536
+#define FMC_WIDOM_SCRATCH_MAP 143360
+int fmc_widom_noalloc(fmc_handle *h, int w, int iguest, int nstep, double griddim, const int *gridfactor,
+                      double *estep_out, int *overlap_out, double *emap_out, double *cmap_out,
+                      fmc_widom_result *res);
 
 /* ---- production statistics (gcmc.f:1050-1130, 1148-1258, 1458-1557) ---- */
 
