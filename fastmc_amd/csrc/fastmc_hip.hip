@@ -106,6 +106,11 @@ struct fmc_handle {
   double selfq2_all_fw = 0.0;
   std::vector<int> kperm;        // internal k -> reference enumeration index
   size_t lds_base_end = 0;       // LDS bytes of the always-staged part of the plan
+  // the plan with the LJ columns alone after the always-staged part
+  int ljo_off64 = 0, ljo_off32 = 0;
+  size_t ljo_end = 0;
+  bool ljo_fits = false;
+  size_t plan_end = 0;           // end of the plan before the Wang-Landau arrays and the re-sum rows
   int wl_maxn = 0, wl_minn = 0, wl_nwin = 0;  // Wang-Landau set-up (0: off)
   // energy-gradient entry: framework types and frozen flags, FIELD LJ rows
   std::vector<int> fw_type, fw_frz, fw_spec;
@@ -216,12 +221,14 @@ template <int NS, typename T>
 void launch_run(fmc_handle *h, int nsteps, int count) {
   dim3 grid(h->nwalkers), block(h->block);
   size_t lds = h->hp.lds_bytes;
-  if (!h->hp.lds_all)
-    hipLaunchKernelGGL((k_run<NS, T, false, 0>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+  if (!h->hp.lds_all && h->hp.lj_lds)
+    hipLaunchKernelGGL((k_run<NS, T, false, 0, true>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+  else if (!h->hp.lds_all)
+    hipLaunchKernelGGL((k_run<NS, T, false, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
   else if (h->nsite == 3)
-    hipLaunchKernelGGL((k_run<NS, T, true, 3>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+    hipLaunchKernelGGL((k_run<NS, T, true, 3, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
   else
-    hipLaunchKernelGGL((k_run<NS, T, true, 0>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+    hipLaunchKernelGGL((k_run<NS, T, true, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
 }
 
 size_t smem_size(int ns) {
@@ -234,8 +241,9 @@ size_t smem_size(int ns) {
 
 template <int NS, typename T>
 hipError_t allow_lds(int bytes) {
-  const void *fns[] = {(const void *)k_run<NS, T, false, 0>, (const void *)k_run<NS, T, true, 3>,
-                       (const void *)k_run<NS, T, true, 0>, (const void *)k_trial<NS, T>,
+  const void *fns[] = {(const void *)k_run<NS, T, false, 0, false>, (const void *)k_run<NS, T, false, 0, true>,
+                       (const void *)k_run<NS, T, true, 3, false>, (const void *)k_run<NS, T, true, 0, false>,
+                       (const void *)k_trial<NS, T>,
                        (const void *)k_accept<NS>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -457,6 +465,29 @@ void lrcorrect_counts(const DevParams &P, const fmc_system *sys, const std::vect
         }
       for (int t = 1; t <= KMAXC; ++t) s.elrc_mol[t] += volm * eadd * e0[t];
     }
+}
+
+// The LDS left after the plan (from `end`) holds the in-kernel re-sum's
+// per-atom phase rows: up to 32 KB, never so much that fewer workgroups fit
+// on a CU, none below 8 atoms a batch.  Sets off_rs, rs_batch, lds_bytes.
+void plan_resum_rows(DevParams &P, size_t end) {
+  const size_t budget = 160 * 1024 - 1024, cu = 160 * 1024;
+  end = (end + 15) & ~(size_t)15;
+  const size_t per_cu = cu / std::max<size_t>(end, 1);
+  size_t cap = per_cu >= 1 ? std::min(budget, cu / per_cu) : budget;
+  size_t room = cap > end ? cap - end : 0;
+  room = std::min(room, (size_t)32 * 1024);
+  const size_t per = (size_t)3 * P.kcp * 16 + 12;
+  const int nb = (int)(room / per);
+  if (nb >= 8 && !getenv("FMC_NO_RESUM_ROWS")) {
+    P.off_rs = (int)end;
+    P.rs_batch = nb;
+    P.lds_bytes = (int)((end + (size_t)nb * per + 15) & ~(size_t)15);
+  } else {
+    P.off_rs = 0;
+    P.rs_batch = 0;
+    P.lds_bytes = (int)end;
+  }
 }
 
 }  // namespace
@@ -912,10 +943,37 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
         off = o;
       }
     }
-    P.lds_bytes = (int)al(off);
+    // without the all-in-LDS plan: the LJ columns alone, when they fit
+    // (mixed mode: FP64 wall entries then the whole FP32 columns; FP64
+    // mode: the whole FP64 columns)
+    {
+      const size_t nc = (size_t)std::max(ncol, 1);
+      size_t o = base_end;
+      h->ljo_off64 = (int)o;
+      if (precision == FMC_MIXED) {
+        o = al(o + (size_t)8 * nc * P.lj_n64);
+        h->ljo_off32 = (int)o;
+        o = al(o + (size_t)4 * nc * P.vstride);
+      } else {
+        h->ljo_off32 = (int)o;
+        o = al(o + (size_t)8 * nc * P.vstride);
+      }
+      h->ljo_end = o;
+      h->ljo_fits = cols_ok && o <= budget && !getenv("FMC_NO_LJ_LDS");
+    }
+    P.lj_lds = 0;
+    if (!P.lds_all && h->ljo_fits) {
+      P.lj_lds = 1;
+      P.off_vvv64 = h->ljo_off64;
+      P.off_vvv = h->ljo_off32;
+      off = h->ljo_end;
+    }
+    h->plan_end = al(off);
+    plan_resum_rows(P, h->plan_end);
     if (getenv("FMC_VERBOSE"))
-      fprintf(stderr, "fastmc: LDS plan %d bytes, all-in-LDS %d, nsite %d, LJ columns %d (wall %d..%d), guest slots %d\n",
-              P.lds_bytes, P.lds_all, h->nsite, ncol, P.lj_lo, P.lj_hi, P.gatoms_total);
+      fprintf(stderr, "fastmc: LDS plan %d bytes, all-in-LDS %d, LJ alone in LDS %d, nsite %d, LJ columns %d "
+              "(wall %d..%d), guest slots %d, re-sum batch %d\n", P.lds_bytes, P.lds_all, P.lj_lds, h->nsite, ncol,
+              P.lj_lo, P.lj_hi, P.gatoms_total, P.rs_batch);
   }
   // ---- device buffers
   HIPCHK(hipStreamCreate(&h->stream));
@@ -1517,15 +1575,26 @@ int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
   DevParams Pn = P;
   const size_t budget = 160 * 1024 - 1024;
   const size_t need = (size_t)12 * nbins + 32;
-  size_t base = (size_t)Pn.lds_bytes;
-  if (Pn.wl_nbins > 0) base = (size_t)Pn.wl_off;
+  size_t base = h->plan_end;
   if (base + need > budget && Pn.lds_all) {
+    // the all-in-LDS plan gives way; the LJ columns alone stay if they fit
     Pn.lds_all = 0;
+    Pn.lj_lds = 0;
+    base = h->lds_base_end;
+    if (h->ljo_fits && h->ljo_end + need <= budget) {
+      Pn.lj_lds = 1;
+      Pn.off_vvv64 = h->ljo_off64;
+      Pn.off_vvv = h->ljo_off32;
+      base = h->ljo_end;
+    }
+  }
+  if (base + need > budget && Pn.lj_lds) {
+    Pn.lj_lds = 0;
     base = h->lds_base_end;
   }
   if (base + need > budget) return FMC_ERR_ARG;
   Pn.wl_off = (int)((base + 15) & ~(size_t)15);
-  Pn.lds_bytes = (int)((Pn.wl_off + (size_t)12 * nbins + 15) & ~(size_t)15);
+  plan_resum_rows(Pn, Pn.wl_off + (size_t)12 * nbins);
   Pn.wl_nbins = nbins;
   Pn.wl_flat = p->flatcoeff;
   Pn.wl_visittol = p->visittol;
@@ -1572,6 +1641,7 @@ int fmc_wl_setup(fmc_handle *h, const fmc_wl_params *p) {
   HIPCHK(hipMemset(h->dw.wl_hist, 0, sizeof(int) * (size_t)h->nwalkers * nbins));
   HIPCHK(allow_lds_dispatch(h->ns, h->precision == FMC_MIXED, Pn.lds_bytes));
   P = Pn;
+  h->plan_end = base;
   HIPCHK(hipMemcpy(h->dp, &P, sizeof P, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dw.ws, ws.data(), sizeof(WState) * ws.size(), hipMemcpyHostToDevice));
   h->wl_maxn = p->maxn;

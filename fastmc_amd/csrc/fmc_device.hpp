@@ -52,6 +52,10 @@ struct DevParams {
   // lj_hi + 1), all others from FP32 values.  LDS copy: off_vvv64 holds
   // [nvcol][lj_n64] doubles (entries lj_lo ..), off_vvv [nvcol][vstride] floats
   int lj_lo, lj_hi, lj_n64, off_vvv64;
+  int off_rs, rs_batch;           // LDS rows of the in-kernel re-sum: rs_batch atoms x [3][kcp]
+                                  // double2, then their charges and species (0: no room)
+  int lj_lds;                     // without lds_all: the LJ columns alone in LDS (FP64 mode:
+                                  // [nvcol][vstride] doubles at off_vvv64)
   int resum_every;                // full FP64 S(k) re-sum after this many accepted moves (0: never)
   // Wang-Landau sampling in N of guest 1 (wang_landau.f); wl_nbins = 0: off
   int wl_nbins;                   // bins per walker: N = 0 .. wl_nbins-1 (maxn + 2)

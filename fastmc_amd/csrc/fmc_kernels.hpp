@@ -624,13 +624,21 @@ __device__ inline LJTab lj_global(const DevParams &P, bool mixed) {
   return t;
 }
 
+// The LDS copy (stage_tables): FP64 mode the whole columns at off_vvv64;
+// mixed mode the wall entries at off_vvv64 and the whole FP32 columns at off_vvv.
+template <typename T>
 __device__ inline LJTab lj_lds(const DevParams &P, const unsigned char *lds) {
   LJTab t;
   t.v64 = (const double *)(lds + P.off_vvv64);
   t.v32 = (const float *)(lds + P.off_vvv);
-  t.s64 = P.lj_n64;
   t.s32 = P.vstride;
-  t.b64 = P.lj_lo;
+  if constexpr (sizeof(T) == 8) {
+    t.s64 = P.vstride;
+    t.b64 = 0;
+  } else {
+    t.s64 = P.lj_n64;
+    t.b64 = P.lj_lo;
+  }
   t.lo = P.lj_lo;
   t.hi = P.lj_hi;
   return t;
@@ -780,8 +788,11 @@ __device__ inline void site_phase(const float2 *ph, int kcp, int ll, int mm, int
 // constants block always; with LF (the plan made room for everything) also
 // the walker's framework, the k list, the LJ columns of the guest sites (in
 // the precision T of the mode, FP32 only) and the guest coordinates.
-template <typename T, bool LF>
+// LF: framework, k list, guest coordinates and (mixed mode) the LJ columns
+// into LDS; LJL: the LJ columns alone (either precision).
+template <typename T, bool LF, bool LJL>
 __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
+  constexpr bool LV = (LF && sizeof(T) == 4) || LJL;
   const int tid = threadIdx.x, nt = blockDim.x;
   double *erc = (double *)(lds + P.off_erc);
   for (int i = tid; i < P.mxegrd + 3; i += nt) erc[i] = P.erc[i];
@@ -799,7 +810,7 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
         int mol = P.locguest[g];
         if (s < P.natoms[mol]) {
           int ts = P.site_t[mol][s];
-          col = (LF && sizeof(T) == 4) ? P.lpairidx[ts][tj] : P.pairidx[ts][tj];
+          col = LV ? P.lpairidx[ts][tj] : P.pairidx[ts][tj];
           sg2 = P.surf2[ts][tj];
         }
       }
@@ -809,6 +820,30 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
     for (int i = tid; i < (KMAXS + 1) * KSITE; i += nt) {
       lsq[i] = P.site_q[i / KSITE][i % KSITE];
       lst[i] = P.site_t[i / KSITE][i % KSITE];
+    }
+  }
+  if constexpr (LV) {
+    if constexpr (sizeof(T) == 4) {
+      // LJ columns: whole in FP32, and the wall entries again in FP64 (LJTab)
+      double *d64 = (double *)(lds + P.off_vvv64);
+      const int n64 = P.nvcol * P.lj_n64;
+      for (int i = tid; i < n64; i += nt) {
+        int c = i / P.lj_n64, r = i - c * P.lj_n64;
+        d64[i] = P.vvv[(size_t)P.vcol_src[c] * P.vstride + P.lj_lo + r];
+      }
+      float *d32 = (float *)(lds + P.off_vvv);
+      const int n32 = P.nvcol * P.vstride;
+      for (int i = tid; i < n32; i += nt) {
+        int c = i / P.vstride, r = i - c * P.vstride;
+        d32[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + r];
+      }
+    } else {
+      double *d64 = (double *)(lds + P.off_vvv64);
+      const int n64 = P.nvcol * P.vstride;
+      for (int i = tid; i < n64; i += nt) {
+        int c = i / P.vstride, r = i - c * P.vstride;
+        d64[i] = P.vvv[(size_t)P.vcol_src[c] * P.vstride + r];
+      }
     }
   }
   if constexpr (LF) {
@@ -829,21 +864,6 @@ __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int
     for (int i = tid; i < P.nk; i += nt) {
       ak[i] = P.akk[i];
       kl[i] = P.klmn[i];
-    }
-    if constexpr (sizeof(T) == 4) {
-      // LJ columns: whole in FP32, and the wall entries again in FP64 (LJTab)
-      double *d64 = (double *)(lds + P.off_vvv64);
-      const int n64 = P.nvcol * P.lj_n64;
-      for (int i = tid; i < n64; i += nt) {
-        int c = i / P.lj_n64, r = i - c * P.lj_n64;
-        d64[i] = P.vvv[(size_t)P.vcol_src[c] * P.vstride + P.lj_lo + r];
-      }
-      float *d32 = (float *)(lds + P.off_vvv);
-      const int n32 = P.nvcol * P.vstride;
-      for (int i = tid; i < n32; i += nt) {
-        int c = i / P.vstride, r = i - c * P.vstride;
-        d32[i] = P.vvv32[(size_t)P.vcol_src[c] * P.vstride + r];
-      }
     }
     double *lx = (double *)(lds + P.off_g), *ly = lx + P.gatoms_total, *lz = ly + P.gatoms_total;
     const size_t b = (size_t)w * P.gatoms_total;
@@ -893,11 +913,11 @@ __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
 // the LJ columns) are read from LDS.  NSITE > 0: guests have at most NSITE
 // sites (padding sites masked) and the site loops are straight-line code.
 // seam: also write s_new / s_old of the register slots to the scratch rows.
-template <int NS, typename T, bool LF, int NSITE>
+template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                          KSlot<NS> *ks, bool seam) {
   using A = Acc<NS>;
-  constexpr bool LV = LF && sizeof(T) == 4;
+  constexpr bool LV = (LF && sizeof(T) == 4) || LJL;  // LJ columns in LDS (stage_tables)
   constexpr int NSR = NSITE > 0 ? NSITE : KSITE;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int natms = uni(sh.natms), mol = uni(sh.mol), g = uni(sh.g);
@@ -955,7 +975,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
   const int *lst = (const int *)(lds + P.off_cst + CST_LST);
   LJTab vvv;
-  if constexpr (LV) vvv = lj_lds(P, lds);
+  if constexpr (LV) vvv = lj_lds<T>(P, lds);
   else vvv = lj_global(P, sizeof(T) == 4);
   // NSITE > 0: site charges and overlap thresholds in registers; padding
   // sites (s >= natms) carry charge 0, no LJ column, no overlap or surface
@@ -1736,68 +1756,178 @@ __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalker
   }
   const double *lsq = (const double *)(lds + P.off_cst + CST_LSQ);
   const double *r = P.rcell;
-  auto one_k = [&](int k, double2 *S) {
-    int lmn = kl[k];
-    double l = (double)(lmn & 255), m = (double)(((lmn >> 8) & 255) - 128), n = (double)(((lmn >> 16) & 255) - 128);
-    double sr[NS + 1], si[NS + 1];
-#pragma unroll
-    for (int i = 0; i <= NS; ++i) sr[i] = si[i] = 0.0;
-    for (int j = 0; j < P.nfw; ++j) {
-      double x = fx[j], y = fy[j], z = fz[j];
-      double sx = r[0] * x + r[3] * y + r[6] * z, sy = r[1] * x + r[4] * y + r[7] * z,
-             sz = r[2] * x + r[5] * y + r[8] * z;
-      double sn, cs;
-      sincospi(2.0 * (l * sx + m * sy + n * sz), &sn, &cs);
-      double q = fq[j];
-      int sp = (fi[j] >> 8) & 255;
-#pragma unroll
-      for (int i = 0; i < NS; ++i)
-        if (sp == i + 1) {
-          sr[i] += q * cs;
-          si[i] += q * sn;
-        }
-      sr[NS] += q * cs;
-      si[NS] += q * sn;
-    }
+  double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+  double dmax = 0.0;
+  if (P.rs_batch > 0) {
+    // Atoms in batches of rs_batch: each batch's axis phase rows
+    // e^{i 2 pi l s_d}, l = 0..kmax_d, by the recursion of ewald.f:108-147
+    // (one row per atom and axis), its charges and species in LDS; every
+    // thread then adds the batch to its k-vectors with the products of
+    // site_phase.  The k-vectors go in groups of KPT per thread: the
+    // register slots first, then the ones in global memory.
+    const int kcp = P.kcp, nb = P.rs_batch;
+    double2 *rows = (double2 *)(lds + P.off_rs);
+    double *rq = (double *)(rows + (size_t)nb * 3 * kcp);
+    int *rsp = (int *)(rq + nb);
+    int natot = P.nfw, gcut[KMAXS + 2];
+    gcut[0] = P.nfw;
     for (int gg = 1; gg <= P.ntpguest; ++gg) {
-      int molg = P.locguest[gg], na = P.natoms[molg], nact = sh.ws.nmols[molg] * na;
-      for (int j = 0; j < nact; ++j) {
-        int a = P.gbase[gg] + j;
-        double x = G.x[a], y = G.y[a], z = G.z[a];
+      natot += sh.ws.nmols[P.locguest[gg]] * P.natoms[P.locguest[gg]];
+      gcut[gg] = natot;
+    }
+    const int nglob = P.nk > KPT * nt ? P.nk - KPT * nt : 0;
+    const int ngroups = 1 + (nglob + KPT * nt - 1) / (KPT * nt);
+    for (int grp = 0; grp < ngroups; ++grp) {
+      double2 acc[KPT][NS + 1];
+      int kk[KPT];
+#pragma unroll
+      for (int m = 0; m < KPT; ++m) {
+        kk[m] = grp == 0 ? kslot(m) : KPT * nt + (grp - 1) * KPT * nt + m * nt + tid;
+#pragma unroll
+        for (int i = 0; i <= NS; ++i) acc[m][i] = make_double2(0.0, 0.0);
+      }
+      for (int a0 = 0; a0 < natot; a0 += nb) {
+        const int cnt = min(nb, natot - a0);
+        __syncthreads();  // the previous batch is consumed
+        for (int t = tid; t < 3 * cnt; t += nt) {
+          const int ia = t / 3, d = t - 3 * ia, a = a0 + ia;
+          double x, y, z, q;
+          int sp;
+          if (a < P.nfw) {
+            x = fx[a];
+            y = fy[a];
+            z = fz[a];
+            q = fq[a];
+            sp = (fi[a] >> 8) & 255;
+          } else {
+            int gg = 1;
+            while (a >= gcut[gg]) ++gg;
+            const int molg = P.locguest[gg], j = a - gcut[gg - 1], s2 = P.gbase[gg] + j;
+            x = G.x[s2];
+            y = G.y[s2];
+            z = G.z[s2];
+            q = lsq[molg * KSITE + j % P.natoms[molg]];
+            sp = molg;
+          }
+          const double fr = r[3 * 0 + d] * x + r[3 * 1 + d] * y + r[3 * 2 + d] * z;
+          double s1, c1;
+          sincospi(2.0 * fr, &s1, &c1);
+          double2 *row = rows + (size_t)(ia * 3 + d) * kcp;
+          const int kmax = d == 0 ? P.kmax1 : (d == 1 ? P.kmax2 : P.kmax3);
+          row[0] = make_double2(1.0, 0.0);
+          double c = c1, sn = s1;
+          for (int l = 1; l <= kmax; ++l) {
+            row[l] = make_double2(c, sn);
+            const double cn = c * c1 - sn * s1, snn = sn * c1 + c * s1;
+            c = cn;
+            sn = snn;
+          }
+          if (d == 0) {
+            rq[ia] = q;
+            rsp[ia] = sp;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < KPT; ++m) {
+          if (kk[m] >= P.nk) continue;
+          const int lmn = kl[kk[m]];
+          const int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
+          for (int ia = 0; ia < cnt; ++ia) {
+            double ckc, cks;
+            site_phase<double>(rows + (size_t)ia * 3 * kcp, kcp, ll, mm, nn, rq[ia], ckc, cks);
+            const int sp = rsp[ia];
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+              if (sp == i + 1) {
+                acc[m][i].x += ckc;
+                acc[m][i].y += cks;
+              }
+            acc[m][NS].x += ckc;
+            acc[m][NS].y += cks;
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < KPT; ++m) {
+        const int k = kk[m];
+        if (k >= P.nk) continue;
+        if (grp == 0) {
+          dmax = fmax(dmax, fmax(fabs(ks[m].S[NS].x - acc[m][NS].x), fabs(ks[m].S[NS].y - acc[m][NS].y)));
+#pragma unroll
+          for (int i = 0; i <= NS; ++i) ks[m].S[i] = acc[m][i];
+        } else {
+          const double2 o = skw[(size_t)NS * P.nk + k];
+          dmax = fmax(dmax, fmax(fabs(o.x - acc[m][NS].x), fabs(o.y - acc[m][NS].y)));
+#pragma unroll
+          for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = acc[m][i];
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    // no LDS room for the rows: one phase per (atom, k) directly
+    auto one_k = [&](int k, double2 *S) {
+      int lmn = kl[k];
+      double l = (double)(lmn & 255), m = (double)(((lmn >> 8) & 255) - 128), n = (double)(((lmn >> 16) & 255) - 128);
+      double sr[NS + 1], si[NS + 1];
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) sr[i] = si[i] = 0.0;
+      for (int j = 0; j < P.nfw; ++j) {
+        double x = fx[j], y = fy[j], z = fz[j];
         double sx = r[0] * x + r[3] * y + r[6] * z, sy = r[1] * x + r[4] * y + r[7] * z,
                sz = r[2] * x + r[5] * y + r[8] * z;
         double sn, cs;
         sincospi(2.0 * (l * sx + m * sy + n * sz), &sn, &cs);
-        double q = lsq[molg * KSITE + j % na];
+        double q = fq[j];
+        int sp = (fi[j] >> 8) & 255;
 #pragma unroll
         for (int i = 0; i < NS; ++i)
-          if (molg == i + 1) {
+          if (sp == i + 1) {
             sr[i] += q * cs;
             si[i] += q * sn;
           }
         sr[NS] += q * cs;
         si[NS] += q * sn;
       }
+      for (int gg = 1; gg <= P.ntpguest; ++gg) {
+        int molg = P.locguest[gg], na = P.natoms[molg], nact = sh.ws.nmols[molg] * na;
+        for (int j = 0; j < nact; ++j) {
+          int a = P.gbase[gg] + j;
+          double x = G.x[a], y = G.y[a], z = G.z[a];
+          double sx = r[0] * x + r[3] * y + r[6] * z, sy = r[1] * x + r[4] * y + r[7] * z,
+                 sz = r[2] * x + r[5] * y + r[8] * z;
+          double sn, cs;
+          sincospi(2.0 * (l * sx + m * sy + n * sz), &sn, &cs);
+          double q = lsq[molg * KSITE + j % na];
+#pragma unroll
+          for (int i = 0; i < NS; ++i)
+            if (molg == i + 1) {
+              sr[i] += q * cs;
+              si[i] += q * sn;
+            }
+          sr[NS] += q * cs;
+          si[NS] += q * sn;
+        }
+      }
+      double d = fmax(fabs(S[NS].x - sr[NS]), fabs(S[NS].y - si[NS]));
+#pragma unroll
+      for (int i = 0; i <= NS; ++i) S[i] = make_double2(sr[i], si[i]);
+      return d;
+    };
+#pragma unroll
+    for (int m = 0; m < KPT; ++m) {
+      int k = kslot(m);
+      if (k < P.nk) dmax = fmax(dmax, one_k(k, ks[m].S));
     }
-    double d = fmax(fabs(S[NS].x - sr[NS]), fabs(S[NS].y - si[NS]));
+    for (int k = KPT * nt + tid; k < P.nk; k += nt) {
+      double2 S[NS + 1];
 #pragma unroll
-    for (int i = 0; i <= NS; ++i) S[i] = make_double2(sr[i], si[i]);
-    return d;
-  };
-  double dmax = 0.0;
+      for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
+      dmax = fmax(dmax, one_k(k, S));
 #pragma unroll
-  for (int m = 0; m < KPT; ++m) {
-    int k = kslot(m);
-    if (k < P.nk) dmax = fmax(dmax, one_k(k, ks[m].S));
-  }
-  double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
-  for (int k = KPT * nt + tid; k < P.nk; k += nt) {
-    double2 S[NS + 1];
-#pragma unroll
-    for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
-    dmax = fmax(dmax, one_k(k, S));
-#pragma unroll
-    for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = S[i];
+      for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = S[i];
+    }
   }
   // workgroup maximum: per wave, then across waves through LDS
   for (int off = 32; off >= 1; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
@@ -2127,7 +2257,7 @@ __device__ inline void stats_step(const DevParams &P, const DevWalkers &W, int w
 // Device-resident chains: one workgroup per walker, nsteps GCMC steps.  LF /
 // NSITE select the all-in-LDS variant and the site count of its straight-line
 // pair code (see evaluate).
-template <int NS, typename T, bool LF, int NSITE>
+template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, DevWalkers W, int nsteps,
                                              int count_stats) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -2137,7 +2267,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
   KSlot<NS> ks[KPT];
   load_slots<NS>(P, W, w, ks);
   stage_in<NS>(P, W, w, sh);
-  stage_tables<T, LF>(P, W, w, lds);
+  stage_tables<T, LF, LJL>(P, W, w, lds);
   if (P.wl_nbins > 0) {
     const WLView v = wl_view(P, lds);
     for (int b = threadIdx.x; b < P.wl_nbins; b += blockDim.x) {
@@ -2164,7 +2294,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       }
     }
     __syncthreads();
-    if (sh.kind != 0) evaluate<NS, T, LF, NSITE>(P, W, w, sh, lds, ks, false);
+    if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
       decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w);
       if (sh.swap_phase != 1) {
@@ -2248,7 +2378,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
   KSlot<NS> ks[KPT];
   load_slots<NS>(P, W, w, ks);
   stage_in<NS>(P, W, w, sh);
-  stage_tables<T, false>(P, W, w, lds);
+  stage_tables<T, false, false>(P, W, w, lds);
   if (threadIdx.x == 0) {
     WState &ws = sh.ws;
     int g = ws.pend_iguest, mol = P.locguest[g], natms = P.natoms[mol];
@@ -2271,7 +2401,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     }
   }
   __syncthreads();
-  evaluate<NS, T, false, 0>(P, W, w, sh, lds, ks, true);
+  evaluate<NS, T, false, 0, false>(P, W, w, sh, lds, ks, true);
   if (threadIdx.x == 0) {
     decide<NS>(P, sh, 0, lds, true);
     fmc_trial *o = W.trial_out;

@@ -26,13 +26,15 @@ def test_gpu_replays_reference_trace_fp64(name):
     _replay(name)
 
 
-@pytest.mark.parametrize("env", ["FMC_NO_LDS_ALL", "FMC_NO_NSITE"])
+@pytest.mark.parametrize("env", ["FMC_NO_LDS_ALL", "FMC_NO_LDS_ALL+FMC_NO_LJ_LDS", "FMC_NO_NSITE"])
 @pytest.mark.parametrize("name", ["cfg0_moves5", "calf20_co2n2"])
 def test_gpu_replay_kernel_variants(name, env, monkeypatch):
-    """The global-memory kernel variant and the general site-count variant
-    replay the same reference traces (the default plan takes the all-in-LDS,
-    at-most-3-sites variant for these systems)."""
-    monkeypatch.setenv(env, "1")
+    """The kernel variant with only the LJ columns in LDS, the all-global
+    variant and the general site-count variant replay the same reference
+    traces (the default plan takes the all-in-LDS, at-most-3-sites variant
+    for these systems)."""
+    for e in env.split("+"):
+        monkeypatch.setenv(e, "1")
     _replay(name)
 
 
