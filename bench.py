@@ -8,16 +8,27 @@ rcut 12.5 A, Ewald precision 1e-6 (871 k-vectors), mixed precision,
 hipGraph-captured sweeps.  One bench step = one sweep = `--moves` trial moves
 per state point.  Synthetic inputs, deterministic generator.
 
+The state points are first filled to their equilibrium loading (sweeps in
+chunks of 25 until the mean guest count moves by less than 0.3 %), then
+`--warmup` untimed and `--steps` timed sweeps.
+
 Printed as one JSON line (rank 0, N=1):
   value          -- attempted trial moves per second, whole job (all walkers)
   roofline       -- dominant kernel (k_run) against HBM: algorithmic bytes
                     B = 17 (N_fw + N_guest_sites) + 32 n_k per trial
   cpu_baseline   -- the reference's serial build on one state point of the
-                    same workload, this box's host, one core
+                    same workload, this box's host, one core (and, apart,
+                    16 concurrent processes: the MPI build's model)
   fp64_single    -- plain FP64, one state point, same device kernel
+  fp64_batched   -- plain FP64 on the same 64 state points, started from the
+                    equilibrated mixed walkers' molecules
   widom_grid     -- Widom insertions on a grid (config-0 framework, CO2):
                     attempts/s on the device and of the reference's serial
                     build on this box's host
+  config1_supercell, config3_gradient, config4_wang_landau
+                 -- the 3392-atom supercell (one walker, FP64 and mixed),
+                    dU/dtheta of 256 configurations, and the 128-walker
+                    Wang-Landau launch (skipped with --no-configs)
 """
 from __future__ import annotations
 
@@ -43,7 +54,10 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed sweeps")
-    ap.add_argument("--warmup", type=int, default=30, help="untimed sweeps (loading the isotherm)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed sweeps after the equilibration")
+    ap.add_argument("--fill", type=int, default=400,
+                    help="most equilibration sweeps before the warm-up (stops once the mean loading settles)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config 1, 3 and 4 measurements")
     ap.add_argument("--moves", type=int, default=1000, help="trial moves per state point per sweep")
     ap.add_argument("--walkers", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -232,6 +246,156 @@ def oracle_cpu(sysm, seconds: float):
             "sample": f"C oracle, one state point, {n} steps"}
 
 
+def note(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_state(src, dst, walkers):
+    """Give dst's walkers the guest molecules of src's (same chemistry):
+    fmc_load_guests rebuilds counts, S(k) and energies from the coordinates."""
+    for w in range(walkers):
+        for g in range(1, len(src.sysm.guests) + 1):
+            dst.load_guests(g, src.guests(g, w=w), w=w)
+
+
+def mean_guests(h, walkers):
+    return sum(h.stats(w).nmols[0] for w in range(walkers)) / walkers
+
+
+def equilibrate(h, walkers, moves, max_sweeps, chunk=25, tol=3e-3):
+    """Equilibration of the state points before anything is timed: sweeps in
+    chunks until the mean loading moves by less than tol between chunks."""
+    done, prev = 0, None
+    while done < max_sweeps:
+        h.run_graph(chunk, moves)
+        done += chunk
+        n = mean_guests(h, walkers)
+        if prev is not None and abs(n - prev) <= tol * max(prev, 1.0):
+            break
+        prev = n
+    return done, mean_guests(h, walkers)
+
+
+def timed_sweeps(h, sweeps, moves, warmup):
+    """warmup untimed sweeps, then `sweeps` timed graph-replayed sweeps (the
+    graph captured and replayed once untimed first)."""
+    if warmup > 0:
+        h.run_graph(warmup, moves)
+    h.run_graph(sweeps, moves)
+    h.reset_stats()
+    device_sync()
+    t0 = time.perf_counter()
+    h.run_graph(sweeps, moves, count_stats=True)
+    device_sync()
+    return time.perf_counter() - t0
+
+
+def config1_supercell(steps: int = 10000, fill: int = 60000):
+    """Config 1: the 2x2x2 supercell (3392 atoms, 6650 k-vectors), CO2 at
+    10 bar, one walker, FP64 and mixed, timed at the same loading."""
+    from fastmc_amd import inputs as I
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    from fastmc_amd.system import read_system
+    d = tempfile.mkdtemp(prefix="fmc_cfg1_")
+    try:
+        I.config1(d, pressure_bar=10.0)
+        s = read_system(d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    out = {"workload": "config 1: 2x2x2 supercell (3392 framework atoms, a = 51.6 A, 6650 k-vectors), CO2 at "
+                       f"10 bar, one walker, timed for {steps} steps after a {fill}-step fill from the empty "
+                       "framework (the mixed walker takes the FP64 walker's molecules)"}
+    hf = Handle(s, precision=FMC_FP64, seeds=[(5, 6, 7, 8)])
+    hm = Handle(s, precision=FMC_MIXED, seeds=[(5, 6, 7, 8)])
+    try:
+        for _ in range(fill // 20000):
+            hf.run(20000)
+        copy_state(hf, hm, 1)
+        for name, h in (("fp64", hf), ("mixed", hm)):
+            h.set_block(512)
+            h.run(2000)
+            device_sync()
+            t0 = time.perf_counter()
+            h.run(steps)
+            device_sync()
+            dt = time.perf_counter() - t0
+            out[name] = {"trial_moves_per_s": steps / dt, "us_per_step": dt / steps * 1e6,
+                         "guests": int(h.stats(0).nmols[0]), "resums": int(h.stats(0).resums)}
+        out["mixed_vs_fp64"] = out["mixed"]["trial_moves_per_s"] / out["fp64"]["trial_moves_per_s"]
+    finally:
+        hf.close()
+        hm.close()
+    return out
+
+
+def config3_gradient(nconf: int = 256, reps: int = 5):
+    """Config 3: dU/dtheta (424 framework charges, LJ eps / sigma of the
+    types) of 256 configurations sampled from config 0 at 1 bar."""
+    from fastmc_amd import adjoint as A
+    from fastmc_amd import inputs as I
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    from fastmc_amd.system import read_system
+    d = tempfile.mkdtemp(prefix="fmc_cfg3_")
+    try:
+        I.config0(d, pressure_bar=1.0)
+        s = read_system(d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    nmols, xyz = A.sample_configurations(s, nconf, every=200, warmup=20000)
+    q, eps, sig = A.theta0(s)
+    out = {"workload": f"config 3: U and dU/dtheta (424 framework charges, LJ eps and sigma of the 4 framework "
+                       f"types) of {nconf} configurations sampled every 200 steps from config 0 at 1 bar "
+                       f"({int(nmols[:, 0].min())}-{int(nmols[:, 0].max())} CO2 each), one fmc_energy_grad call"}
+    h = Handle(s, precision=FMC_FP64)
+    try:
+        for prec, name in ((FMC_MIXED, "mixed"), (FMC_FP64, "fp64")):
+            h.energy_grad(q, eps, sig, nmols, xyz, prec)
+            device_sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                h.energy_grad(q, eps, sig, nmols, xyz, prec)
+            device_sync()
+            dt = (time.perf_counter() - t0) / reps
+            out[name] = {"configurations_per_s": nconf / dt, "ms_per_call": dt * 1e3}
+    finally:
+        h.close()
+    return out
+
+
+def config4_wang_landau(walkers: int = 128, steps: int = 2000):
+    """Config 4: Wang-Landau in N(CO2) 0-400 for CO2/N2 on the supercell,
+    128 walkers over 8 windows in one launch, mixed precision."""
+    from fastmc_amd import inputs as I
+    from fastmc_amd.lib import FMC_MIXED, Handle
+    from fastmc_amd.system import read_system
+    d = tempfile.mkdtemp(prefix="fmc_cfg4_")
+    try:
+        I.config4(d)
+        s = read_system(d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    seeds = [(1 + (7 * i) % 177, 1 + (13 * i) % 177, 1 + (29 * i) % 177, (37 * i) % 168) for i in range(walkers)]
+    h = Handle(s, precision=FMC_MIXED, seeds=seeds, n=walkers)
+    try:
+        h.set_block(512)
+        h.wl_setup(maxn=400, nwindows=8, visittol=10000, flatcoeff=0.8, prectol=1e-6)
+        h.run(2 * steps)  # preload to each window's lower edge, first flat-histogram steps
+        device_sync()
+        t0 = time.perf_counter()
+        h.run(steps)
+        device_sync()
+        dt = time.perf_counter() - t0
+        n = [h.stats(w).nmols[0] for w in range(walkers)]
+    finally:
+        h.close()
+    return {"workload": "config 4: Wang-Landau in N(CO2) 0-400 for CO2/N2 (N2 grand-canonical at 0.8 bar) on "
+                        "the 2x2x2 supercell, T = 293 K, 128 walkers over 8 N-windows in one launch, ln f from 1 at "
+                        f"80 % flatness, mixed precision; timed {steps} steps after {2 * steps} (preload)",
+            "walker_steps_per_s": walkers * steps / dt, "us_per_step": dt / steps * 1e6,
+            "co2_min_max": [int(min(n)), int(max(n))]}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -259,6 +423,10 @@ def main():
         h = Handle(base, walkers=walkers, precision=FMC_MIXED)
         h.set_block(args.block)  # 0: autotune threads per walker on a snapshot
         block = h.get_block()
+        # the state points are filled to their equilibrium loading first
+        note(f"{args.walkers} walkers, {block} threads each; equilibrating")
+        fill_sweeps, fill_n = equilibrate(h, args.walkers, args.moves, args.fill) if args.fill > 0 else (0, 0.0)
+        note(f"equilibrated after {fill_sweeps} sweeps, mean N {fill_n:.1f}; timing {args.steps} sweeps")
         if args.eager:
             for _ in range(args.warmup):
                 h.run(args.moves)
@@ -272,16 +440,7 @@ def main():
             device_sync()
             wall = time.perf_counter() - t0
         else:
-            # warm-up: fills each state point to its loading, then captures the graph
-            for _ in range(max(1, args.warmup // 10)):
-                h.run_graph(10, args.moves)
-            h.run_graph(args.steps, args.moves)   # capture + one untimed replay of the timed graph
-            h.reset_stats()
-            device_sync()
-            t0 = time.perf_counter()
-            h.run_graph(args.steps, args.moves, count_stats=True)
-            device_sync()
-            wall = time.perf_counter() - t0
+            wall = timed_sweeps(h, args.steps, args.moves, args.warmup)
         gpu_ms, launches = h.last_timing()
         if args.eager:
             gpu_ms, launches = ms_tot, args.steps
@@ -299,19 +458,32 @@ def main():
         B = algorithmic_bytes(h.nframework, n_avg * nsites, h.newld)
         launch_s = gpu_ms / 1e3 / max(launches, 1)
         achieved = B * args.walkers * args.moves / launch_s
-        traffic = None
+        traffic = valu = None
         pmc = os.path.join(ROOT, "profiles", "k_run_batched_isotherm_counters.json")
         if os.path.exists(pmc):
             try:
-                per_trial = json.load(open(pmc)).get("hbm_bytes_per_trial")
+                cj = json.load(open(pmc))
+                per_trial = cj.get("hbm_bytes_per_trial")
                 traffic = per_trial * args.walkers * args.moves if per_trial else None
+                valu = cj.get("valu")
             except Exception:
-                traffic = None
+                traffic = valu = None
         import math
         mid_i = min(range(len(systems)), key=lambda i: (abs(systems[i].temperature - 293.0),
                                                         abs(math.log10(systems[i].pressures[0] / 1e5) + 1.0)))
-        fp64 = None
+        fp64 = fp64_batched = None
         if not args.no_fp64:
+            note(f"mixed {value:.4g} trial moves/s; FP64 batch")
+            # plain FP64 on the same 64 state points at the same loading
+            hd = Handle(base, walkers=walkers, precision=FMC_FP64)
+            hd.set_block(block)
+            copy_state(h, hd, args.walkers)
+            wd = timed_sweeps(hd, min(args.steps, 10), args.moves, 1)
+            fp64_batched = {"value": args.walkers * args.moves * min(args.steps, 10) / wd, "unit": "trial moves/s",
+                            "config": "plain FP64 on the same 64 state points, started from the mixed walkers' "
+                                      "molecules, same launch shape and graph sweeps",
+                            "mean_guests": round(mean_guests(hd, args.walkers), 2)}
+            hd.close()
             mid = systems[mid_i]  # T = 293 K, P = 0.1 bar: the config-0 state point
             hs = Handle(base, walkers=[make_walker(base, seed=(11, 23, 37, 41), temperature=mid.temperature,
                                                    pressures=mid.pressures, framework_xyz=mid.framework_xyz(),
@@ -330,6 +502,7 @@ def main():
             hs.close()
         cpu = cpu_mpi = None
         if not args.no_cpu:
+            note("CPU baseline")
             cpu = reference_cpu(dirs[mid_i], args.cpu_seconds)
             if cpu is None:
                 cpu = oracle_cpu(systems[mid_i], args.cpu_seconds)
@@ -346,6 +519,7 @@ def main():
             wdir = os.path.join(tmp, "widom")
             os.makedirs(wdir, exist_ok=True)
             I.config0(wdir, pressure_bar=1.0)
+            note("Widom grid")
             widom = {"workload": "config-0 framework (424 atoms, 871 k-vectors), CO2, grid spacing 0.5 A, "
                                  "20 orientations per point, empty framework, one walker"}
             widom.update(widom_device(read_system(wdir)))
@@ -354,6 +528,14 @@ def main():
                 if widom["cpu_reference"]:
                     widom["fp64_vs_cpu_reference"] = (widom["fp64"]["attempts_per_s"] /
                                                       widom["cpu_reference"]["attempts_per_s"])
+        extra = {}
+        if not args.no_configs:
+            note("config 1")
+            extra["config1_supercell"] = config1_supercell()
+            note("config 3")
+            extra["config3_gradient"] = config3_gradient()
+            note("config 4")
+            extra["config4_wang_landau"] = config4_wang_landau()
         line = {
             "metric": METRIC, "value": value, "unit": "trial moves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
@@ -361,20 +543,24 @@ def main():
             "dtype": "fp32+fp64", "data": "synthetic",
             "config": {"workload": "batched isotherm: 64 state points (T 273/293/323 K x P 1e-3..10 bar), "
                                    "424-atom jittered cubic frameworks, CO2, rcut 12.5 A, 871 k-vectors, "
-                                   "mixed precision, hipGraph sweeps",
+                                   "mixed precision, hipGraph sweeps, equilibrated before timing",
                        "walkers": args.walkers, "moves_per_walker_per_step": args.moves,
                        "threads_per_walker": block, "mean_guests": round(n_avg, 2),
-                       "guests_min_max": [nmin, nmax]},
+                       "guests_min_max": [nmin, nmax],
+                       "equilibration_sweeps": fill_sweeps, "mean_guests_after_equilibration": round(fill_n, 2)},
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK, "traffic": traffic,
                          "kernel": "k_run", "bytes_per_trial": B,
-                         "avg_launch_ms": launch_s * 1e3},
+                         "avg_launch_ms": launch_s * 1e3, "valu": valu},
             "cpu_baseline": cpu,
             "cpu_baseline_concurrent": cpu_mpi,
             "fp64_single": fp64,
             "batched_vs_fp64_single": (value / fp64["value"]) if fp64 else None,
+            "fp64_batched": fp64_batched,
+            "mixed_vs_fp64_batched": (value / fp64_batched["value"]) if fp64_batched else None,
             "widom_grid": widom,
         }
+        line.update(extra)
         print(json.dumps(line), flush=True)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
