@@ -63,7 +63,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--block", type=int, default=0, help="threads per walker (0: autotune)")
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0: autotune)")
+    ap.add_argument("--team", type=int, default=0, help="workgroups per walker (0: autotune)")
     ap.add_argument("--no-widom", action="store_true")
     ap.add_argument("--eager", action="store_true",
                     help="plain launches instead of graph replays (for counter collection)")
@@ -313,7 +314,7 @@ def config1_supercell(steps: int = 10000, fill: int = 60000):
             hf.run(20000)
         copy_state(hf, hm, 1)
         for name, h in (("fp64", hf), ("mixed", hm)):
-            h.set_block(512)
+            h.set_block(0)  # threads and workgroups per walker, tuned on a snapshot
             h.run(2000)
             device_sync()
             t0 = time.perf_counter()
@@ -321,7 +322,8 @@ def config1_supercell(steps: int = 10000, fill: int = 60000):
             device_sync()
             dt = time.perf_counter() - t0
             out[name] = {"trial_moves_per_s": steps / dt, "us_per_step": dt / steps * 1e6,
-                         "guests": int(h.stats(0).nmols[0]), "resums": int(h.stats(0).resums)}
+                         "guests": int(h.stats(0).nmols[0]), "resums": int(h.stats(0).resums),
+                         "threads_per_walker": h.get_block(), "workgroups_per_walker": h.get_team()}
         out["mixed_vs_fp64"] = out["mixed"]["trial_moves_per_s"] / out["fp64"]["trial_moves_per_s"]
     finally:
         hf.close()
@@ -378,8 +380,8 @@ def config4_wang_landau(walkers: int = 128, steps: int = 2000):
     seeds = [(1 + (7 * i) % 177, 1 + (13 * i) % 177, 1 + (29 * i) % 177, (37 * i) % 168) for i in range(walkers)]
     h = Handle(s, precision=FMC_MIXED, seeds=seeds, n=walkers)
     try:
-        h.set_block(512)
         h.wl_setup(maxn=400, nwindows=8, visittol=10000, flatcoeff=0.8, prectol=1e-6)
+        h.set_block(0)
         h.run(2 * steps)  # preload to each window's lower edge, first flat-histogram steps
         device_sync()
         t0 = time.perf_counter()
@@ -387,13 +389,14 @@ def config4_wang_landau(walkers: int = 128, steps: int = 2000):
         device_sync()
         dt = time.perf_counter() - t0
         n = [h.stats(w).nmols[0] for w in range(walkers)]
+        shape = {"threads_per_walker": h.get_block(), "workgroups_per_walker": h.get_team()}
     finally:
         h.close()
     return {"workload": "config 4: Wang-Landau in N(CO2) 0-400 for CO2/N2 (N2 grand-canonical at 0.8 bar) on "
                         "the 2x2x2 supercell, T = 293 K, 128 walkers over 8 N-windows in one launch, ln f from 1 at "
                         f"80 % flatness, mixed precision; timed {steps} steps after {2 * steps} (preload)",
             "walker_steps_per_s": walkers * steps / dt, "us_per_step": dt / steps * 1e6,
-            "co2_min_max": [int(min(n)), int(max(n))]}
+            "co2_min_max": [int(min(n)), int(max(n))], **shape}
 
 
 def main():
@@ -421,12 +424,17 @@ def main():
                                        temperature=s.temperature, pressures=s.pressures,
                                        framework_xyz=s.framework_xyz(), keep=keep))
         h = Handle(base, walkers=walkers, precision=FMC_MIXED)
-        h.set_block(args.block)  # 0: autotune threads per walker on a snapshot
-        block = h.get_block()
         # the state points are filled to their equilibrium loading first
-        note(f"{args.walkers} walkers, {block} threads each; equilibrating")
+        note(f"{args.walkers} walkers; equilibrating")
         fill_sweeps, fill_n = equilibrate(h, args.walkers, args.moves, args.fill) if args.fill > 0 else (0, 0.0)
-        note(f"equilibrated after {fill_sweeps} sweeps, mean N {fill_n:.1f}; timing {args.steps} sweeps")
+        # then the launch shape: threads per workgroup and workgroups per
+        # walker, tuned on a snapshot of the equilibrated walkers (0 = tune)
+        if args.team > 0:
+            h.set_team(args.team)
+        h.set_block(args.block)
+        block, team = h.get_block(), h.get_team()
+        note(f"equilibrated after {fill_sweeps} sweeps, mean N {fill_n:.1f}; {team} x {block} threads per walker; "
+             f"timing {args.steps} sweeps")
         if args.eager:
             for _ in range(args.warmup):
                 h.run(args.moves)
@@ -476,6 +484,7 @@ def main():
             note(f"mixed {value:.4g} trial moves/s; FP64 batch")
             # plain FP64 on the same 64 state points at the same loading
             hd = Handle(base, walkers=walkers, precision=FMC_FP64)
+            hd.set_team(team)
             hd.set_block(block)
             copy_state(h, hd, args.walkers)
             wd = timed_sweeps(hd, min(args.steps, 10), args.moves, 1)
@@ -498,7 +507,7 @@ def main():
             fp64 = {"value": 5 * args.moves / w1, "unit": "trial moves/s",
                     "config": f"plain FP64, one state point (T={mid.temperature:g} K, "
                               f"P={mid.pressures[0] / 1e5:.4g} bar), same device kernel, 1 walker",
-                    "threads_per_walker": hs.get_block()}
+                    "threads_per_walker": hs.get_block(), "workgroups_per_walker": hs.get_team()}
             hs.close()
         cpu = cpu_mpi = None
         if not args.no_cpu:
@@ -545,7 +554,8 @@ def main():
                                    "424-atom jittered cubic frameworks, CO2, rcut 12.5 A, 871 k-vectors, "
                                    "mixed precision, hipGraph sweeps, equilibrated before timing",
                        "walkers": args.walkers, "moves_per_walker_per_step": args.moves,
-                       "threads_per_walker": block, "mean_guests": round(n_avg, 2),
+                       "threads_per_walker": block, "workgroups_per_walker": team,
+                       "mean_guests": round(n_avg, 2),
                        "guests_min_max": [nmin, nmax],
                        "equilibration_sweeps": fill_sweeps, "mean_guests_after_equilibration": round(fill_n, 2)},
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

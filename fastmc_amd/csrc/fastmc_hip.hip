@@ -81,6 +81,8 @@ int scratch_address(void **out) {
 
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
+  int team = 1;                  // workgroups per walker in fmc_run (k_run)
+  int tune_team_max = KTEAM;     // largest team the block autotuner tries
   void *scratch = nullptr;       // this device's address of `scratch` (fmc_widom_noalloc)
   int trace_w = 0;               // walker whose steps fmc_run records
   int nsite = 0;                 // 3: every guest has at most 3 sites (straight-line pair code)
@@ -217,18 +219,112 @@ double ewald3_template(const DevParams &P, int mol, int g, int natms) {
   return tot;
 }
 
+// Workgroups of a k_run launch: one per walker, or with teams 8 * ceil(nw / 8)
+// * team (the members of a walker sit on one XCD, see k_run).
+int run_grid(const fmc_handle *h, int team) {
+  return team > 1 ? 8 * ((h->nwalkers + 7) / 8) * team : h->nwalkers;
+}
+
+// The k_run variant the LDS plan selects (its index in allow_lds's list).
+int run_variant(const fmc_handle *h) {
+  if (!h->hp.lds_all && h->hp.lj_lds) return 1;
+  if (!h->hp.lds_all) return 0;
+  return h->nsite == 3 ? 2 : 3;
+}
+
+template <int NS, typename T>
+const void *run_fn(int variant) {
+  switch (variant) {
+    case 1: return (const void *)k_run<NS, T, false, 0, true>;
+    case 2: return (const void *)k_run<NS, T, true, 3, false>;
+    case 3: return (const void *)k_run<NS, T, true, 0, false>;
+    default: return (const void *)k_run<NS, T, false, 0, false>;
+  }
+}
+
 template <int NS, typename T>
 void launch_run(fmc_handle *h, int nsteps, int count) {
-  dim3 grid(h->nwalkers), block(h->block);
+  const int team = h->team, nw = h->nwalkers;
+  dim3 grid(run_grid(h, team)), block(h->block);
   size_t lds = h->hp.lds_bytes;
-  if (!h->hp.lds_all && h->hp.lj_lds)
-    hipLaunchKernelGGL((k_run<NS, T, false, 0, true>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
-  else if (!h->hp.lds_all)
-    hipLaunchKernelGGL((k_run<NS, T, false, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
-  else if (h->nsite == 3)
-    hipLaunchKernelGGL((k_run<NS, T, true, 3, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
-  else
-    hipLaunchKernelGGL((k_run<NS, T, true, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count);
+  switch (run_variant(h)) {
+    case 1:
+      hipLaunchKernelGGL((k_run<NS, T, false, 0, true>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count,
+                         team, nw);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_run<NS, T, true, 3, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count,
+                         team, nw);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_run<NS, T, true, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count,
+                         team, nw);
+      break;
+    default:
+      hipLaunchKernelGGL((k_run<NS, T, false, 0, false>), grid, block, lds, h->stream, h->dp, h->dw, nsteps, count,
+                         team, nw);
+  }
+}
+
+// Workgroups of the current k_run variant the device holds at once with
+// `block` threads each (the bound a team launch must stay within: a member
+// waits on the others, so all of them must be resident together).
+int run_capacity(const fmc_handle *h, int block) {
+  const void *fn = nullptr;
+  const bool mixed = h->precision == FMC_MIXED;
+  const int v = run_variant(h);
+  switch (h->ns) {
+    case 2: fn = mixed ? run_fn<2, float>(v) : run_fn<2, double>(v); break;
+    case 3: fn = mixed ? run_fn<3, float>(v) : run_fn<3, double>(v); break;
+    default: fn = mixed ? run_fn<4, float>(v) : run_fn<4, double>(v); break;
+  }
+  int per_cu = 0, dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, h->hp.lds_bytes) != hipSuccess) return 0;
+  return per_cu * prop.multiProcessorCount;
+}
+
+// A team launch must fit on the device at once.
+int team_fits(const fmc_handle *h) {
+  if (h->team <= 1) return FMC_OK;
+  return run_grid(h, h->team) <= run_capacity(h, h->block) ? FMC_OK : FMC_ERR_ARG;
+}
+
+// After a team run: a wait that gave up (the walkers it touched carry
+// overflow bit FMC_OVF_TEAM) makes the call fail; the flag is cleared.
+int team_error(fmc_handle *h) {
+  if (h->team <= 1) return FMC_OK;
+  int e = 0;
+  HIPCHK(hipMemcpy(&e, h->dw.xerr, sizeof(int), hipMemcpyDeviceToHost));
+  if (!e) return FMC_OK;
+  // members may have stopped at different exchanges: clear every record and
+  // count, so that nothing of this run can pass for a later exchange (the
+  // flagged walkers' states are not to be trusted)
+  HIPCHK(hipMemset(h->dw.xch, 0, sizeof(double) * (size_t)h->nwalkers * 2 * KTEAM * XREC));
+  HIPCHK(hipMemset(h->dw.xcnt, 0, sizeof(long long) * (size_t)h->nwalkers));
+  HIPCHK(hipMemset(h->dw.xerr, 0, sizeof(int)));
+  return FMC_ERR_TEAM;
+}
+
+// The private guest arrays of members 1 .. team - 1 (grown on demand).
+int team_arrays(fmc_handle *h, int team) {
+  if (team <= h->dw.team_alloc) return FMC_OK;
+  double *p = nullptr;
+  const size_t n = (size_t)h->nwalkers * (team - 1) * 3 * h->hp.gatoms_total;
+  if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return FMC_ERR_ALLOC;
+  if (h->dw.gxt) {
+    for (auto it = h->allocs.begin(); it != h->allocs.end(); ++it)
+      if (*it == (void *)h->dw.gxt) {
+        h->allocs.erase(it);
+        break;
+      }
+    (void)hipFree(h->dw.gxt);
+  }
+  h->allocs.push_back(p);
+  h->dw.gxt = p;
+  h->dw.team_alloc = team;
+  return FMC_OK;
 }
 
 size_t smem_size(int ns) {
@@ -1006,6 +1102,12 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   rc |= dalloc(h, &h->dw.sk, (size_t)nwalkers * (P.maxmls + 1) * P.nk);
   rc |= dalloc(h, &h->dw.sscr, (size_t)nwalkers * 2 * P.nk);
   rc |= dalloc(h, &h->dw.sbak, (size_t)nwalkers * (P.maxmls + 1) * P.nk);
+  // team exchange records and counters (zero: no record is current)
+  rc |= dalloc(h, &h->dw.xch, (size_t)nwalkers * 2 * KTEAM * XREC);
+  rc |= dalloc(h, &h->dw.xcnt, (size_t)nwalkers);
+  rc |= dalloc(h, &h->dw.xerr, 1);
+  h->dw.gxt = nullptr;
+  h->dw.team_alloc = 1;
   rc |= dalloc(h, (char **)&h->dw.trial_out, 4096);
   rc |= dalloc(h, &h->dscratch, (size_t)nwalkers * 3 * (KMAXC + 1) + 64);
   bool per_walker_fw = false;
@@ -1021,6 +1123,9 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
     fmc_destroy(h);
     return FMC_ERR_ALLOC;
   }
+  HIPCHK(hipMemset(h->dw.xch, 0, sizeof(double) * (size_t)nwalkers * 2 * KTEAM * XREC));
+  HIPCHK(hipMemset(h->dw.xcnt, 0, sizeof(long long) * (size_t)nwalkers));
+  HIPCHK(hipMemset(h->dw.xerr, 0, sizeof(int)));
   std::vector<float> erc32(erc.begin(), erc.end()), vvv32(vvv.begin(), vvv.end());
   HIPCHK(hipMemcpy(d_erc, erc.data(), erc.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d_vvv, vvv.data(), vvv.size() * 8, hipMemcpyHostToDevice));
@@ -1392,6 +1497,7 @@ int fmc_reset_stats(fmc_handle *h) {
 int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   if (!h || nsteps < 0) return FMC_ERR_ARG;
   if (nsteps == 0) return FMC_OK;
+  if (team_fits(h) != FMC_OK) return FMC_ERR_ARG;
   DevWalkers dw = h->dw;
   if (trace) {
     if (h->dtrace_cap < nsteps) {
@@ -1419,7 +1525,7 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   h->last_launches = 1;
   if (trace)
     HIPCHK(hipMemcpy(trace, h->dtrace, sizeof(fmc_trace_rec) * (size_t)nsteps, hipMemcpyDeviceToHost));
-  return FMC_OK;
+  return team_error(h);
 }
 
 int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_stats) {
@@ -1427,6 +1533,7 @@ int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_st
   GraphKey key{nsweeps, nsteps_per_sweep, count_stats};
   auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {
+    if (team_fits(h) != FMC_OK) return FMC_ERR_ARG;
     hipGraph_t graph;
     DevWalkers saved = h->dw;
     h->dw.trace = nullptr;
@@ -1447,7 +1554,7 @@ int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_st
   HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
   h->last_launches = nsweeps;
-  return FMC_OK;
+  return team_error(h);
 }
 
 int fmc_set_trace_walker(fmc_handle *h, int w) {
@@ -1494,30 +1601,43 @@ int fmc_set_block(fmc_handle *h, int threads) {
         break;
       }
     }
-    const int prev = h->block;
-    int best = prev;
+    // the tiles: threads per workgroup x workgroups per walker (each member
+    // of a team takes 1 / team of the pair atoms and of the k-vectors), every
+    // team the device holds at once
+    const int prev = h->block, prev_team = h->team;
+    int best = prev, best_team = prev_team;
     double best_ms = 1e30;
     const int cand[3] = {128, 256, 512};
-    for (int i = 0; i < 3 && rc == FMC_OK; ++i) {
-      h->block = cand[i];
-      int e = fmc_run(h, 64, nullptr, 0);
-      if (e == FMC_OK) e = fmc_run(h, 256, nullptr, 0);
-      if (e == FMC_OK && h->last_ms < best_ms) {
-        best_ms = h->last_ms;
-        best = cand[i];
+    const int teams[8] = {1, 2, 3, 4, 6, 8, 12, 16};
+    for (int ti = 0; ti < 8 && rc == FMC_OK; ++ti) {
+      const int team = teams[ti];
+      if (team > 1 && (h->tune_team_max < team || team_arrays(h, team) != FMC_OK)) continue;
+      for (int i = 0; i < 3 && rc == FMC_OK; ++i) {
+        if (team > 1 && run_grid(h, team) > run_capacity(h, cand[i])) continue;
+        h->block = cand[i];
+        h->team = team;
+        int e = fmc_run(h, 64, nullptr, 0);
+        if (e == FMC_OK) e = fmc_run(h, 256, nullptr, 0);
+        if (e == FMC_OK && h->last_ms < best_ms) {
+          best_ms = h->last_ms;
+          best = cand[i];
+          best_team = team;
+        }
+        for (auto &sn : snap)
+          if (hipMemcpy(sn.dev, sn.copy, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = FMC_ERR_HIP;
+        if (e != FMC_OK && rc == FMC_OK) rc = e;
       }
-      for (auto &sn : snap)
-        if (hipMemcpy(sn.dev, sn.copy, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = FMC_ERR_HIP;
-      if (e != FMC_OK && rc == FMC_OK) rc = e;
     }
     for (auto &sn : snap)
       if (sn.copy) (void)hipFree(sn.copy);
     h->block = rc == FMC_OK ? best : prev;
+    h->team = rc == FMC_OK ? best_team : prev_team;
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
     return rc;
   }
   if (threads < 64 || threads > 512 || threads % 64) return FMC_ERR_ARG;
+  if (h->team > 1 && run_grid(h, h->team) > run_capacity(h, threads)) return FMC_ERR_ARG;
   h->block = threads;
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
@@ -1527,6 +1647,29 @@ int fmc_set_block(fmc_handle *h, int threads) {
 int fmc_get_block(const fmc_handle *h, int *threads) {
   if (!h || !threads) return FMC_ERR_ARG;
   *threads = h->block;
+  return FMC_OK;
+}
+
+int fmc_set_team(fmc_handle *h, int team) {
+  if (!h || team < 0 || team > KTEAM) return FMC_ERR_ARG;
+  if (team == 0) {
+    // let the autotuner of fmc_set_block(0) consider teams (up to KTEAM)
+    h->tune_team_max = KTEAM;
+    return FMC_OK;
+  }
+  if (team > 1 && run_grid(h, team) > run_capacity(h, h->block)) return FMC_ERR_ARG;
+  const int rc = team_arrays(h, team);
+  if (rc != FMC_OK) return rc;
+  h->team = team;
+  h->tune_team_max = team;
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return FMC_OK;
+}
+
+int fmc_get_team(const fmc_handle *h, int *team) {
+  if (!h || !team) return FMC_ERR_ARG;
+  *team = h->team;
   return FMC_OK;
 }
 

@@ -20,6 +20,14 @@ constexpr int KTYPE = FMC_MAX_TYPES;
 constexpr int KCAP = 20;          // kmax + 1 cap for the per-site phase tables
 constexpr int KTRI = KTYPE * (KTYPE + 1) / 2;
 constexpr int KWAVES = 16;        // up to 1024 threads per workgroup
+// Teams: a walker's step split over up to KTEAM workgroups (pair atoms and
+// k-vectors by member), their partial sums exchanged once per step through
+// XREC-double records in global memory, the last slot of each carrying the
+// exchange number.  A wait that sees nothing for XSPIN polls gives up and
+// flags the walker (FMC_OVF_TEAM), so that no wait is unbounded.
+constexpr int KTEAM = 16;
+constexpr int XREC = 32;
+constexpr long long XSPIN = 1LL << 22;
 
 constexpr double R4PIE0 = 138935.4835;      // utility_pack.f:113
 constexpr double KBOLTZ = 1.9872041e-3;     // utility_pack.f:121
@@ -186,6 +194,16 @@ struct DevWalkers {
   fmc_window *win;           // [w][maxwin] closed averaging windows
   double *prob;              // [w][nplots][pgridsize] probability plots
   fmc_numguests_rec *numg;   // [w][numg_cap] guest-count trace
+  // teams: exchange records [w][2 (exchange parity)][KTEAM][XREC], the
+  // number of exchanges each walker has made (never reset: a record whose
+  // number is not the current one is old), and the private guest
+  // coordinates of members 1.. when they live in global memory
+  // [w][team_alloc - 1][3][gatoms_total]
+  double *xch;
+  long long *xcnt;
+  double *gxt;
+  int team_alloc;
+  int *xerr;                 // set when a team wait gave up (read and cleared by the host)
 };
 
 __host__ __device__ inline int loc2(int i, int j) {  // utility_pack.f:2859-2872

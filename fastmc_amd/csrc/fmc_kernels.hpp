@@ -57,12 +57,17 @@ struct Smem {
   // constants of the current trial, computed while the energy terms run
   double qfix0, pref;
   double qfm[KMAXS + 1];
+  // team of the walker: size, this workgroup's member index and its
+  // k-vectors [k0, k1); exchanges made so far; set when a wait gave up
+  int team, member, k0, k1;
+  long long xseq;
+  int team_fail;
 };
 
 constexpr int KPT = 2;  // k-vectors per thread held in registers for a whole launch
 
 // Register-resident share of the walker's structure factor: slot m of a
-// thread is k = kslot(m).
+// thread is k = kslot(k0, m) of the workgroup's k-vectors [k0, k1).
 template <int NS>
 struct KSlot {
   double2 S[NS + 1];  // species rows, then the total
@@ -71,27 +76,31 @@ struct KSlot {
 
 // k of register slot m (reverse thread order: the low threads hold the extra
 // atoms of the pair loop, the high threads the extra k-vectors)
-__device__ inline int kslot(int m) { return (int)(blockDim.x - 1 - threadIdx.x) + m * (int)blockDim.x; }
+__device__ inline int kslot(int k0, int m) {
+  return k0 + (int)(blockDim.x - 1 - threadIdx.x) + m * (int)blockDim.x;
+}
 
 template <int NS>
-__device__ __forceinline__ void load_slots(const DevParams &P, const DevWalkers &W, int w, KSlot<NS> *ks) {
+__device__ __forceinline__ void load_slots(const DevParams &P, const DevWalkers &W, int w, int k0, int k1,
+                                           KSlot<NS> *ks) {
   const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
-    int k = kslot(m);
+    int k = kslot(k0, m);
 #pragma unroll
-    for (int i = 0; i <= NS; ++i) ks[m].S[i] = k < P.nk ? skw[(size_t)i * P.nk + k] : make_double2(0.0, 0.0);
+    for (int i = 0; i <= NS; ++i) ks[m].S[i] = k < k1 ? skw[(size_t)i * P.nk + k] : make_double2(0.0, 0.0);
     ks[m].sn = ks[m].so = make_double2(0.0, 0.0);
   }
 }
 
 template <int NS>
-__device__ __forceinline__ void store_slots(const DevParams &P, const DevWalkers &W, int w, const KSlot<NS> *ks) {
+__device__ __forceinline__ void store_slots(const DevParams &P, const DevWalkers &W, int w, int k0, int k1,
+                                            const KSlot<NS> *ks) {
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
-    int k = kslot(m);
-    if (k < P.nk) {
+    int k = kslot(k0, m);
+    if (k < k1) {
 #pragma unroll
       for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = ks[m].S[i];
     }
@@ -144,21 +153,36 @@ __device__ inline void quat_from(double r1, double r2, double r3, double r4, dou
   q[3] = st * (b3 / norm);
 }
 
-// Guest coordinates of the walker: global arrays, or their LDS copy (kept in
-// step with the global arrays by apply) when LF is set.
+// Guest coordinates of the walker: read from the global arrays, or from
+// their LDS copy when LF is set; gx, gy, gz are the global arrays this
+// workgroup keeps current.  Team member 0 (and a walker without a team)
+// owns the walker's arrays; a member m > 0 reads its LDS copy (LF) and
+// writes no global array, or (without LF) keeps a private global copy.
 template <bool LF>
 struct GuestXYZ {
   const double *x, *y, *z;  // slot 0 of guest 1
-  __device__ GuestXYZ(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
+  double *gx, *gy, *gz;
+  __device__ GuestXYZ(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds, int member = 0) {
+    const size_t n = P.gatoms_total;
+    if (member == 0) {
+      gx = W.gx + (size_t)w * n;
+      gy = W.gy + (size_t)w * n;
+      gz = W.gz + (size_t)w * n;
+    } else if (!LF) {
+      gx = W.gxt + ((size_t)w * (W.team_alloc - 1) + member - 1) * 3 * n;
+      gy = gx + n;
+      gz = gy + n;
+    } else {
+      gx = gy = gz = nullptr;
+    }
     if constexpr (LF) {
       x = (const double *)(lds + P.off_g);
-      y = x + P.gatoms_total;
-      z = y + P.gatoms_total;
+      y = x + n;
+      z = y + n;
     } else {
-      size_t b = (size_t)w * P.gatoms_total;
-      x = W.gx + b;
-      y = W.gy + b;
-      z = W.gz + b;
+      x = gx;
+      y = gy;
+      z = gz;
     }
   }
 };
@@ -913,6 +937,49 @@ __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
 // the LJ columns) are read from LDS.  NSITE > 0: guests have at most NSITE
 // sites (padding sites masked) and the site loops are straight-line code.
 // seam: also write s_new / s_old of the register slots to the scratch rows.
+// The team's sums of one step (lanes < NRED of wave 0, v = this member's
+// sum of row `lane`): each member publishes its row sums in its record of
+// the exchange's parity, the release store of the exchange number last;
+// lane q < team waits for member q's number (bounded: XSPIN polls, then the
+// walker is flagged and the members leave their loops); the rows are then
+// added in member order, so that every member holds the same bits.
+template <int NS>
+__device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem<NS> &sh, double v, int lane, int team,
+                                                int member) {
+  using A = Acc<NS>;
+  const long long x = sh.xseq + 1;
+  double *recs = W.xch + ((size_t)w * 2 + (size_t)(x & 1)) * KTEAM * XREC;
+  double *mine = recs + (size_t)member * XREC;
+  if (lane < A::NRED) mine[lane] = v;
+  if (lane == 0) __hip_atomic_store((long long *)(mine + XREC - 1), x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  int fail = 0;
+  if (lane < team && lane != member) {
+    const long long *f = (const long long *)(recs + (size_t)lane * XREC + XREC - 1);
+    long long spin = 0;
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != x) {
+      if (++spin > XSPIN) {
+        fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double t = 0.0;
+  if (lane < A::NRED)
+    for (int q = 0; q < team; ++q) t += q == member ? v : __hip_atomic_load(recs + (size_t)q * XREC + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  fail = __any(fail);
+  if (lane == 0) {
+    sh.xseq = x;
+    if (fail) {
+      sh.team_fail = 1;
+      sh.ws.overflow |= FMC_OVF_TEAM;
+      *W.xerr = 1;
+    }
+  }
+  return t;
+}
+
 template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                          KSlot<NS> *ks, bool seam) {
@@ -1008,7 +1075,11 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   }
   const int ntot = uni(sh.ntot);
   const int excl_lo = uni(sh.excl_lo), excl_hi = uni(sh.excl_hi);
-  const GuestXYZ<LF> G(P, W, w, lds);
+  const int team = uni(sh.team), member = uni(sh.member);
+  const int k0 = uni(sh.k0), k1 = uni(sh.k1);
+  // a team member takes the atoms [a0, a1) of the pair loop
+  const int a0 = (int)((long long)ntot * member / team), a1 = (int)((long long)ntot * (member + 1) / team);
+  const GuestXYZ<LF> G(P, W, w, lds, member);
   const double *fx, *fy, *fz, *fq;
   const int *fi;
   if constexpr (LF) {
@@ -1025,7 +1096,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     fi = P.fwinfo;
   }
   // ---- pair terms
-  for (int j = tid; j < ntot; j += nt) {
+  for (int j = a0 + tid; j < a1; j += nt) {
     double xj, yj, zj, qj;
     int tj, spj;
     bool skip = false, frz = false;
@@ -1180,8 +1251,8 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   };
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
-    const int k = kslot(m);
-    if (k < P.nk) {
+    const int k = kslot(k0, m);
+    if (k < k1) {
       if (k < P.nk64) recip_k(k, ks[m].S, ks[m].sn, ks[m].so, 0.0);
       else recip_k(k, ks[m].S, ks[m].sn, ks[m].so, (T)0);
       if (seam) {
@@ -1190,9 +1261,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       }
     }
   }
-  if (P.nk > KPT * nt) {
+  if (k1 - k0 > KPT * nt) {
     const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
-    for (int k = KPT * nt + tid; k < P.nk; k += nt) {
+    for (int k = k0 + KPT * nt + tid; k < k1; k += nt) {
       double2 S[NS + 1], sn = make_double2(0.0, 0.0), so = make_double2(0.0, 0.0);
 #pragma unroll
       for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
@@ -1217,17 +1288,22 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   // wave 0 finishes the sums; the decision that reads them runs on wave 0
   // too, so no further barrier is needed before it (callers that read
   // sh.tot or the overlap flags from other waves synchronise first)
-  if (wave == 0 && lane < A::NRED) {
-    // pairwise over the waves (missing waves count as zero)
+  if (wave == 0) {
     double v[KWAVES];
+    if (lane < A::NRED) {
+      // pairwise over the waves (missing waves count as zero)
 #pragma unroll
-    for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
+      for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
 #pragma unroll
-    for (int st = KWAVES / 2; st >= 1; st >>= 1)
+      for (int st = KWAVES / 2; st >= 1; st >>= 1)
 #pragma unroll
-      for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
-    sh.tot[lane] = v[0];
-    if (lane >= A::NACC) {
+        for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
+    } else {
+      v[0] = 0.0;
+    }
+    if (team > 1) v[0] = team_exchange<NS>(W, w, sh, v[0], lane, team, member);
+    if (lane < A::NRED) sh.tot[lane] = v[0];
+    if (lane >= A::NACC && lane < A::NRED) {
       const long long n = (long long)v[0];
       int &ovl = lane == A::NACC ? sh.ovl_new : sh.ovl_old;
       int &srf = lane == A::NACC ? sh.srf_new : sh.srf_old;
@@ -1484,16 +1560,18 @@ __device__ inline void sk_update(int kind, int mol, double2 &sm, double2 &st, do
 // accept_move, mc_moves.f:485-621 (plus the S(k) bookkeeping that
 // mc_displace / displace_guest do around it, gcmc.f:1863-1930,2862-2871).
 // REG: the register slots ks hold S(k) and this trial's s_new / s_old for
-// their k-vectors; every other k goes through global memory.  LF: the LDS
-// copy of the guest coordinates is updated with the global arrays.
+// their k-vectors; every other k of the workgroup's range goes through
+// global memory.  LF: the LDS copy of the guest coordinates is updated with
+// the global arrays the workgroup keeps (GuestXYZ).
 template <int NS, bool LF, bool REG>
 __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                       KSlot<NS> *ks) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int g = uni(sh.g), mol = uni(sh.mol), natms = uni(sh.natms), kind = uni(sh.kind);
+  const int k0 = uni(sh.k0), k1 = uni(sh.k1);
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
   const double2 *scr = W.sscr + (size_t)w * 2 * P.nk;
-  int kfirst = 0;
+  int kfirst = k0;
   if constexpr (REG) {
 #pragma unroll
     for (int m = 0; m < KPT; ++m) {
@@ -1502,15 +1580,18 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
       for (int i = 0; i < NS; ++i)
         if (i == mol - 1) sk_update<NS>(kind, mol, ks[m].S[i], ks[m].S[NS], ks[m].sn, ks[m].so);
     }
-    kfirst = KPT * nt;
+    kfirst = k0 + KPT * nt;
   }
-  for (int k = kfirst + tid; k < P.nk; k += nt) {
+  for (int k = kfirst + tid; k < k1; k += nt) {
     double2 sm = skw[(size_t)(mol - 1) * P.nk + k], st = skw[(size_t)NS * P.nk + k];
     sk_update<NS>(kind, mol, sm, st, scr[k], scr[P.nk + k]);
     skw[(size_t)(mol - 1) * P.nk + k] = sm;
     skw[(size_t)NS * P.nk + k] = st;
   }
-  size_t base = (size_t)w * P.gatoms_total + P.gbase[g];
+  const GuestXYZ<LF> G(P, W, w, lds, uni(sh.member));
+  double *gxw = G.gx ? G.gx + P.gbase[g] : nullptr;
+  double *gyw = G.gy ? G.gy + P.gbase[g] : nullptr;
+  double *gzw = G.gz ? G.gz + P.gbase[g] : nullptr;
   double *lx = nullptr, *ly = nullptr, *lz = nullptr;
   if constexpr (LF) {
     lx = (double *)(lds + P.off_g) + P.gbase[g];
@@ -1522,9 +1603,11 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
     if (tid < natms) {
       int a = (kind == FMC_MOVE_INSERT ? nm : choice - 1) * natms + tid;
       double x = sh.xn[tid][0], y = sh.xn[tid][1], z = sh.xn[tid][2];
-      W.gx[base + a] = x;
-      W.gy[base + a] = y;
-      W.gz[base + a] = z;
+      if (gxw) {
+        gxw[a] = x;
+        gyw[a] = y;
+        gzw[a] = z;
+      }
       if constexpr (LF) {
         lx[a] = x;
         ly[a] = y;
@@ -1543,16 +1626,18 @@ __device__ __forceinline__ void apply(const DevParams &P, const DevWalkers &W, i
           y = ly[j + natms];
           z = lz[j + natms];
         } else {
-          x = W.gx[base + j + natms];
-          y = W.gy[base + j + natms];
-          z = W.gz[base + j + natms];
+          x = gxw[j + natms];
+          y = gyw[j + natms];
+          z = gzw[j + natms];
         }
       }
       __syncthreads();
       if (j < hi) {
-        W.gx[base + j] = x;
-        W.gy[base + j] = y;
-        W.gz[base + j] = z;
+        if (gxw) {
+          gxw[j] = x;
+          gyw[j] = y;
+          gzw[j] = z;
+        }
         if constexpr (LF) {
           lx[j] = x;
           ly[j] = y;
@@ -1604,15 +1689,16 @@ __device__ __forceinline__ void swap_backup(const DevParams &P, const DevWalkers
   // move the register slots to scratch memory)
   double2 *bak = W.sbak + (size_t)w * (NS + 1) * P.nk;
   const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+  const int k0 = uni(sh.k0), k1 = uni(sh.k1);
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
-    const int k = kslot(m);
-    if (k < P.nk) {
+    const int k = kslot(k0, m);
+    if (k < k1) {
 #pragma unroll
       for (int i = 0; i <= NS; ++i) bak[(size_t)i * P.nk + k] = ks[m].S[i];
     }
   }
-  for (int k = KPT * nt + tid; k < P.nk; k += nt)
+  for (int k = k0 + KPT * nt + tid; k < k1; k += nt)
     for (int i = 0; i <= NS; ++i) bak[(size_t)i * P.nk + k] = skw[(size_t)i * P.nk + k];
   if (tid == 0) {
     const WState &ws = sh.ws;
@@ -1640,18 +1726,22 @@ __device__ __forceinline__ void swap_restore(const DevParams &P, const DevWalker
   const int ig = uni(sh.swap_ig), imol = P.locguest[ig], natms = P.natoms[imol];
   const double2 *bak = W.sbak + (size_t)w * (NS + 1) * P.nk;
   double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
+  const int k0 = uni(sh.k0), k1 = uni(sh.k1);
 #pragma unroll
   for (int m = 0; m < KPT; ++m) {
-    const int k = kslot(m);
-    if (k < P.nk) {
+    const int k = kslot(k0, m);
+    if (k < k1) {
 #pragma unroll
       for (int i = 0; i <= NS; ++i) ks[m].S[i] = bak[(size_t)i * P.nk + k];
     }
   }
-  for (int k = KPT * nt + tid; k < P.nk; k += nt)
+  for (int k = k0 + KPT * nt + tid; k < k1; k += nt)
     for (int i = 0; i <= NS; ++i) skw[(size_t)i * P.nk + k] = bak[(size_t)i * P.nk + k];
   // molecule swap_ichoice back in place: the ones after it move up one slot
-  const size_t base = (size_t)w * P.gatoms_total + P.gbase[ig];
+  const GuestXYZ<LF> G(P, W, w, lds, uni(sh.member));
+  double *gxw = G.gx ? G.gx + P.gbase[ig] : nullptr;
+  double *gyw = G.gy ? G.gy + P.gbase[ig] : nullptr;
+  double *gzw = G.gz ? G.gz + P.gbase[ig] : nullptr;
   double *lx = nullptr, *ly = nullptr, *lz = nullptr;
   if constexpr (LF) {
     lx = (double *)(lds + P.off_g) + P.gbase[ig];
@@ -1669,16 +1759,18 @@ __device__ __forceinline__ void swap_restore(const DevParams &P, const DevWalker
         y = ly[j];
         z = lz[j];
       } else {
-        x = W.gx[base + j];
-        y = W.gy[base + j];
-        z = W.gz[base + j];
+        x = gxw[j];
+        y = gyw[j];
+        z = gzw[j];
       }
     }
     __syncthreads();
     if (j >= lo) {
-      W.gx[base + j + natms] = x;
-      W.gy[base + j + natms] = y;
-      W.gz[base + j + natms] = z;
+      if (gxw) {
+        gxw[j + natms] = x;
+        gyw[j + natms] = y;
+        gzw[j + natms] = z;
+      }
       if constexpr (LF) {
         lx[j + natms] = x;
         ly[j + natms] = y;
@@ -1689,9 +1781,11 @@ __device__ __forceinline__ void swap_restore(const DevParams &P, const DevWalker
   }
   if (tid < natms) {
     const double x = sh.xo[tid][0], y = sh.xo[tid][1], z = sh.xo[tid][2];
-    W.gx[base + lo + tid] = x;
-    W.gy[base + lo + tid] = y;
-    W.gz[base + lo + tid] = z;
+    if (gxw) {
+      gxw[lo + tid] = x;
+      gyw[lo + tid] = y;
+      gzw[lo + tid] = z;
+    }
     if constexpr (LF) {
       lx[lo + tid] = x;
       ly[lo + tid] = y;
@@ -1735,7 +1829,8 @@ template <int NS, bool LF>
 __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                              KSlot<NS> *ks) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const GuestXYZ<LF> G(P, W, w, lds);
+  const GuestXYZ<LF> G(P, W, w, lds, uni(sh.member));
+  const int k0 = uni(sh.k0), k1 = uni(sh.k1);
   const double *fx, *fy, *fz, *fq;
   const int *fi;
   const int *kl;
@@ -1775,14 +1870,14 @@ __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalker
       natot += sh.ws.nmols[P.locguest[gg]] * P.natoms[P.locguest[gg]];
       gcut[gg] = natot;
     }
-    const int nglob = P.nk > KPT * nt ? P.nk - KPT * nt : 0;
+    const int nglob = k1 - k0 > KPT * nt ? k1 - k0 - KPT * nt : 0;
     const int ngroups = 1 + (nglob + KPT * nt - 1) / (KPT * nt);
     for (int grp = 0; grp < ngroups; ++grp) {
       double2 acc[KPT][NS + 1];
       int kk[KPT];
 #pragma unroll
       for (int m = 0; m < KPT; ++m) {
-        kk[m] = grp == 0 ? kslot(m) : KPT * nt + (grp - 1) * KPT * nt + m * nt + tid;
+        kk[m] = grp == 0 ? kslot(k0, m) : k0 + KPT * nt + (grp - 1) * KPT * nt + m * nt + tid;
 #pragma unroll
         for (int i = 0; i <= NS; ++i) acc[m][i] = make_double2(0.0, 0.0);
       }
@@ -1830,7 +1925,7 @@ __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalker
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < KPT; ++m) {
-          if (kk[m] >= P.nk) continue;
+          if (kk[m] >= k1) continue;
           const int lmn = kl[kk[m]];
           const int ll = lmn & 255, mm = ((lmn >> 8) & 255) - 128, nn = ((lmn >> 16) & 255) - 128;
           for (int ia = 0; ia < cnt; ++ia) {
@@ -1851,7 +1946,7 @@ __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalker
 #pragma unroll
       for (int m = 0; m < KPT; ++m) {
         const int k = kk[m];
-        if (k >= P.nk) continue;
+        if (k >= k1) continue;
         if (grp == 0) {
           dmax = fmax(dmax, fmax(fabs(ks[m].S[NS].x - acc[m][NS].x), fabs(ks[m].S[NS].y - acc[m][NS].y)));
 #pragma unroll
@@ -1917,10 +2012,10 @@ __device__ __forceinline__ void resum_walker(const DevParams &P, const DevWalker
     };
 #pragma unroll
     for (int m = 0; m < KPT; ++m) {
-      int k = kslot(m);
-      if (k < P.nk) dmax = fmax(dmax, one_k(k, ks[m].S));
+      int k = kslot(k0, m);
+      if (k < k1) dmax = fmax(dmax, one_k(k, ks[m].S));
     }
-    for (int k = KPT * nt + tid; k < P.nk; k += nt) {
+    for (int k = k0 + KPT * nt + tid; k < k1; k += nt) {
       double2 S[NS + 1];
 #pragma unroll
       for (int i = 0; i <= NS; ++i) S[i] = skw[(size_t)i * P.nk + k];
@@ -1961,8 +2056,11 @@ __device__ inline void gstart_from(const DevParams &P, const int *nmols, Smem<NS
   sh.ntot = n;
 }
 
+// The walker's state into LDS; team / member: the workgroup's place in the
+// walker's team (1 / 0 for a walker without one), which sets its k-vectors.
 template <int NS>
-__device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
+__device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, int team = 1,
+                                int member = 0) {
   const int *src = (const int *)(W.ws + w);
   int *dst = (int *)&sh.ws;
   for (int i = threadIdx.x; i < (int)(sizeof(WState) / 4); i += blockDim.x) dst[i] = src[i];
@@ -1990,6 +2088,12 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     gstart_from(P, W.ws[w].nmols, sh);
     sh.mkind = 0;
     sh.swap_phase = 0;
+    sh.team = team;
+    sh.member = member;
+    sh.k0 = (int)((long long)P.nk * member / team);
+    sh.k1 = (int)((long long)P.nk * (member + 1) / team);
+    sh.xseq = team > 1 ? W.xcnt[w] : 0;
+    sh.team_fail = 0;
   }
   if (t <= KMAXS) {
     // guest kinds beyond ntpguest: species 1 (never read), no sites
@@ -2005,6 +2109,8 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
 template <int NS>
 __device__ inline void stage_out(const DevWalkers &W, int w, Smem<NS> &sh) {
   __syncthreads();
+  if (sh.member != 0) return;  // member 0 holds the walker's state
+  if (threadIdx.x == 0 && sh.team > 1) W.xcnt[w] = sh.xseq;
   int *dst = (int *)(W.ws + w);
   const int *src = (const int *)&sh.ws;
   for (int i = threadIdx.x; i < (int)(sizeof(WState) / 4); i += blockDim.x) dst[i] = src[i];
@@ -2254,20 +2360,46 @@ __device__ inline void stats_step(const DevParams &P, const DevWalkers &W, int w
 }
 
 // ------------------------------------------------------------- kernels --
-// Device-resident chains: one workgroup per walker, nsteps GCMC steps.  LF /
-// NSITE select the all-in-LDS variant and the site count of its straight-line
-// pair code (see evaluate).
+// Device-resident chains, nsteps GCMC steps per walker.  LF / NSITE select
+// the all-in-LDS variant and the site count of its straight-line pair code
+// (see evaluate).  team == 1: one workgroup per walker.  team > 1: `team`
+// workgroups per walker, each with the whole walker state in LDS, running
+// the same proposal and decision; each evaluates its share of the pair atoms
+// and of the k-vectors (whose S(k) it alone keeps), and the shares are
+// added through team_exchange once per step.  The grid is then
+// 8 * ceil(nw / 8) * team workgroups, which the host keeps within what the
+// device holds at once (a member waits on the others).
 template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, DevWalkers W, int nsteps,
-                                             int count_stats) {
+                                             int count_stats, int team, int nw) {
   extern __shared__ __align__(16) unsigned char lds[];
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
-  const int w = blockIdx.x;
+  int w = blockIdx.x, member = 0;
+  if (team > 1) {
+    // workgroups are dealt to the 8 XCDs in turn, so blocks b, b + 8,
+    // b + 16, ... share an XCD and its L2: a walker's members are taken
+    // from one such sequence
+    const int b = blockIdx.x, j = b >> 3;
+    w = (b & 7) + 8 * (j / team);
+    member = j - (j / team) * team;
+    if (w >= nw) return;  // padding of the last XCD round
+  }
   KSlot<NS> ks[KPT];
-  load_slots<NS>(P, W, w, ks);
-  stage_in<NS>(P, W, w, sh);
+  load_slots<NS>(P, W, w, (int)((long long)P.nk * member / team), (int)((long long)P.nk * (member + 1) / team), ks);
+  stage_in<NS>(P, W, w, sh, team, member);
   stage_tables<T, LF, LJL>(P, W, w, lds);
+  if (!LF && member > 0) {
+    // a private copy of the guest coordinates (the walker's own arrays are
+    // member 0's; it writes them only after the first exchange)
+    const size_t n = P.gatoms_total, src = (size_t)w * n;
+    double *dst = W.gxt + ((size_t)w * (W.team_alloc - 1) + member - 1) * 3 * n;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+      dst[i] = W.gx[src + i];
+      dst[n + i] = W.gy[src + i];
+      dst[2 * n + i] = W.gz[src + i];
+    }
+  }
   if (P.wl_nbins > 0) {
     const WLView v = wl_view(P, lds);
     for (int b = threadIdx.x; b < P.wl_nbins; b += blockDim.x) {
@@ -2275,13 +2407,14 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       v.hist[b] = W.wl_hist[(size_t)w * P.wl_nbins + b];
     }
   }
-  const GuestXYZ<LF> G(P, W, w, lds);
+  const GuestXYZ<LF> G(P, W, w, lds, member);
   __syncthreads();
   // A swap takes two passes of the loop: its deletion (applied provisionally)
   // and the insertion of the other kind, accepted or undone as a whole; only
   // the second pass completes a step.
   for (int step = 0; step < nsteps;) {
     if (sh.ws.wl_done) break;  // converged Wang-Landau walker (uniform: written before the last barrier)
+    if (sh.team_fail) break;   // a team wait gave up (the walker is flagged)
     if (threadIdx.x < 64) {
       if (sh.swap_phase == 1) {
         swap_insert_setup<NS>(P, sh);
@@ -2296,7 +2429,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false);
     if (threadIdx.x == 0) {
-      decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w);
+      decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w && member == 0);
       if (sh.swap_phase != 1) {
         WState &ws = sh.ws;
         const int mk = sh.mkind;
@@ -2327,6 +2460,11 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       swap_restore<NS, LF>(P, W, w, sh, lds, ks);
     }
     if (phase == 1) continue;
+    if (member > 0) {
+      // the statistics, records and plots are member 0's
+      ++step;
+      continue;
+    }
     if (threadIdx.x == 0 && W.trace && w == W.trace_w) record<NS>(P, W, sh, step);
     // the step's statistics on wave 1, idle while wave 0 proposes the next
     // step (the fields written here are read nowhere before the next barrier)
@@ -2355,8 +2493,8 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     }
     ++step;
   }
-  store_slots<NS>(P, W, w, ks);
-  if (P.wl_nbins > 0) {
+  store_slots<NS>(P, W, w, uni(sh.k0), uni(sh.k1), ks);
+  if (P.wl_nbins > 0 && member == 0) {
     __syncthreads();
     const WLView v = wl_view(P, lds);
     for (int b = threadIdx.x; b < P.wl_nbins; b += blockDim.x) {
@@ -2376,7 +2514,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
   const DevParams &P = *Pp;
   KSlot<NS> ks[KPT];
-  load_slots<NS>(P, W, w, ks);
+  load_slots<NS>(P, W, w, 0, P.nk, ks);
   stage_in<NS>(P, W, w, sh);
   stage_tables<T, false, false>(P, W, w, lds);
   if (threadIdx.x == 0) {

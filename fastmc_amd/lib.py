@@ -29,6 +29,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_trial_insert", "fmc_trial_delete", "fmc_trial_displace", "fmc_accept",
            "fmc_reject", "fmc_get_guests", "fmc_get_sk", "fmc_get_stats", "fmc_reset_stats",
            "fmc_run", "fmc_run_graph", "fmc_last_timing", "fmc_set_block", "fmc_get_block",
+           "fmc_set_team", "fmc_get_team",
            "fmc_set_resum", "fmc_wl_setup", "fmc_wl_get", "fmc_wl_window", "fmc_wl_stitch",
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
            "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob",
@@ -203,6 +204,15 @@ class Handle:
     def get_block(self) -> int:
         t = C.c_int()
         _chk("fmc_get_block", self.L.fmc_get_block(self.h, C.byref(t)))
+        return t.value
+
+    def set_team(self, team: int):
+        """Workgroups per walker (1..16); 0 lets set_block(0) try every team size."""
+        _chk("fmc_set_team", self.L.fmc_set_team(self.h, C.c_int(team)))
+
+    def get_team(self) -> int:
+        t = C.c_int()
+        _chk("fmc_get_team", self.L.fmc_get_team(self.h, C.byref(t)))
         return t.value
 
     # ---- state readback

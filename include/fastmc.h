@@ -40,6 +40,8 @@ extern "C" {
 #define FMC_ERR_HIP 9003             /* HIP runtime failure */
 #define FMC_ERR_IMCON 9004           /* boundary condition other than imcon 1, 2, 3 */
 #define FMC_ERR_SCRATCH 9005         /* request larger than a fixed scratch buffer */
+#define FMC_ERR_TEAM 9006            /* a team member's sums did not arrive (a bounded wait expired) */
+#define FMC_OVF_TEAM 8               /* fmc_walker_stats.overflow bit of FMC_ERR_TEAM */
 
 /* precision modes */
 #define FMC_FP64 0   /* every term in FP64 (parity mode) */
@@ -130,7 +132,8 @@ typedef struct fmc_walker_stats {
   int64_t steps, accepted[9], attempted[9];  /* index by move kind FMC_MOVE_* (1..8) */
   int nmols[FMC_MAX_SPECIES];
   int overflow;                              /* bit 0: guest slots full; bit 1: window buffer full;
-                                                bit 2: numguests records full */
+                                                bit 2: numguests records full; bit 3
+                                                (FMC_OVF_TEAM): a team wait gave up */
   double energy[FMC_MAX_SPECIES];            /* cumulative per-species energy (FP64) */
   double sum_n[FMC_MAX_SPECIES];             /* sum of N over counted steps */
   double sum_n2[FMC_MAX_SPECIES];
@@ -233,9 +236,24 @@ int fmc_set_trace_walker(fmc_handle *h, int w);
 int fmc_last_timing(fmc_handle *h, double *ms, int *launches);
 
 /* Launch-shape control: threads per walker block (64..512, multiple of 64);
- * 0 = autotune on a snapshot of the walkers. */
+ * 0 = autotune on a snapshot of the walkers, over the block size and the
+ * team size (below, up to the limit fmc_set_team last set). */
 int fmc_set_block(fmc_handle *h, int threads);
 int fmc_get_block(const fmc_handle *h, int *threads);
+
+/* Workgroups per walker in fmc_run / fmc_run_graph (1..16): the step's pair
+ * terms (the atoms of the neighbour loop, ewald.f:689-726 and
+ * vdw_terms.f:418-460) and reciprocal terms (the k-vectors of ewald1_guest,
+ * ewald.f:155-315) are split over the team, whose members exchange their
+ * sums once per step; proposal and decision run in every member alike.  All
+ * team workgroups must be resident together: FMC_ERR_ARG if the launch would
+ * exceed what the device holds at once with the current block and LDS plan.
+ * A run whose exchange wait gave up returns FMC_ERR_TEAM and flags the
+ * walker (overflow bit FMC_OVF_TEAM).  Also the autotuner's upper limit;
+ * 0 restores the autotuner's full range.  No reference counterpart (the
+ * reference runs one chain per MPI rank). */
+int fmc_set_team(fmc_handle *h, int team);
+int fmc_get_team(const fmc_handle *h, int *team);
 
 /* Full FP64 re-sum of each walker's S(k) (ewald1, ewald.f:354-661) inside the
  * device chain after every `every` accepted moves of that walker; 0 = never.
