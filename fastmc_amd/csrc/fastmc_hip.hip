@@ -872,6 +872,7 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
   // this many accepted moves; the FP64 mode follows the reference (never)
   P.resum_every = precision == FMC_MIXED ? 1000 : 0;
   if (getenv("FMC_RESUM_EVERY")) P.resum_every = std::max(0, atoi(getenv("FMC_RESUM_EVERY")));
+  P.spec_prop = getenv("FMC_NO_SPEC") ? 0 : 1;  // speculative proposals in k_run (FMC_NO_SPEC: off)
   // guest capacities: one molecule per 40 A^3 of cell, denser than a liquid
   // of small molecules (liquid CO2 holds one per ~66 A^3); a walker that
   // fills its slots stops inserting and sets overflow bit 0

@@ -65,6 +65,7 @@ struct DevParams {
   int lj_lds;                     // without lds_all: the LJ columns alone in LDS (FP64 mode:
                                   // [nvcol][vstride] doubles at off_vvv64)
   int resum_every;                // full FP64 S(k) re-sum after this many accepted moves (0: never)
+  int spec_prop;                  // k_run builds the next step's proposal speculatively (1) or not (0)
   // Wang-Landau sampling in N of guest 1 (wang_landau.f); wl_nbins = 0: off
   int wl_nbins;                   // bins per walker: N = 0 .. wl_nbins-1 (maxn + 2)
   int wl_off;                     // LDS offset of the walker's ln g (double) then visits (int)

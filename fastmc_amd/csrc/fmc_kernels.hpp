@@ -19,6 +19,20 @@ struct Acc {
   static constexpr int NRED = NACC + 2;
 };
 
+// One proposal: the move and its trial sites as propose makes them, and the
+// walker-state updates that come with it (ring position, the move's counter,
+// adapted step sizes).  commit makes a record the step's live proposal; the
+// two speculative records are built during a step for the next one.
+struct Prop {
+  int ok;  // a speculative record that commit may take (not a swap)
+  int wlmode, mkind, swap_phase, kind, g, mol, natms, nmols, choice, has_new, has_old, excl_lo, excl_hi;
+  int cnt, cval;  // counter of the move (1 disp_count, 2 tran_count, 3 rota_count of guest g) and its new value
+  int chg;        // step sizes adapted: bit 0 delrdisp[g], 1 rotangle, 2 tran_delr[g], 3 rota_rotangle
+  long long used0, used1;  // ring position before and after the proposal's draws
+  double delrdisp, rotangle, tran_delr, rota_rotangle;
+  double xn[KSITE][3], xo[KSITE][3];
+};
+
 template <int NS>
 struct Smem {
   WState ws;
@@ -62,6 +76,14 @@ struct Smem {
   int team, member, k0, k1;
   long long xseq;
   int team_fail;
+  int xfast;  // team_exchange's ordering: -1 not yet known, 1 one XCD, 0 several
+  // proposals: [0], [1] built speculatively for the next step assuming this
+  // one is rejected after 0 / 1 draws of its decision, [2] the live one;
+  // spec_on: [0] and [1] were built this step; mz_after: the ring position
+  // after the live proposal's draws
+  Prop prop[3];
+  int spec_on;
+  long long mz_after;
 };
 
 constexpr int KPT = 2;  // k-vectors per thread held in registers for a whole launch
@@ -287,17 +309,21 @@ __device__ inline int choose_move(const float *f, double randmov, bool skip_ins,
   return FMC_MOVE_DISPLACE;
 }
 
-// Wave-parallel proposal: the 64 lanes of wave 0 run it with wave-uniform
+// Wave-parallel proposal: the 64 lanes of one wave run it with wave-uniform
 // control flow.  The uniforms of the step are read once from the duni ring
-// (lane i holds call used+i+1; the n-th draw is a readlane), lane i builds
-// site i of the trial, and values every lane computes alike stay uniform.
-// LDS state is written by lane 0, or by all lanes with identical values
-// (the Wang-Landau visit bookkeeping).
+// from position used0 (lane i holds call used0+i+1; the n-th draw is a
+// readlane), lane i builds site i of the trial, and values every lane
+// computes alike stay uniform.  The proposal goes to the record o (lane 0,
+// and lane i for site i); the walker state is only read, except by the
+// Wang-Landau bookkeeping (preload count, visits at the window edges: all
+// lanes with identical values), which runs for live proposals only.
+// spec: a speculative proposal for the next step (it leaves a swap's set-up
+// undone and marks the record not to be taken).
 template <int NS, bool LF>
-__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh, unsigned char *lds) {
+__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh, unsigned char *lds,
+                        const long long used0, Prop &o, const bool spec) {
   WState &ws = sh.ws;
   const int lane = threadIdx.x & 63;
-  const long long used0 = ws.mz_used;
   const int rbits = __float_as_int(ws.rq[(int)((used0 + lane) & 127)]);
   int nu = 0;
   auto draw = [&]() -> double {
@@ -346,18 +372,21 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   const int mol = P.locguest[g], natms = P.natoms[mol], nmols = uni(ws.nmols[mol]);
   if (kind == 0) {
     if (lane == 0) {
-      sh.wlmode = wlmode;
-      sh.mkind = 0;
-      sh.swap_phase = 0;
-      sh.kind = 0;
-      sh.g = g;
-      sh.mol = mol;
-      sh.natms = natms;
-      sh.nmols = nmols;
-      sh.choice = 0;
-      sh.has_new = sh.has_old = 0;
-      sh.excl_lo = sh.excl_hi = 0;
-      ws.mz_used = used0 + nu;
+      o.ok = 1;
+      o.wlmode = wlmode;
+      o.mkind = 0;
+      o.swap_phase = 0;
+      o.kind = 0;
+      o.g = g;
+      o.mol = mol;
+      o.natms = natms;
+      o.nmols = nmols;
+      o.choice = 0;
+      o.has_new = o.has_old = 0;
+      o.excl_lo = o.excl_hi = 0;
+      o.cnt = o.cval = o.chg = 0;
+      o.used0 = used0;
+      o.used1 = used0 + nu;
     }
     return;
   }
@@ -367,6 +396,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   double tran_delr = ws.tran_delr[g], rota_rotangle = ws.rota_rotangle;
   const int dcnt = uni(ws.disp_count[g]), tcnt = uni(ws.tran_count[g]), rcnt = uni(ws.rota_count[g]);
   const int dacc = uni(ws.accept_disp[g]), tacc = uni(ws.accept_tran[g]), racc = uni(ws.accept_rota[g]);
+  int chg = 0, cnt = 0, cval = 0;
   if (!wl_on && nmols >= 1) {
     bool chd = false, chr = false, cht = false, chq = false;
     if (dcnt >= 1) {
@@ -386,12 +416,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
       if ((racc / (double)rcnt) > 0.5) rota_rotangle *= 1.05, chq = true;
       else if (rota_rotangle > MINANGLE) rota_rotangle *= 0.95, chq = true;
     }
-    if (lane == 0) {
-      if (chd) ws.delrdisp[g] = delrdisp;
-      if (chr) ws.rotangle = rotangle;
-      if (cht) ws.tran_delr[g] = tran_delr;
-      if (chq) ws.rota_rotangle = rota_rotangle;
-    }
+    chg = (int)chd | ((int)chr << 1) | ((int)cht << 2) | ((int)chq << 3);
   }
   if (wlmode == 0 && nmols == 0) kind = FMC_MOVE_INSERT;
   const int mkind = kind;
@@ -424,9 +449,9 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     const double ty = Q[3] * a + Q[4] * b + Q[5] * d;
     const double tz = Q[6] * a + Q[7] * b + Q[8] * d;
     if (site) {
-      sh.xn[i][0] = tx + xc;
-      sh.xn[i][1] = ty + yc;
-      sh.xn[i][2] = tz + zc;
+      o.xn[i][0] = tx + xc;
+      o.xn[i][1] = ty + yc;
+      o.xn[i][2] = tz + zc;
     }
   } else if (kind == FMC_MOVE_DELETE) {
     // mc_delete; or mc_swap's choice of molecule (floor(duni*n + 1)) and of
@@ -435,11 +460,11 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     const int a = P.gbase[g] + (choice - 1) * natms + i;
     const double x = G.x[a], y = G.y[a], z = G.z[a];
     if (site) {
-      sh.xo[i][0] = x;
-      sh.xo[i][1] = y;
-      sh.xo[i][2] = z;
+      o.xo[i][0] = x;
+      o.xo[i][1] = y;
+      o.xo[i][2] = z;
     }
-    if (mkind == FMC_MOVE_SWAP) {
+    if (mkind == FMC_MOVE_SWAP && !spec) {
       double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
       for (int k = 0; k < natms; ++k) {
         const double m = sh.sm[mol][k];
@@ -502,11 +527,13 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
         a = (2.0 * draw() - 1.0) * h;
         b = (2.0 * draw() - 1.0) * h;
         c = (2.0 * draw() - 1.0) * h;
-        if (lane == 0) ws.tran_count[g] = tcnt + 1;
+        cnt = 2;
+        cval = tcnt + 1;
       } else if (kind == FMC_MOVE_ROTATE) {
         const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
         quat_from(r1, r2, r3, r4, rota_rotangle, q);
-        if (lane == 0) ws.rota_count[g] = rcnt + 1;
+        cnt = 3;
+        cval = rcnt + 1;
       } else {
         const double h = delrdisp * 0.5;
         const double da = draw(), db = draw(), dc = draw();
@@ -515,7 +542,8 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
         c = (2.0 * dc - 1.0) * h;
         const double r1 = draw(), r2 = draw(), r3 = draw(), r4 = draw();
         quat_from(r1, r2, r3, r4, rotangle, q);
-        if (lane == 0) ws.disp_count[g] = dcnt + 1;
+        cnt = 1;
+        cval = dcnt + 1;
       }
       choice = uni((int)floor(draw() * nmols) + 1);
       const int at = P.gbase[g] + (choice - 1) * natms + i;
@@ -555,29 +583,79 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     const double ty = Q[3] * dx + Q[4] * dy + Q[5] * dz;
     const double tz = Q[6] * dx + Q[7] * dy + Q[8] * dz;
     if (site) {
-      sh.xo[i][0] = ox;
-      sh.xo[i][1] = oy;
-      sh.xo[i][2] = oz;
-      sh.xn[i][0] = tx + wx;
-      sh.xn[i][1] = ty + wy;
-      sh.xn[i][2] = tz + wz;
+      o.xo[i][0] = ox;
+      o.xo[i][1] = oy;
+      o.xo[i][2] = oz;
+      o.xn[i][0] = tx + wx;
+      o.xn[i][1] = ty + wy;
+      o.xn[i][2] = tz + wz;
     }
   }
   if (lane == 0) {
-    sh.wlmode = wlmode;
-    sh.mkind = mkind;
-    sh.swap_phase = mkind == FMC_MOVE_SWAP ? 1 : 0;
-    sh.kind = kind;
+    o.ok = mkind != FMC_MOVE_SWAP;
+    o.wlmode = wlmode;
+    o.mkind = mkind;
+    o.swap_phase = mkind == FMC_MOVE_SWAP ? 1 : 0;
+    o.kind = kind;
+    o.g = g;
+    o.mol = mol;
+    o.natms = natms;
+    o.nmols = nmols;
+    o.choice = choice;
+    o.has_new = kind != 0 && kind != FMC_MOVE_DELETE;
+    o.has_old = kind != 0 && kind != FMC_MOVE_INSERT;
+    o.excl_lo = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : (choice - 1) * natms;
+    o.excl_hi = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : choice * natms;
+    o.cnt = cnt;
+    o.cval = cval;
+    o.chg = chg;
+    o.delrdisp = delrdisp;
+    o.rotangle = rotangle;
+    o.tran_delr = tran_delr;
+    o.rota_rotangle = rota_rotangle;
+    o.used0 = used0;
+    o.used1 = used0 + nu;
+  }
+}
+
+// Makes record o the step's live proposal (wave 0): the trial sites (all
+// slots, one per lane), the move, and the walker-state updates that come
+// with it.
+template <int NS>
+__device__ inline void commit(Smem<NS> &sh, const Prop &o) {
+  const int lane = threadIdx.x & 63;
+  static_assert(2 * KSITE * 3 <= 64, "trial sites exceed one wave");
+  if (lane < 2 * KSITE * 3) {
+    const double *src = lane < KSITE * 3 ? &o.xn[0][0] + lane : &o.xo[0][0] + (lane - KSITE * 3);
+    double *dst = lane < KSITE * 3 ? &sh.xn[0][0] + lane : &sh.xo[0][0] + (lane - KSITE * 3);
+    *dst = *src;
+  }
+  if (lane == 0) {
+    WState &ws = sh.ws;
+    const int g = o.g;
+    sh.wlmode = o.wlmode;
+    sh.mkind = o.mkind;
+    sh.swap_phase = o.swap_phase;
+    sh.kind = o.kind;
     sh.g = g;
-    sh.mol = mol;
-    sh.natms = natms;
-    sh.nmols = nmols;
-    sh.choice = choice;
-    sh.has_new = kind != 0 && kind != FMC_MOVE_DELETE;
-    sh.has_old = kind != 0 && kind != FMC_MOVE_INSERT;
-    sh.excl_lo = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : (choice - 1) * natms;
-    sh.excl_hi = kind == 0 || kind == FMC_MOVE_INSERT ? 0 : choice * natms;
-    ws.mz_used = used0 + nu;
+    sh.mol = o.mol;
+    sh.natms = o.natms;
+    sh.nmols = o.nmols;
+    sh.choice = o.choice;
+    sh.has_new = o.has_new;
+    sh.has_old = o.has_old;
+    sh.excl_lo = o.excl_lo;
+    sh.excl_hi = o.excl_hi;
+    if (o.cnt == 1) ws.disp_count[g] = o.cval;
+    else if (o.cnt == 2) ws.tran_count[g] = o.cval;
+    else if (o.cnt == 3) ws.rota_count[g] = o.cval;
+    const int chg = o.chg;
+    if (chg & 1) ws.delrdisp[g] = o.delrdisp;
+    if (chg & 2) ws.rotangle = o.rotangle;
+    if (chg & 4) ws.tran_delr[g] = o.tran_delr;
+    if (chg & 8) ws.rota_rotangle = o.rota_rotangle;
+    ws.mz_used = o.used1;
+    sh.mz_after = o.used1;
   }
 }
 
@@ -939,19 +1017,42 @@ __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
 // seam: also write s_new / s_old of the register slots to the scratch rows.
 // The team's sums of one step (lanes < NRED of wave 0, v = this member's
 // sum of row `lane`): each member publishes its row sums in its record of
-// the exchange's parity, the release store of the exchange number last;
-// lane q < team waits for member q's number (bounded: XSPIN polls, then the
-// walker is flagged and the members leave their loops); the rows are then
-// added in member order, so that every member holds the same bits.
+// the exchange's parity, the store of the exchange number last; lane q < team
+// waits for member q's number (bounded: XSPIN polls, then the walker is
+// flagged and the members leave their loops); the rows are then added in
+// member order, so that every member holds the same bits.
+//
+// Ordering.  The first exchange of a launch is a release store of the number
+// (which writes the L2 of this XCD back to memory) and an acquire fence after
+// the wait, visible across the whole device; with it every member publishes
+// the id of the XCD it runs on (hardware register XCC_ID).  When all members
+// share one XCD they share its L2, the point where their agent-scope stores
+// and loads meet, and the later exchanges of the launch only wait until this
+// wave's row stores are done (s_waitcnt vmcnt(0)) before the number is
+// stored, and read the rows with agent-scope loads after the number: the
+// same order without the L2 write-back and invalidation.  Members on
+// different XCDs keep the release / acquire form.
+__device__ __forceinline__ int xcd_id() { return (int)__builtin_amdgcn_s_getreg(20 | (3 << 11)); }  // HW_REG_XCC_ID[3:0]
+
 template <int NS>
 __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem<NS> &sh, double v, int lane, int team,
                                                 int member) {
   using A = Acc<NS>;
+  static_assert(A::NRED < XREC - 1, "exchange record too short");
   const long long x = sh.xseq + 1;
+  const int fast = uni(sh.xfast);  // 1: all members on this XCD; 0: not; -1: first exchange (find out)
   double *recs = W.xch + ((size_t)w * 2 + (size_t)(x & 1)) * KTEAM * XREC;
   double *mine = recs + (size_t)member * XREC;
-  if (lane < A::NRED) mine[lane] = v;
-  if (lane == 0) __hip_atomic_store((long long *)(mine + XREC - 1), x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane < A::NRED) __hip_atomic_store(mine + lane, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == A::NRED)
+    __hip_atomic_store(mine + A::NRED, (double)xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  long long *myflag = (long long *)(mine + XREC - 1);
+  if (fast == 1) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (lane == 0) __hip_atomic_store(myflag, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    if (lane == 0) __hip_atomic_store(myflag, x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
   int fail = 0;
   if (lane < team && lane != member) {
     const long long *f = (const long long *)(recs + (size_t)lane * XREC + XREC - 1);
@@ -964,13 +1065,20 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
       __builtin_amdgcn_s_sleep(1);
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (fast == 1) asm volatile("" ::: "memory");
+  else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   double t = 0.0;
   if (lane < A::NRED)
     for (int q = 0; q < team; ++q) t += q == member ? v : __hip_atomic_load(recs + (size_t)q * XREC + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   fail = __any(fail);
+  int same = 1;
+  if (fast < 0 && lane < team)
+    same = __hip_atomic_load(recs + (size_t)lane * XREC + A::NRED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           (double)xcd_id();
+  same = __all(same);
   if (lane == 0) {
     sh.xseq = x;
+    if (fast < 0) sh.xfast = same && !fail;
     if (fail) {
       sh.team_fail = 1;
       sh.ws.overflow |= FMC_OVF_TEAM;
@@ -2094,7 +2202,11 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.k1 = (int)((long long)P.nk * (member + 1) / team);
     sh.xseq = team > 1 ? W.xcnt[w] : 0;
     sh.team_fail = 0;
+    sh.xfast = -1;
+    sh.spec_on = 0;
+    sh.accepted = 0;
   }
+  for (int i = t; i < (int)(sizeof(sh.prop) / 4); i += blockDim.x) ((int *)sh.prop)[i] = 0;
   if (t <= KMAXS) {
     // guest kinds beyond ntpguest: species 1 (never read), no sites
     const bool real = t >= 1 && t <= P.ntpguest;
@@ -2419,15 +2531,43 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       if (sh.swap_phase == 1) {
         swap_insert_setup<NS>(P, sh);
       } else {
+        // after a rejected step, the proposal built for it speculatively
+        // from the ring position its decision left (0 or 1 draws on)
+        int take = -1;
+        if (uni(sh.spec_on) && !uni(sh.accepted)) {
+          const long long u = sh.ws.mz_used;
+          if (sh.prop[0].ok && sh.prop[0].used0 == u) take = 0;
+          else if (sh.prop[1].ok && sh.prop[1].used0 == u) take = 1;
+        }
+        take = uni(take);
+        if (take >= 0) commit<NS>(sh, sh.prop[take]);
+        // at least 33 draws ahead of the step: enough for this step and the
+        // next one's speculative proposals (at most 12 per step)
         if (sh.ws.mz_made - sh.ws.mz_used < 33) {
           mzt_refill(sh.ws, threadIdx.x);
           if (threadIdx.x == 0) mzt_advance(sh.ws);
         }
-        propose<NS, LF>(P, G, sh, lds);
+        if (take < 0) {
+          propose<NS, LF>(P, G, sh, lds, sh.ws.mz_used, sh.prop[2], false);
+          commit<NS>(sh, sh.prop[2]);
+        }
       }
     }
     __syncthreads();
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false);
+    {
+      // the next step's proposal, assuming this step is rejected, on waves 1
+      // and 2 (for 0 and 1 draws of the decision) while wave 0 finishes the
+      // sums, waits for the team and decides; a swap or a Wang-Landau walker
+      // (whose proposals update its visit histogram) proposes live.  Wave 1
+      // records whether the proposals were made (read after the barrier).
+      const int wv = threadIdx.x >> 6;
+      if (wv == 1 || wv == 2) {
+        const bool spec = P.spec_prop && uni(sh.swap_phase) == 0 && !uni(sh.ws.wl_on) && blockDim.x >= 192;
+        if (spec) propose<NS, LF>(P, G, sh, lds, sh.mz_after + (wv - 1), sh.prop[wv - 1], true);
+        if (threadIdx.x == 64) sh.spec_on = spec;
+      }
+    }
     if (threadIdx.x == 0) {
       decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w && member == 0);
       if (sh.swap_phase != 1) {
