@@ -70,6 +70,8 @@ def parse():
                     help="plain launches instead of graph replays (for counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates the HBM byte counters) and exit")
+    ap.add_argument("--only", choices=("config1", "config3", "config4"),
+                    help="run only this measurement and print its JSON (for profiling it alone)")
     return ap.parse_args()
 
 
@@ -403,6 +405,10 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.only:
+        fn = {"config1": config1_supercell, "config3": config3_gradient, "config4": config4_wang_landau}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
