@@ -178,3 +178,66 @@ def test_gpu_config4_wang_landau_isotherm_matches_gcmc():
         if abs(z) > 3.0:
             bad.append((p, round(z, 2)))
     assert not bad, bad
+
+
+def test_gpu_config4_mixed_du_error():
+    """Mixed precision against FP64 at the loadings of config 4's windows: the
+    128-walker launch (FP64) preloads every window; for the first walker of
+    windows 0, 3 and 7 a traced continuation of its Wang-Landau chain gives
+    1000 trials (both guest kinds; insertions, deletions, displacements),
+    replayed through the one-trial seam on a mixed and an FP64 handle that
+    hold the walker's molecules (the FP64 seam reproduces the traced dU):
+    |dU_mixed - dU_fp64| <= 1e-6 * max(|dU_fp64|, kB T); maximum and 99th
+    percentile printed per window."""
+    from _helpers import KB_KCAL, du_tol
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    sysm = config4_system()
+    T = sysm.temperature
+    nw, nwin, maxn, n = 128, 8, 400, 1000
+    h = Handle(sysm, precision=FMC_FP64, seeds=seeds(nw, 11), n=nw)
+    hd = Handle(sysm, precision=FMC_FP64)
+    hm = Handle(sysm, precision=FMC_MIXED)
+    try:
+        h.wl_setup(maxn=maxn, nwindows=nwin, visittol=10000, flatcoeff=0.8, prectol=1e-6)
+        h.run(4000)
+        for k in (0, 3, 7):
+            w = k * nw // nwin
+            for hh in (hd, hm):
+                for g in (1, 2):
+                    hh.load_guests(g, h.guests(g, w=w))
+            n0 = [len(h.guests(g, w=w)) // 3 for g in (1, 2)]
+            h.set_trace_walker(w)
+            tr = h.run(n, trace=True)
+            errs, kinds = [], set()
+            for r in tr:
+                g, mv, c = int(r["iguest"]), int(r["move"]), int(r["choice"])
+                if mv == 0:
+                    continue  # an empty Wang-Landau step
+                assert mv <= 6, mv  # no swap or switch in this move set
+                xyz = r["xyz"][:9].reshape(3, 3)
+                if mv == 1:
+                    a, b = hd.trial_insert(g, xyz), hm.trial_insert(g, xyz)
+                elif mv == 2:
+                    a, b = hd.trial_delete(g, c), hm.trial_delete(g, c)
+                else:
+                    a, b = hd.trial_displace(g, c, xyz), hm.trial_displace(g, c, xyz)
+                assert a.overlap == b.overlap == int(r["overlap"])
+                if not a.overlap:
+                    assert abs(a.estep - r["estep"]) <= du_tol(r["estep"], T), (k, a.estep, r["estep"])
+                    errs.append(abs(a.estep - b.estep) / max(abs(a.estep), KB_KCAL * T))
+                    kinds.add((g, mv))
+                if r["accepted"]:
+                    hd.accept()
+                    hm.accept()
+                else:
+                    hd.reject()
+                    hm.reject()
+            errs = np.array(errs)
+            say(f"config 4 mixed, window {k} (walker {w}, {n0[0]} CO2 + {n0[1]} N2): scaled dU error "
+                f"max {errs.max():.3g}, p99 {np.quantile(errs, 0.99):.3g} over {len(errs)} trials, "
+                f"(guest, move) kinds {sorted(kinds)}")
+            assert len(errs) > 300
+            assert errs.max() <= 1e-6, (k, errs.max())
+    finally:
+        for hh in (h, hd, hm):
+            hh.close()
