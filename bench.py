@@ -70,6 +70,9 @@ def parse():
                     help="plain launches instead of graph replays (for counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates the HBM byte counters) and exit")
+    ap.add_argument("--phases", action="store_true",
+                    help="with libfastmc_hip_phases.so (python -m fastmc_amd._build --phases): print the per-phase "
+                         "step clock of walker 0 over the timed sweeps instead of the benchmark line")
     ap.add_argument("--only", choices=("config1", "config3", "config4"),
                     help="run only this measurement and print its JSON (for profiling it alone)")
     return ap.parse_args()
@@ -415,6 +418,10 @@ def main():
               "(state points shard across GPUs as whole walkers, with no per-trial communication)",
               file=sys.stderr)
         sys.exit(2)
+    from fastmc_amd import lib as fmc_lib
+    if args.phases:
+        from fastmc_amd._build import LIB_PHASES
+        fmc_lib.load_library(LIB_PHASES)
     from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
     from fastmc_amd.system import make_walker
 
@@ -441,6 +448,23 @@ def main():
         block, team = h.get_block(), h.get_team()
         note(f"equilibrated after {fill_sweeps} sweeps, mean N {fill_n:.1f}; {team} x {block} threads per walker; "
              f"timing {args.steps} sweeps")
+        if args.phases:
+            for _ in range(args.warmup):
+                h.run(args.moves)
+            fmc_lib.phase_clock(clear=True)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                h.run(args.moves)
+            device_sync()
+            wall = time.perf_counter() - t0
+            pc = fmc_lib.phase_clock()
+            pc["wall_us_per_step"] = wall / (args.steps * args.moves) * 1e6
+            pc["config"] = {"walkers": args.walkers, "threads_per_walker": block, "workgroups_per_walker": team,
+                            "mean_guests": round(mean_guests(h, args.walkers), 2), "launches": args.steps,
+                            "steps_per_launch": args.moves}
+            print(json.dumps({"phases": pc}), flush=True)
+            h.close()
+            return
         if args.eager:
             for _ in range(args.warmup):
                 h.run(args.moves)

@@ -22,19 +22,25 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
          "-Wno-unused-result"]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source (always) and return the library path."""
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
+LIB_PHASES = os.path.join(HERE, "libfastmc_hip_phases.so")
+
+
+def build(verbose: bool = False, phases: bool = False) -> str:
+    """Compile every source (always) and return the library path.  phases:
+    the profiling build with k_run's step clock (FMC_PHASE_CLOCK), written to
+    libfastmc_hip_phases.so."""
+    lib = LIB_PHASES if phases else LIB
+    tmp = lib + ".tmp"
+    cmd = [HIPCC, *FLAGS, *(["-DFMC_PHASE_CLOCK"] if phases else []), "-o", tmp, *SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for libfastmc_hip.so")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, phases="--phases" in sys.argv))

@@ -2473,4 +2473,25 @@ int fmc_production_stats(fmc_handle *h, int w, fmc_guest_stats *out) {
   return fmc_stats_from_windows(win.data(), n, h->hp.ntpguest, s.temp, out);
 }
 
+// Step clock of k_run (fmc_kernels.hpp, FMC_PHASE_CLOCK builds only).
+int fmc_phase_clock(unsigned long long *ticks, int n, int clear) {
+#ifdef FMC_PHASE_CLOCK
+  const int m = fmc::FMC_NPHASE + 3;
+  if (!ticks || n < m) return FMC_ERR_ARG;
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpyFromSymbol(ticks, HIP_SYMBOL(fmc::fmc_phase_ticks), sizeof(unsigned long long) * m, 0,
+                             hipMemcpyDeviceToHost));
+  if (clear) {
+    std::vector<unsigned long long> z(m, 0ULL);
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(fmc::fmc_phase_ticks), z.data(), sizeof(unsigned long long) * m, 0,
+                             hipMemcpyHostToDevice));
+  }
+  return FMC_OK;
+#else
+  (void)ticks;
+  (void)n;
+  (void)clear;
+  return FMC_ERR_ARG;  // a library built without the step clock
+#endif
+}
 }  // extern "C"

@@ -9,6 +9,34 @@
 
 namespace fmc {
 
+// Step clock (built with -DFMC_PHASE_CLOCK only): thread 0 of workgroup 0
+// (walker 0, team member 0) adds the time of each phase of k_run's steps,
+// read from the 100 MHz real-time counter, to an LDS row, and the row is
+// added to fmc_phase_ticks at the end of the launch (fmc_phase_clock reads
+// it).  Slot FMC_NPHASE holds the last stamp, FMC_NPHASE + 1 the steps and
+// FMC_NPHASE + 2 the steps that took a speculative proposal.  Without the
+// macro every stamp compiles to nothing.
+constexpr int FMC_NPHASE = 15;
+#ifdef FMC_PHASE_CLOCK
+__device__ unsigned long long fmc_phase_ticks[FMC_NPHASE + 3];
+__shared__ unsigned long long fmc_phase_row[FMC_NPHASE + 3];
+__device__ __forceinline__ void phase_stamp(int i) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const unsigned long long t = __builtin_amdgcn_s_memrealtime();
+    fmc_phase_row[i] += t - fmc_phase_row[FMC_NPHASE];
+    fmc_phase_row[FMC_NPHASE] = t;
+  }
+}
+#define FMC_PHASE(i) phase_stamp(i)
+#define FMC_PHASE_COUNT(i) \
+  do {                                                        \
+    if (threadIdx.x == 0 && blockIdx.x == 0) fmc_phase_row[i] += 1; \
+  } while (0)
+#else
+#define FMC_PHASE(i) ((void)0)
+#define FMC_PHASE_COUNT(i) ((void)0)
+#endif
+
 template <int NS>
 struct Acc {
   static constexpr int NACC = 6 * NS + 2;
@@ -1276,7 +1304,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       else ovl_old |= ov;
     }
   }
+  FMC_PHASE(3);
   __syncthreads();  // phase tables written
+  FMC_PHASE(4);
   // ---- reciprocal terms
   const int *kl;
   const double *ak;
@@ -1381,6 +1411,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       scr[P.nk + k] = so;
     }
   }
+  FMC_PHASE(5);
   // ---- workgroup reduction: within each wave by recursive halving (the
   // accumulators of an absent configuration hold zeros), then across waves
   {
@@ -1392,7 +1423,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     ax[A::NACC + 1] = (double)((ovl_old & 1) + 1024 * ((ovl_old >> 1) & 1));
     wave_sum_rows<A::NRED>(ax, lane, sh.red[wave]);
   }
+  FMC_PHASE(6);
   __syncthreads();
+  FMC_PHASE(7);
   // wave 0 finishes the sums; the decision that reads them runs on wave 0
   // too, so no further barrier is needed before it (callers that read
   // sh.tot or the overlap flags from other waves synchronise first)
@@ -1409,7 +1442,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     } else {
       v[0] = 0.0;
     }
+    FMC_PHASE(8);
     if (team > 1) v[0] = team_exchange<NS>(W, w, sh, v[0], lane, team, member);
+    FMC_PHASE(9);
     if (lane < A::NRED) sh.tot[lane] = v[0];
     if (lane >= A::NACC && lane < A::NRED) {
       const long long n = (long long)v[0];
@@ -2520,7 +2555,13 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     }
   }
   const GuestXYZ<LF> G(P, W, w, lds, member);
+#ifdef FMC_PHASE_CLOCK
+  if (threadIdx.x < FMC_NPHASE + 3) fmc_phase_row[threadIdx.x] = 0;
+#endif
   __syncthreads();
+#ifdef FMC_PHASE_CLOCK
+  if (threadIdx.x == 0) fmc_phase_row[FMC_NPHASE] = __builtin_amdgcn_s_memrealtime();
+#endif
   // A swap takes two passes of the loop: its deletion (applied provisionally)
   // and the insertion of the other kind, accepted or undone as a whole; only
   // the second pass completes a step.
@@ -2540,7 +2581,10 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
           else if (sh.prop[1].ok && sh.prop[1].used0 == u) take = 1;
         }
         take = uni(take);
-        if (take >= 0) commit<NS>(sh, sh.prop[take]);
+        if (take >= 0) {
+          commit<NS>(sh, sh.prop[take]);
+          FMC_PHASE_COUNT(FMC_NPHASE + 2);
+        }
         // at least 33 draws ahead of the step: enough for this step and the
         // next one's speculative proposals (at most 12 per step)
         if (sh.ws.mz_made - sh.ws.mz_used < 33) {
@@ -2553,7 +2597,9 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
         }
       }
     }
+    FMC_PHASE(1);
     __syncthreads();
+    FMC_PHASE(2);
     if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false);
     {
       // the next step's proposal, assuming this step is rejected, on waves 1
@@ -2569,7 +2615,9 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       }
     }
     if (threadIdx.x == 0) {
+      FMC_PHASE(10);
       decide<NS>(P, sh, 1, lds, W.trace != nullptr && w == W.trace_w && member == 0);
+      FMC_PHASE(11);
       if (sh.swap_phase != 1) {
         WState &ws = sh.ws;
         const int mk = sh.mkind;
@@ -2583,7 +2631,9 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
         ws.steps += 1;
       }
     }
+    FMC_PHASE(12);
     __syncthreads();
+    FMC_PHASE(13);
     const int phase = sh.swap_phase;
     if (phase == 1) swap_backup<NS>(P, W, w, sh, ks);
     if (sh.accepted) {
@@ -2599,6 +2649,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     } else if (phase == 2) {
       swap_restore<NS, LF>(P, W, w, sh, lds, ks);
     }
+    FMC_PHASE(14);
     if (phase == 1) continue;
     if (member > 0) {
       // the statistics, records and plots are member 0's
@@ -2631,8 +2682,14 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ws.sum_e2 += e * e;
       ws.counted += 1;
     }
+    FMC_PHASE(0);
+    FMC_PHASE_COUNT(FMC_NPHASE + 1);
     ++step;
   }
+#ifdef FMC_PHASE_CLOCK
+  if (threadIdx.x < FMC_NPHASE + 3 && threadIdx.x != FMC_NPHASE && blockIdx.x == 0)
+    atomicAdd(&fmc_phase_ticks[threadIdx.x], fmc_phase_row[threadIdx.x]);
+#endif
   store_slots<NS>(P, W, w, uni(sh.k0), uni(sh.k1), ks);
   if (P.wl_nbins > 0 && member == 0) {
     __syncthreads();

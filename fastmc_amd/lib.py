@@ -34,7 +34,7 @@ EXPORTS = ("fmc_create", "fmc_destroy", "fmc_info", "fmc_single_point", "fmc_ful
            "fmc_wl_result", "fmc_wl_isotherm", "fmc_energy_grad", "fmc_widom", "fmc_set_sampling",
            "fmc_get_windows", "fmc_stats_from_windows", "fmc_production_stats", "fmc_set_prob", "fmc_get_prob",
            "fmc_set_numguests", "fmc_get_numguests", "fmc_load_guests",
-           "fmc_set_trace_walker", "fmc_widom_noalloc")
+           "fmc_set_trace_walker", "fmc_widom_noalloc", "fmc_phase_clock")
 
 _lib = None
 
@@ -84,6 +84,25 @@ def stats_from_windows(windows, nwin: int, ntpguest: int, temperature: float):
     _chk("fmc_stats_from_windows", L.fmc_stats_from_windows(windows, C.c_int(nwin), C.c_int(ntpguest),
                                                              C.c_double(temperature), out))
     return _guest_stats(out, ntpguest)
+
+
+PHASE_NAMES = ("statistics and records at the step's end", "proposal (or commit of a speculative one)",
+               "barrier after the proposal", "pair terms (thread 0's share)", "barrier after the pair terms",
+               "reciprocal terms (thread 0's share)", "wave sums", "barrier after the wave sums", "cross-wave sums",
+               "team exchange", "until the decision", "decision", "counters", "barrier after the decision",
+               "accept")
+
+
+def phase_clock(clear: bool = False):
+    """k_run's step clock (library built with FMC_PHASE_CLOCK): per phase the
+    mean microseconds per step of walker 0, the step count and the steps that
+    took a speculative proposal."""
+    L = load_library()
+    t = (C.c_ulonglong * 18)()
+    _chk("fmc_phase_clock", L.fmc_phase_clock(t, 18, 1 if clear else 0))
+    steps = max(int(t[16]), 1)
+    return {"us_per_step": {PHASE_NAMES[i]: t[i] / steps * 1e-2 for i in range(15)},
+            "steps": int(t[16]), "speculative": int(t[17])}
 
 
 class Handle:

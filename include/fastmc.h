@@ -472,6 +472,21 @@ int fmc_set_prob(fmc_handle *h, double griddim, const int gridfactor[3], int npl
  * production steps (the cube files divide by voxel volume and step count). */
 int fmc_get_prob(fmc_handle *h, int w, int plot, double *out);
 
+/* Step clock (a profiling build: libfastmc_hip_phases.so, compiled with
+ * FMC_PHASE_CLOCK).  No counterpart in the reference, whose timer covers the
+ * whole loop (gcmc.f:1461-1462).  The run kernel of that build adds, for
+ * walker 0 (team member 0), the time of each phase of every GCMC step in
+ * ticks of the GPU's 100 MHz real-time counter: ticks[0] statistics and
+ * records at the step's end, [1] proposal (or commit of a speculative one),
+ * [2] barrier, [3] pair terms (thread 0's share), [4] barrier, [5] reciprocal
+ * terms (thread 0's share), [6] wave sums, [7] barrier, [8] cross-wave sums,
+ * [9] team exchange, [10] until the decision starts, [11] decision, [12]
+ * counters, [13] barrier, [14] accept; ticks[16] counts the steps and
+ * ticks[17] those that took a speculative proposal (ticks[15] is unused).
+ * n >= 18; clear != 0 zeroes the sums.  A library built without the clock
+ * returns FMC_ERR_ARG. */
+int fmc_phase_clock(unsigned long long *ticks, int n, int clear);
+
 #ifdef __cplusplus
 }
 #endif
