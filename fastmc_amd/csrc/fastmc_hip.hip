@@ -125,6 +125,8 @@ struct fmc_handle {
   int g_cap = 0, g_gcap = 0, g_nptcap = 0, g_efw_ready = 0;
   int *g_nmols = nullptr, *g_ptab = nullptr, *g_fwt = nullptr;
   double *g_xyz = nullptr, *g_fwq = nullptr, *g_out = nullptr, *g_vt = nullptr;
+  double *g_dqpart = nullptr;  // reciprocal d/dq partial sums by k range
+  size_t g_dqcap = 0;
 };
 
 namespace {
@@ -1268,7 +1270,7 @@ int fmc_destroy(fmc_handle *h) {
   if (h->dw.numg) (void)hipFree(h->dw.numg);
   for (void *p : {(void *)h->g_efw32, (void *)h->g_efw64, (void *)h->g_sfw, (void *)h->g_sc, (void *)h->g_nmols,
                   (void *)h->g_ptab, (void *)h->g_fwt, (void *)h->g_xyz, (void *)h->g_fwq, (void *)h->g_out,
-                  (void *)h->g_vt})
+                  (void *)h->g_vt, (void *)h->g_dqpart})
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -2147,7 +2149,10 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
   const int npt = (int)peps_t.size();
   // ---- device workspace
   const size_t NTT = (size_t)TT * TT;
-  const size_t outn = (size_t)nconf * (1 + nfw + 2 + 2 * NTT) + nfw;
+  // outputs: type pairs at stride ntypes + 1; the framework part by KSFW k ranges
+  const int ntt = nt + 1, KSFW = 16;
+  const size_t NTC = (size_t)ntt * ntt;
+  const size_t outn = (size_t)nconf * (1 + nfw + 2 + 2 * NTC) + (size_t)KSFW * nfw;
   if (!h->g_efw32) {
     HIPCHK(hipMalloc(&h->g_efw32, sizeof(float2) * (size_t)nk * nfw));
     HIPCHK(hipMalloc(&h->g_efw64, sizeof(double2) * (size_t)nk * nfw));
@@ -2167,12 +2172,12 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     HIPCHK(hipMalloc(&h->g_sc, sizeof(double2) * (size_t)h->g_cap * nk));
     HIPCHK(hipMalloc(&h->g_nmols, sizeof(int) * (size_t)h->g_cap * ng));
     HIPCHK(hipMalloc(&h->g_xyz, sizeof(double) * 3 * (size_t)h->g_cap * h->g_gcap));
-    HIPCHK(hipMalloc(&h->g_out, sizeof(double) * ((size_t)h->g_cap * (1 + nfw + 2 + 2 * NTT) + nfw)));
+    HIPCHK(hipMalloc(&h->g_out, sizeof(double) * ((size_t)h->g_cap * (1 + nfw + 2 + 2 * NTC) + (size_t)KSFW * nfw)));
   }
   if (npt > h->g_nptcap) {
     if (h->g_vt) (void)hipFree(h->g_vt);
     h->g_nptcap = npt;
-    HIPCHK(hipMalloc(&h->g_vt, sizeof(double) * 3 * (size_t)vs * std::max(npt, 1)));
+    HIPCHK(hipMalloc(&h->g_vt, sizeof(double) * (3 * (size_t)vs + 1) * std::max(npt, 1)));
   }
   GradArgs A{};
   A.nconf = nconf;
@@ -2196,9 +2201,11 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
   A.e2 = A.dq + (size_t)nconf * nfw;
   A.ev = A.e2 + nconf;
   A.dpe = A.ev + nconf;
-  A.dps = A.dpe + (size_t)nconf * NTT;
-  A.dqfw = A.dps + (size_t)nconf * NTT;
+  A.dps = A.dpe + (size_t)nconf * NTC;
+  A.dqfw = A.dps + (size_t)nconf * NTC;
+  A.ntt = ntt;
   A.vt = h->g_vt;
+  A.rpe = h->g_vt + 3 * (size_t)vs * npt;
   A.ptab = h->g_ptab;
   A.vstride = vs;
   A.npt = npt;
@@ -2207,27 +2214,49 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
                         h->stream));
   HIPCHK(hipMemcpyAsync(h->g_fwq, theta->fw_charge, sizeof(double) * nfw, hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(h->g_ptab, ptab.data(), sizeof(int) * NTT, hipMemcpyHostToDevice, h->stream));
-  if (npt)
+  if (npt) {
+    for (int i = 0; i < npt; ++i) vt.push_back(1.0 / peps_t[i]);  // theta pairs with LJ have eps > 0
     HIPCHK(hipMemcpyAsync(h->g_vt, vt.data(), sizeof(double) * vt.size(), hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(hipMemsetAsync(h->g_out, 0, sizeof(double) * outn, h->stream));
   if (!h->g_efw_ready) {
     hipLaunchKernelGGL(k_grad_phases, dim3((nfw + 255) / 256, nk), dim3(256), 0, h->stream, h->dp, A);
     h->g_efw_ready = 1;
   }
   hipLaunchKernelGGL(k_grad_sfw, dim3(nk), dim3(256), 0, h->stream, A);
-  hipLaunchKernelGGL(k_grad_sconf, dim3((nk + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A);
-  hipLaunchKernelGGL((k_grad_dq_recip<double>), dim3((nfw + 255) / 256, 1), dim3(256), 0, h->stream, h->dp, A, 1);
-  if (precision == FMC_MIXED)
-    hipLaunchKernelGGL((k_grad_dq_recip<float>), dim3((nfw + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A, 0);
-  else
-    hipLaunchKernelGGL((k_grad_dq_recip<double>), dim3((nfw + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A, 0);
-  hipLaunchKernelGGL(k_grad_pairs, dim3((nfw + gmax + 255) / 256, nconf), dim3(256), 0, h->stream, h->dp, A);
+  {
+    // guest sites in batches whose axis phase rows fill at most 48 KB of LDS
+    const size_t per = (size_t)3 * P.kcp * sizeof(double2) + sizeof(double);
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>(256, (48 * 1024) / per));
+    hipLaunchKernelGGL(k_grad_sconf, dim3(nconf), dim3(256), per * nb, h->stream, h->dp, A, nb);
+  }
+  hipLaunchKernelGGL(k_grad_dq_fw, dim3((nfw + 63) / 64, KSFW), dim3(64), 0, h->stream, h->dp, A, KSFW);
+  {
+    // k split so that the grid has at least 512 blocks
+    const int tiles = ((nfw + 255) / 256) * ((nconf + GCT - 1) / GCT);
+    A.ksplit = std::max(1, std::min(nk / 32 + 1, (512 + tiles - 1) / tiles));
+    const size_t need = (size_t)A.ksplit * nconf * nfw;
+    if (need > h->g_dqcap) {
+      if (h->g_dqpart) (void)hipFree(h->g_dqpart);
+      HIPCHK(hipMalloc(&h->g_dqpart, sizeof(double) * need));
+      h->g_dqcap = need;
+    }
+    A.dqpart = h->g_dqpart;
+    const dim3 gb((nfw + 255) / 256, (nconf + GCT - 1) / GCT, A.ksplit);
+    if (precision == FMC_MIXED) hipLaunchKernelGGL((k_grad_dq_block<float>), gb, dim3(256), 0, h->stream, h->dp, A);
+    else hipLaunchKernelGGL((k_grad_dq_block<double>), gb, dim3(256), 0, h->stream, h->dp, A);
+  }
+  hipLaunchKernelGGL(k_grad_pairs, dim3(((nfw + gmax) * GPS + 255) / 256, nconf), dim3(256),
+                     sizeof(double) * ((size_t)P.mxegrd + 3), h->stream, h->dp, A);
   HIPCHK(hipGetLastError());
   std::vector<double> out(outn);
   HIPCHK(hipMemcpyAsync(out.data(), h->g_out, sizeof(double) * outn, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   const double *urec = out.data(), *dq = urec + nconf, *e2 = dq + (size_t)nconf * nfw, *ev = e2 + nconf;
-  const double *dpe = ev + nconf, *dps = dpe + (size_t)nconf * NTT, *dqfw = dps + (size_t)nconf * NTT;
+  const double *dpe = ev + nconf, *dps = dpe + (size_t)nconf * NTC, *dqfwp = dps + (size_t)nconf * NTC;
+  std::vector<double> dqfw(nfw, 0.0);
+  for (int z = 0; z < KSFW; ++z)
+    for (int a = 0; a < nfw; ++a) dqfw[a] += dqfwp[(size_t)z * nfw + a];
   // ---- closed forms (FP64): self term, charged-system term, ewald3, tail
   const double kq = R4PIE0 / P.epsq, C = P.dc_C, alpha = P.alpha, volm = P.volm, twopi = 2.0 * PI;
   const double rc = h->rcut, rc3 = rc * rc * rc, rc9 = rc3 * rc3 * rc3;
@@ -2296,7 +2325,7 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     // pair LJ gradients, chain rule to the type parameters
     for (int i = 1; i <= nt; ++i)
       for (int j = 1; j <= i; ++j) {
-        double de = dpe[(size_t)c * NTT + i * TT + j], ds = dps[(size_t)c * NTT + i * TT + j];
+        double de = dpe[(size_t)c * NTC + i * ntt + j], ds = dps[(size_t)c * NTC + i * ntt + j];
         if (de == 0.0 && ds == 0.0) continue;
         double e = std::sqrt(theta->type_eps[i - 1] * theta->type_eps[j - 1]);
         if (i == j) {
