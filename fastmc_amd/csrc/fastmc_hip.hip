@@ -82,6 +82,7 @@ int scratch_address(void **out) {
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
   int team = 1;                  // workgroups per walker in fmc_run (k_run)
+  int variant_ovr = -1;          // k_run variant chosen by the autotuner (-1: the LDS plan's)
   int tune_team_max = KTEAM;     // largest team the block autotuner tries
   void *scratch = nullptr;       // this device's address of `scratch` (fmc_widom_noalloc)
   int trace_w = 0;               // walker whose steps fmc_run records
@@ -229,6 +230,7 @@ int run_grid(const fmc_handle *h, int team) {
 
 // The k_run variant the LDS plan selects (its index in allow_lds's list).
 int run_variant(const fmc_handle *h) {
+  if (h->variant_ovr >= 0) return h->variant_ovr;
   if (!h->hp.lds_all && h->hp.lj_lds) return 1;
   if (!h->hp.lds_all) return 0;
   return h->nsite == 3 ? 2 : 3;
@@ -1606,35 +1608,46 @@ int fmc_set_block(fmc_handle *h, int threads) {
     }
     // the tiles: threads per workgroup x workgroups per walker (each member
     // of a team takes 1 / team of the pair atoms and of the k-vectors), every
-    // team the device holds at once
-    const int prev = h->block, prev_team = h->team;
-    int best = prev, best_team = prev_team;
+    // team the device holds at once, and where the tables are read from: the
+    // LDS plan's kernel variant or the one that reads the framework, the k
+    // list and the LJ columns from global memory (L2)
+    const int prev = h->block, prev_team = h->team, prev_var = h->variant_ovr;
+    int best = prev, best_team = prev_team, best_var = prev_var;
     double best_ms = 1e30;
     const int cand[3] = {128, 256, 512};
     const int teams[8] = {1, 2, 3, 4, 6, 8, 12, 16};
-    for (int ti = 0; ti < 8 && rc == FMC_OK; ++ti) {
-      const int team = teams[ti];
-      if (team > 1 && (h->tune_team_max < team || team_arrays(h, team) != FMC_OK)) continue;
-      for (int i = 0; i < 3 && rc == FMC_OK; ++i) {
-        if (team > 1 && run_grid(h, team) > run_capacity(h, cand[i])) continue;
-        h->block = cand[i];
-        h->team = team;
-        int e = fmc_run(h, 64, nullptr, 0);
-        if (e == FMC_OK) e = fmc_run(h, 256, nullptr, 0);
-        if (e == FMC_OK && h->last_ms < best_ms) {
-          best_ms = h->last_ms;
-          best = cand[i];
-          best_team = team;
+    h->variant_ovr = -1;
+    const int plan_var = run_variant(h);
+    const int vars[2] = {-1, plan_var != 0 ? 0 : -2};
+    for (int vi = 0; vi < 2 && rc == FMC_OK; ++vi) {
+      if (vars[vi] == -2) continue;
+      h->variant_ovr = vars[vi];
+      for (int ti = 0; ti < 8 && rc == FMC_OK; ++ti) {
+        const int team = teams[ti];
+        if (team > 1 && (h->tune_team_max < team || team_arrays(h, team) != FMC_OK)) continue;
+        for (int i = 0; i < 3 && rc == FMC_OK; ++i) {
+          if (team > 1 && run_grid(h, team) > run_capacity(h, cand[i])) continue;
+          h->block = cand[i];
+          h->team = team;
+          int e = fmc_run(h, 64, nullptr, 0);
+          if (e == FMC_OK) e = fmc_run(h, 256, nullptr, 0);
+          if (e == FMC_OK && h->last_ms < best_ms) {
+            best_ms = h->last_ms;
+            best = cand[i];
+            best_team = team;
+            best_var = vars[vi];
+          }
+          for (auto &sn : snap)
+            if (hipMemcpy(sn.dev, sn.copy, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = FMC_ERR_HIP;
+          if (e != FMC_OK && rc == FMC_OK) rc = e;
         }
-        for (auto &sn : snap)
-          if (hipMemcpy(sn.dev, sn.copy, sn.bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = FMC_ERR_HIP;
-        if (e != FMC_OK && rc == FMC_OK) rc = e;
       }
     }
     for (auto &sn : snap)
       if (sn.copy) (void)hipFree(sn.copy);
     h->block = rc == FMC_OK ? best : prev;
     h->team = rc == FMC_OK ? best_team : prev_team;
+    h->variant_ovr = rc == FMC_OK ? best_var : prev_var;
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     h->graphs.clear();
     return rc;

@@ -112,6 +112,9 @@ struct Smem {
   Prop prop[3];
   int spec_on;
   long long mz_after;
+  // the move of the last completed step, for the work after it (numguests
+  // records on wave 1) while wave 0 already commits the next proposal
+  int done_mkind, done_g, done_swap_ig, done_swap_jg;
 };
 
 constexpr int KPT = 2;  // k-vectors per thread held in registers for a whole launch
@@ -2426,7 +2429,7 @@ template <int NS>
 __device__ inline void numg_step(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh) {
   WState &ws = sh.ws;
   const int lane = threadIdx.x & 63;
-  if (lane < P.ntpguest && sh.mkind != FMC_MOVE_SWITCH) {
+  if (lane < P.ntpguest && sh.done_mkind != FMC_MOVE_SWITCH) {
     const int m = P.locguest[lane + 1];
     ws.delE_last[m] = sh.accepted ? sh.delE[m] : 0.0;
   }
@@ -2435,9 +2438,9 @@ __device__ inline void numg_step(const DevParams &P, const DevWalkers &W, int w,
   const bool room = n < P.numg_cap;
   if (room && lane < P.ntpguest) {
     fmc_numguests_rec *r = W.numg + (size_t)w * P.numg_cap + n;
-    const int g = lane + 1, m = P.locguest[g], mk = sh.mkind;
+    const int g = lane + 1, m = P.locguest[g], mk = sh.done_mkind;
     const bool swap = mk == FMC_MOVE_SWAP;
-    const int gm = swap ? sh.swap_ig : sh.g;
+    const int gm = swap ? sh.done_swap_ig : sh.done_g;
     if (lane == 0) r->step = ws.steps;
     r->nmols[lane] = ws.nmols[m];
     r->energy[lane] = ws.energy[m];
@@ -2450,7 +2453,7 @@ __device__ inline void numg_step(const DevParams &P, const DevWalkers &W, int w,
                         : mk == FMC_MOVE_ROTATE    ? 3
                                                    : 0;
     r->moves[lane][3] = mk == FMC_MOVE_JUMP && gm == g;
-    r->moves[lane][4] = swap && (sh.swap_ig == g || sh.swap_jg == g);
+    r->moves[lane][4] = swap && (sh.done_swap_ig == g || sh.done_swap_jg == g);
     r->moves[lane][5] = 0;  // a switch does not move with fewer than three guest kinds
   }
   if (lane == 0) {
@@ -2621,6 +2624,10 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       if (sh.swap_phase != 1) {
         WState &ws = sh.ws;
         const int mk = sh.mkind;
+        sh.done_mkind = mk;
+        sh.done_g = sh.g;
+        sh.done_swap_ig = sh.swap_ig;
+        sh.done_swap_jg = sh.swap_jg;
         if (mk != FMC_MOVE_SWITCH) ws.att[mk] += 1;
         if (sh.accepted) {
           ws.acc[mk] += 1;
