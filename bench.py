@@ -70,6 +70,8 @@ def parse():
                     help="plain launches instead of graph replays (for counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (calibrates the HBM byte counters) and exit")
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
+                    help="precision of the timed batch (the metric is quoted in mixed precision)")
     ap.add_argument("--phases", action="store_true",
                     help="with libfastmc_hip_phases.so (python -m fastmc_amd._build --phases): print the per-phase "
                          "step clock of walker 0 over the timed sweeps instead of the benchmark line")
@@ -436,7 +438,7 @@ def main():
                                                    (37 * i) % 168),
                                        temperature=s.temperature, pressures=s.pressures,
                                        framework_xyz=s.framework_xyz(), keep=keep))
-        h = Handle(base, walkers=walkers, precision=FMC_MIXED)
+        h = Handle(base, walkers=walkers, precision=FMC_MIXED if args.precision == "mixed" else FMC_FP64)
         # the state points are filled to their equilibrium loading first
         note(f"{args.walkers} walkers; equilibrating")
         fill_sweeps, fill_n = equilibrate(h, args.walkers, args.moves, args.fill) if args.fill > 0 else (0, 0.0)

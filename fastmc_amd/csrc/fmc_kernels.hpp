@@ -95,7 +95,7 @@ struct Smem {
   int kmol[KMAXS + 1], kna[KMAXS + 1], kslot[KMAXS + 1];
   // first index of each guest kind in the pair loop's atom space (framework
   // atoms first) and the end of the space; kept current by apply
-  int gstart[KMAXS + 1], ntot;
+  int gstart[KMAXS + 1], ntot, a0, a1;
   // constants of the current trial, computed while the energy terms run
   double qfix0, pref;
   double qfm[KMAXS + 1];
@@ -364,7 +364,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   };
   const int wl_on = uni(ws.wl_on);
   int g = 1, kind, wlmode = 0;
-  if (wl_on && uni(ws.nmols[P.locguest[1]]) < uni(ws.wl_preload)) {
+  if (__builtin_expect(wl_on && uni(ws.nmols[P.locguest[1]]) < uni(ws.wl_preload), 0)) {
     // insert_guests (mc_moves.f:84-149) up to the window's lower edge
     wlmode = 2;
     kind = FMC_MOVE_INSERT;
@@ -378,7 +378,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
     const double randmov = draw();
     const float *f = ws.mcf[g];
     kind = uni(choose_move(f, randmov, false, false, !wl_on));
-    if (wl_on && g == 1) {
+    if (__builtin_expect(wl_on && g == 1, 0)) {
       // window edges of the flat-histogram loop, wang_landau.f:161-235 (the
       // visit bookkeeping is written by every lane with the same values)
       wlmode = 1;
@@ -430,7 +430,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
   int chg = 0, cnt = 0, cval = 0;
   if (!wl_on && nmols >= 1) {
     bool chd = false, chr = false, cht = false, chq = false;
-    if (dcnt >= 1) {
+    if (__builtin_expect(dcnt >= 1 && (dcnt % 50 == 0), 0)) {
       if (dcnt % 100 == 0) {
         if ((dacc / (double)dcnt) > 0.5) delrdisp *= 1.05, chd = true;
         else if (delrdisp > DELRMIN) delrdisp *= 0.95, chd = true;
@@ -439,11 +439,11 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
         else if (rotangle > MINANGLE) rotangle *= 0.95, chr = true;
       }
     }
-    if (tcnt >= 1 && tcnt % 100 == 0) {
+    if (__builtin_expect(tcnt >= 1 && tcnt % 100 == 0, 0)) {
       if ((tacc / (double)tcnt) > 0.5) tran_delr *= 1.05, cht = true;
       else if (tran_delr > DELRMIN) tran_delr *= 0.95, cht = true;
     }
-    if (rcnt >= 1 && rcnt % 100 == 0) {
+    if (__builtin_expect(rcnt >= 1 && rcnt % 100 == 0, 0)) {
       if ((racc / (double)rcnt) > 0.5) rota_rotangle *= 1.05, chq = true;
       else if (rota_rotangle > MINANGLE) rota_rotangle *= 0.95, chq = true;
     }
@@ -495,7 +495,7 @@ __device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh,
       o.xo[i][1] = y;
       o.xo[i][2] = z;
     }
-    if (mkind == FMC_MOVE_SWAP && !spec) {
+    if (__builtin_expect(mkind == FMC_MOVE_SWAP && !spec, 0)) {
       double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
       for (int k = 0; k < natms; ++k) {
         const double m = sh.sm[mol][k];
@@ -1089,7 +1089,7 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
     const long long *f = (const long long *)(recs + (size_t)lane * XREC + XREC - 1);
     long long spin = 0;
     while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != x) {
-      if (++spin > XSPIN) {
+      if (__builtin_expect(++spin > XSPIN, 0)) {
         fail = 1;
         break;
       }
@@ -1103,7 +1103,7 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
     for (int q = 0; q < team; ++q) t += q == member ? v : __hip_atomic_load(recs + (size_t)q * XREC + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   fail = __any(fail);
   int same = 1;
-  if (fast < 0 && lane < team)
+  if (__builtin_expect(fast < 0, 0) && lane < team)
     same = __hip_atomic_load(recs + (size_t)lane * XREC + A::NRED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
            (double)xcd_id();
   same = __all(same);
@@ -1216,8 +1216,8 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   const int excl_lo = uni(sh.excl_lo), excl_hi = uni(sh.excl_hi);
   const int team = uni(sh.team), member = uni(sh.member);
   const int k0 = uni(sh.k0), k1 = uni(sh.k1);
-  // a team member takes the atoms [a0, a1) of the pair loop
-  const int a0 = (int)((long long)ntot * member / team), a1 = (int)((long long)ntot * (member + 1) / team);
+  // a team member takes the atoms [a0, a1) of the pair loop (gstart_from)
+  const int a0 = uni(sh.a0), a1 = uni(sh.a1);
   const GuestXYZ<LF> G(P, W, w, lds, member);
   const double *fx, *fy, *fz, *fq;
   const int *fi;
@@ -1402,7 +1402,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
       }
     }
   }
-  if (k1 - k0 > KPT * nt) {
+  if (__builtin_expect(k1 - k0 > KPT * nt, 0)) {
     const double2 *skw = W.sk + (size_t)w * (NS + 1) * P.nk;
     for (int k = k0 + KPT * nt + tid; k < k1; k += nt) {
       double2 S[NS + 1], sn = make_double2(0.0, 0.0), so = make_double2(0.0, 0.0);
@@ -1562,13 +1562,13 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
   sh.ovl_new = overlap;
   sh.rande = -1.0;
   sh.drew = 0;
-  if (swap_phase == 1) {
+  if (__builtin_expect(swap_phase == 1, 0)) {
     // mc_swap's deletion, accepted provisionally (gcmc.f:2345-2353)
     sh.swap_estepi = -estep;
     sh.accepted = 1;
     return;
   }
-  if (swap_phase == 2) {
+  if (__builtin_expect(swap_phase == 2, 0)) {
     // mc_swap's test (gcmc.f:2390-2413): the deletion and the insertion each
     // against the reference's own factors (volume in A^3), the second draw
     // only when the first passes; jnmols counts the inserted molecule
@@ -1597,10 +1597,10 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
     return;
   }
   int accepted = 0;
-  if (draw && sh.wlmode == 2) {
+  if (__builtin_expect(draw && sh.wlmode == 2, 0)) {
     // insert_guests: kept whenever it does not overlap (mc_moves.f:126-142)
     accepted = !overlap;
-  } else if (draw && sh.wlmode == 1 && (kind == FMC_MOVE_INSERT || kind == FMC_MOVE_DELETE)) {
+  } else if (__builtin_expect(draw && sh.wlmode == 1 && (kind == FMC_MOVE_INSERT || kind == FMC_MOVE_DELETE), 0)) {
     // eval_wl_move, wang_landau.f:540-588: the grand-canonical weight without
     // the fugacity, biased by exp(ln g(N) - ln g(N +- 1))
     const double beta = ws.beta;
@@ -1661,12 +1661,12 @@ __device__ void decide(const DevParams &P, Smem<NS> &sh, int draw, unsigned char
       }
     }
   }
-  if (draw && accepted && kind == FMC_MOVE_INSERT && ws.nmols[mol] >= P.gcap[g]) {
+  if (__builtin_expect(draw && accepted && kind == FMC_MOVE_INSERT && ws.nmols[mol] >= P.gcap[g], 0)) {
     accepted = 0;  // guest slots full: the walker stops growing and says so
     ws.overflow = 1;
   }
   sh.accepted = accepted;
-  if (draw && ws.wl_on) {
+  if (__builtin_expect(draw && ws.wl_on, 0)) {
     const WLView v = wl_view(P, lds);
     if (sh.wlmode == 1) {
       // the bin of the resulting N (wang_landau.f:668-677, 757-766)
@@ -2200,6 +2200,10 @@ __device__ inline void gstart_from(const DevParams &P, const int *nmols, Smem<NS
     }
   }
   sh.ntot = n;
+  // this team member's atoms [a0, a1) of the pair loop (sh.team and
+  // sh.member are set before the first call)
+  sh.a0 = n * sh.member / sh.team;
+  sh.a1 = n * (sh.member + 1) / sh.team;
 }
 
 // The walker's state into LDS; team / member: the workgroup's place in the
@@ -2231,11 +2235,11 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.sm[i / KSITE][i % KSITE] = P.site_m[i / KSITE][i % KSITE];
   }
   if (t == 0) {
+    sh.team = team;
+    sh.member = member;
     gstart_from(P, W.ws[w].nmols, sh);
     sh.mkind = 0;
     sh.swap_phase = 0;
-    sh.team = team;
-    sh.member = member;
     sh.k0 = (int)((long long)P.nk * member / team);
     sh.k1 = (int)((long long)P.nk * (member + 1) / team);
     sh.xseq = team > 1 ? W.xcnt[w] : 0;
@@ -2572,7 +2576,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     if (sh.ws.wl_done) break;  // converged Wang-Landau walker (uniform: written before the last barrier)
     if (sh.team_fail) break;   // a team wait gave up (the walker is flagged)
     if (threadIdx.x < 64) {
-      if (sh.swap_phase == 1) {
+      if (__builtin_expect(sh.swap_phase == 1, 0)) {
         swap_insert_setup<NS>(P, sh);
       } else {
         // after a rejected step, the proposal built for it speculatively
@@ -2590,7 +2594,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
         }
         // at least 33 draws ahead of the step: enough for this step and the
         // next one's speculative proposals (at most 12 per step)
-        if (sh.ws.mz_made - sh.ws.mz_used < 33) {
+        if (__builtin_expect(sh.ws.mz_made - sh.ws.mz_used < 33, 0)) {
           mzt_refill(sh.ws, threadIdx.x);
           if (threadIdx.x == 0) mzt_advance(sh.ws);
         }
@@ -2642,7 +2646,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     __syncthreads();
     FMC_PHASE(13);
     const int phase = sh.swap_phase;
-    if (phase == 1) swap_backup<NS>(P, W, w, sh, ks);
+    if (__builtin_expect(phase == 1, 0)) swap_backup<NS>(P, W, w, sh, ks);
     if (sh.accepted) {
       apply<NS, LF, true>(P, W, w, sh, lds, ks);
       if (threadIdx.x == 0 && sh.kind != FMC_MOVE_INSERT && sh.kind != FMC_MOVE_DELETE) surf_displaced<NS>(sh);
@@ -2651,9 +2655,9 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
         int due = sh.ws.acc_since_resum + 1 >= P.resum_every;
         __syncthreads();
         if (threadIdx.x == 0) sh.ws.acc_since_resum += 1;
-        if (due) resum_walker<NS, LF>(P, W, w, sh, lds, ks);
+        if (__builtin_expect(due, 0)) resum_walker<NS, LF>(P, W, w, sh, lds, ks);
       }
-    } else if (phase == 2) {
+    } else if (__builtin_expect(phase == 2, 0)) {
       swap_restore<NS, LF>(P, W, w, sh, lds, ks);
     }
     FMC_PHASE(14);
@@ -2663,15 +2667,15 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       ++step;
       continue;
     }
-    if (threadIdx.x == 0 && W.trace && w == W.trace_w) record<NS>(P, W, sh, step);
+    if (__builtin_expect(threadIdx.x == 0 && W.trace && w == W.trace_w, 0)) record<NS>(P, W, sh, step);
     // the step's statistics on wave 1, idle while wave 0 proposes the next
     // step (the fields written here are read nowhere before the next barrier)
     const int stat_wave = blockDim.x > 64 ? 1 : 0;
     if ((int)(threadIdx.x >> 6) == stat_wave && (count_stats & FMC_STATS_COUNT))
       stats_step<NS>(P, W, w, sh, (count_stats & FMC_STATS_CLOSE) && step == nsteps - 1);
-    if ((int)(threadIdx.x >> 6) == stat_wave && P.nnumg > 0) numg_step<NS>(P, W, w, sh);
+    if (__builtin_expect((int)(threadIdx.x >> 6) == stat_wave && P.nnumg > 0, 0)) numg_step<NS>(P, W, w, sh);
     // probability plots on waves 2 and up (every thread of a smaller block)
-    if (P.nplots > 0 && (count_stats & FMC_STATS_COUNT)) {
+    if (__builtin_expect(P.nplots > 0 && (count_stats & FMC_STATS_COUNT), 0)) {
       const int p0 = blockDim.x > 128 ? 128 : 0;
       if ((int)threadIdx.x >= p0) prob_step<LF>(P, W, w, sh.ws.nmols, G, threadIdx.x - p0, blockDim.x - p0);
     }
