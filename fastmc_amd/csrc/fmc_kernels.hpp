@@ -1433,13 +1433,16 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   // too, so no further barrier is needed before it (callers that read
   // sh.tot or the overlap flags from other waves synchronise first)
   if (wave == 0) {
-    double v[KWAVES];
+    // pairwise over the waves (missing waves count as zero); the kernels that
+    // evaluate trials run at most 512 threads, 8 waves
+    constexpr int KWR = 8;
+    static_assert(KWR <= KWAVES, "reduction rows");
+    double v[KWR];
     if (lane < A::NRED) {
-      // pairwise over the waves (missing waves count as zero)
 #pragma unroll
-      for (int wv = 0; wv < KWAVES; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
+      for (int wv = 0; wv < KWR; ++wv) v[wv] = wv < nwaves ? sh.red[wv][lane] : 0.0;
 #pragma unroll
-      for (int st = KWAVES / 2; st >= 1; st >>= 1)
+      for (int st = KWR / 2; st >= 1; st >>= 1)
 #pragma unroll
         for (int wv = 0; wv < st; ++wv) v[wv] += v[wv + st];
     } else {
