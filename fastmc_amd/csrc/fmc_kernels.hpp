@@ -350,9 +350,13 @@ __device__ inline int choose_move(const float *f, double randmov, bool skip_ins,
 // lanes with identical values), which runs for live proposals only.
 // spec: a speculative proposal for the next step (it leaves a swap's set-up
 // undone and marks the record not to be taken).
+// Kept out of line: one copy serves the live and the speculative calls, which
+// keeps k_run's code within the instruction cache (the guest-coordinate view
+// is passed by value for the same reason: it is two pointers).
 template <int NS, bool LF>
-__device__ void propose(const DevParams &P, const GuestXYZ<LF> &G, Smem<NS> &sh, unsigned char *lds,
-                        const long long used0, Prop &o, const bool spec) {
+__device__ __attribute__((noinline)) void propose(const DevParams &P, const GuestXYZ<LF> G, Smem<NS> &sh,
+                                                  unsigned char *lds, const long long used0, Prop &o,
+                                                  const bool spec) {
   WState &ws = sh.ws;
   const int lane = threadIdx.x & 63;
   const int rbits = __float_as_int(ws.rq[(int)((used0 + lane) & 127)]);
