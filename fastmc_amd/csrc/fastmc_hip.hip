@@ -82,6 +82,7 @@ int scratch_address(void **out) {
 struct fmc_handle {
   int nwalkers = 0, precision = FMC_FP64, ns = 0, block = 512;
   int team = 1;                  // workgroups per walker in fmc_run (k_run)
+  int xch_team = 0;              // team whose exchanges the records hold (0: cleared)
   int variant_ovr = -1;          // k_run variant chosen by the autotuner (-1: the LDS plan's)
   int tune_team_max = KTEAM;     // largest team the block autotuner tries
   void *scratch = nullptr;       // this device's address of `scratch` (fmc_widom_noalloc)
@@ -309,6 +310,17 @@ int team_error(fmc_handle *h) {
   HIPCHK(hipMemset(h->dw.xcnt, 0, sizeof(long long) * (size_t)h->nwalkers));
   HIPCHK(hipMemset(h->dw.xerr, 0, sizeof(int)));
   return FMC_ERR_TEAM;
+}
+
+// Before a team run: the records hold only exchanges of the current team (a
+// record that another team size left behind could carry a current tag), so
+// they are cleared when the team changes.  Outside any graph capture.
+int team_records(fmc_handle *h) {
+  if (h->team <= 1 || h->team == h->xch_team) return FMC_OK;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemset(h->dw.xch, 0, sizeof(double) * (size_t)h->nwalkers * 2 * KTEAM * XREC));
+  h->xch_team = h->team;
+  return FMC_OK;
 }
 
 // The private guest arrays of members 1 .. team - 1 (grown on demand).
@@ -1503,6 +1515,7 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
   if (!h || nsteps < 0) return FMC_ERR_ARG;
   if (nsteps == 0) return FMC_OK;
   if (team_fits(h) != FMC_OK) return FMC_ERR_ARG;
+  if (team_records(h) != FMC_OK) return FMC_ERR_HIP;
   DevWalkers dw = h->dw;
   if (trace) {
     if (h->dtrace_cap < nsteps) {
@@ -1535,6 +1548,7 @@ int fmc_run(fmc_handle *h, int nsteps, fmc_trace_rec *trace, int count_stats) {
 
 int fmc_run_graph(fmc_handle *h, int nsweeps, int nsteps_per_sweep, int count_stats) {
   if (!h || nsweeps < 1 || nsteps_per_sweep < 1) return FMC_ERR_ARG;
+  if (team_records(h) != FMC_OK) return FMC_ERR_HIP;
   GraphKey key{nsweeps, nsteps_per_sweep, count_stats};
   auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {

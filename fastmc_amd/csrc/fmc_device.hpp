@@ -195,11 +195,11 @@ struct DevWalkers {
   fmc_window *win;           // [w][maxwin] closed averaging windows
   double *prob;              // [w][nplots][pgridsize] probability plots
   fmc_numguests_rec *numg;   // [w][numg_cap] guest-count trace
-  // teams: exchange records [w][2 (exchange parity)][KTEAM][XREC], the
-  // number of exchanges each walker has made (never reset: a record whose
-  // number is not the current one is old), and the private guest
-  // coordinates of members 1.. when they live in global memory
-  // [w][team_alloc - 1][3][gatoms_total]
+  // teams: exchange records [w][2 (exchange parity)][KTEAM][XREC] (row
+  // values tagged with their exchange in the low mantissa bits; zero: no
+  // value is current), the number of exchanges each walker has made, and
+  // the private guest coordinates of members 1.. when they live in global
+  // memory [w][team_alloc - 1][3][gatoms_total]
   double *xch;
   long long *xcnt;
   double *gxt;

@@ -104,7 +104,6 @@ struct Smem {
   int team, member, k0, k1;
   long long xseq;
   int team_fail;
-  int xfast;  // team_exchange's ordering: -1 not yet known, 1 one XCD, 0 several
   // proposals: [0], [1] built speculatively for the next step assuming this
   // one is rejected after 0 / 1 draws of its decision, [2] the live one;
   // spec_on: [0] and [1] were built this step; mz_after: the ring position
@@ -1052,69 +1051,58 @@ __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
 // seam: also write s_new / s_old of the register slots to the scratch rows.
 // The team's sums of one step (lanes < NRED of wave 0, v = this member's
 // sum of row `lane`): each member publishes its row sums in its record of
-// the exchange's parity, the store of the exchange number last; lane q < team
-// waits for member q's number (bounded: XSPIN polls, then the walker is
-// flagged and the members leave their loops); the rows are then added in
-// member order, so that every member holds the same bits.
+// the exchange's parity and gathers the other members' rows; the rows are
+// then added in member order, so that every member holds the same bits.
 //
-// Ordering.  The first exchange of a launch is a release store of the number
-// (which writes the L2 of this XCD back to memory) and an acquire fence after
-// the wait, visible across the whole device; with it every member publishes
-// the id of the XCD it runs on (hardware register XCC_ID).  When all members
-// share one XCD they share its L2, the point where their agent-scope stores
-// and loads meet, and the later exchanges of the launch only wait until this
-// wave's row stores are done (s_waitcnt vmcnt(0)) before the number is
-// stored, and read the rows with agent-scope loads after the number: the
-// same order without the L2 write-back and invalidation.  Members on
-// different XCDs keep the release / acquire form.
-__device__ __forceinline__ int xcd_id() { return (int)__builtin_amdgcn_s_getreg(20 | (3 << 11)); }  // HW_REG_XCC_ID[3:0]
-
+// Each value carries its exchange in its two lowest mantissa bits (bit 1
+// set, bit 0 = bit 1 of the exchange number x; the record of x's parity
+// last held x - 2, whose bit differs, and a cleared record has bit 1 clear),
+// and those bits are cleared before the value is used, by every member for
+// every row, its own included (a relative change below 2^-50).  A value is
+// then complete in itself: members store their rows with relaxed
+// agent-scope 64-bit atomics, readers poll each row until its tag is the
+// current one (bounded: XSPIN polls, then the walker is flagged and the
+// members leave their loops), and no fence, store completion wait or
+// separate flag orders the two.  The host clears the records whenever the
+// team changes, so that a member's record holds only this team's exchanges.
+// The gathered rows are staged in sh.red, which wave 0 has finished reading.
 template <int NS>
 __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem<NS> &sh, double v, int lane, int team,
                                                 int member) {
-  using A = Acc<NS>;
-  static_assert(A::NRED < XREC - 1, "exchange record too short");
+  constexpr int NR = Acc<NS>::NRED;
+  static_assert(NR <= XREC, "exchange record too short");
+  static_assert(KTEAM <= KWAVES, "exchange rows staged in sh.red");
   const long long x = sh.xseq + 1;
-  const int fast = uni(sh.xfast);  // 1: all members on this XCD; 0: not; -1: first exchange (find out)
-  double *recs = W.xch + ((size_t)w * 2 + (size_t)(x & 1)) * KTEAM * XREC;
-  double *mine = recs + (size_t)member * XREC;
-  if (lane < A::NRED) __hip_atomic_store(mine + lane, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (lane == A::NRED)
-    __hip_atomic_store(mine + A::NRED, (double)xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  long long *myflag = (long long *)(mine + XREC - 1);
-  if (fast == 1) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (lane == 0) __hip_atomic_store(myflag, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    if (lane == 0) __hip_atomic_store(myflag, x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  const long long tag = 2 | ((x >> 1) & 1);
+  long long *recs = (long long *)W.xch + ((size_t)w * 2 + (size_t)(x & 1)) * KTEAM * XREC;
+  if (lane < NR) {
+    const long long b = (__double_as_longlong(v) & ~3LL) | tag;
+    __hip_atomic_store(recs + (size_t)member * XREC + lane, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh.red[member][lane] = __longlong_as_double(b & ~3LL);
   }
   int fail = 0;
-  if (lane < team && lane != member) {
-    const long long *f = (const long long *)(recs + (size_t)lane * XREC + XREC - 1);
-    long long spin = 0;
-    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != x) {
+  for (int i = lane; i < team * NR; i += 64) {
+    const int q = i / NR, row = i - q * NR;
+    if (q == member) continue;
+    const long long *src = recs + (size_t)q * XREC + row;
+    long long b, spin = 0;
+    while (((b = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 3) != tag) {
       if (__builtin_expect(++spin > XSPIN, 0)) {
         fail = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    sh.red[q][row] = __longlong_as_double(b & ~3LL);
   }
-  if (fast == 1) asm volatile("" ::: "memory");
-  else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  double t = 0.0;
-  if (lane < A::NRED)
-    for (int q = 0; q < team; ++q) t += q == member ? v : __hip_atomic_load(recs + (size_t)q * XREC + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   fail = __any(fail);
-  int same = 1;
-  if (__builtin_expect(fast < 0, 0) && lane < team)
-    same = __hip_atomic_load(recs + (size_t)lane * XREC + A::NRED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-           (double)xcd_id();
-  same = __all(same);
+  __builtin_amdgcn_wave_barrier();  // the rows other lanes staged are read below
+  double t = 0.0;
+  if (lane < NR)
+    for (int q = 0; q < team; ++q) t += sh.red[q][lane];
   if (lane == 0) {
     sh.xseq = x;
-    if (fast < 0) sh.xfast = same && !fail;
-    if (fail) {
+    if (__builtin_expect(fail, 0)) {
       sh.team_fail = 1;
       sh.ws.overflow |= FMC_OVF_TEAM;
       *W.xerr = 1;
@@ -2251,7 +2239,6 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.k1 = (int)((long long)P.nk * (member + 1) / team);
     sh.xseq = team > 1 ? W.xcnt[w] : 0;
     sh.team_fail = 0;
-    sh.xfast = -1;
     sh.spec_on = 0;
     sh.accepted = 0;
   }
