@@ -923,12 +923,12 @@ __device__ inline void site_phase(const float2 *ph, int kcp, int ll, int mm, int
 // Copies the read-only per-launch data into LDS: the erfc table and the
 // constants block always; with LF (the plan made room for everything) also
 // the walker's framework, the k list, the LJ columns of the guest sites (in
-// the precision T of the mode, FP32 only) and the guest coordinates.
-// LF: framework, k list, guest coordinates and (mixed mode) the LJ columns
-// into LDS; LJL: the LJ columns alone (either precision).
+// the precision T of the mode) and the guest coordinates.
+// LF: framework, k list and guest coordinates into LDS; LJL: the LJ columns
+// (either precision).
 template <typename T, bool LF, bool LJL>
 __device__ inline void stage_tables(const DevParams &P, const DevWalkers &W, int w, unsigned char *lds) {
-  constexpr bool LV = (LF && sizeof(T) == 4) || LJL;
+  constexpr bool LV = LJL;
   const int tid = threadIdx.x, nt = blockDim.x;
   double *erc = (double *)(lds + P.off_erc);
   for (int i = tid; i < P.mxegrd + 3; i += nt) erc[i] = P.erc[i];
@@ -1115,7 +1115,12 @@ template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                          KSlot<NS> *ks, bool seam) {
   using A = Acc<NS>;
-  constexpr bool LV = (LF && sizeof(T) == 4) || LJL;  // LJ columns in LDS (stage_tables)
+  // LJ columns in LDS (stage_tables), in the precision T of the mode; read
+  // from global memory (L2) they are FP64 in both modes: on gfx950 an FP32
+  // interpolation issues at the FP64 rate and adds the conversions and the
+  // wall test, so FP32 values pay only where they halve the LDS they take
+  constexpr bool LV = LJL;
+  using TP = typename std::conditional<LV, T, double>::type;
   constexpr int NSR = NSITE > 0 ? NSITE : KSITE;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int natms = uni(sh.natms), mol = uni(sh.mol), g = uni(sh.g);
@@ -1174,7 +1179,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   const int *lst = (const int *)(lds + P.off_cst + CST_LST);
   LJTab vvv;
   if constexpr (LV) vvv = lj_lds<T>(P, lds);
-  else vvv = lj_global(P, sizeof(T) == 4);
+  else vvv = lj_global(P, false);
   // NSITE > 0: site charges and overlap thresholds in registers; padding
   // sites (s >= natms) carry charge 0, no LJ column, no overlap or surface
   // test, so whatever their LDS coordinates hold is selected away.  The site
@@ -1282,10 +1287,10 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         if (NSITE == 0 && s >= natms) break;
         double e2, ev;
         if constexpr (NSITE > 0)
-          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], cq[s], col[s], ov2[s], sg2[s], xj, yj, zj, qj, e2,
+          pair_eval<TP>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], cq[s], col[s], ov2[s], sg2[s], xj, yj, zj, qj, e2,
                        ev, ov);
         else
-          pair_eval<T>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, sg2[s], xj, yj,
+          pair_eval<TP>(P, erc, vvv, c == 0 ? sh.xn[s] : sh.xo[s], sh.scq[mol][s], col[s], sh.ovl2, sg2[s], xj, yj,
                        zj, qj, e2, ev, ov);
         er += e2;
         evd += ev;
