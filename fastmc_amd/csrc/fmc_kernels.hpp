@@ -354,8 +354,17 @@ __device__ inline int choose_move(const float *f, double randmov, bool skip_ins,
 // is passed by value for the same reason: it is two pointers).
 template <int NS, bool LF>
 __device__ __attribute__((noinline)) void propose(const DevParams &P, const GuestXYZ<LF> G, Smem<NS> &sh,
-                                                  unsigned char *lds, const long long used0, Prop &o,
+                                                  unsigned char *lds, const long long used0, const int slot,
                                                   const bool spec) {
+  Prop &o = sh.prop[slot];
+  // guest coordinates: with LF their LDS copy, addressed from the LDS base
+  // here so that the reads are LDS reads
+  const double *Gx = G.x, *Gy = G.y, *Gz = G.z;
+  if constexpr (LF) {
+    Gx = (const double *)(lds + P.off_g);
+    Gy = Gx + P.gatoms_total;
+    Gz = Gy + P.gatoms_total;
+  }
   WState &ws = sh.ws;
   const int lane = threadIdx.x & 63;
   const int rbits = __float_as_int(ws.rq[(int)((used0 + lane) & 127)]);
@@ -492,7 +501,7 @@ __device__ __attribute__((noinline)) void propose(const DevParams &P, const Gues
     // the other guest kind (gcmc.f:2311-2322)
     choice = uni(mkind == FMC_MOVE_SWAP ? (int)floor(draw() * nmols + 1) : (int)floor(draw() * nmols) + 1);
     const int a = P.gbase[g] + (choice - 1) * natms + i;
-    const double x = G.x[a], y = G.y[a], z = G.z[a];
+    const double x = Gx[a], y = Gy[a], z = Gz[a];
     if (site) {
       o.xo[i][0] = x;
       o.xo[i][1] = y;
@@ -531,9 +540,9 @@ __device__ __attribute__((noinline)) void propose(const DevParams &P, const Gues
       // old COM there
       choice = uni((int)floor(draw() * nmols) + 1);
       const int at = P.gbase[g] + (choice - 1) * natms + i;
-      ox = G.x[at];
-      oy = G.y[at];
-      oz = G.z[at];
+      ox = Gx[at];
+      oy = Gy[at];
+      oz = Gz[at];
       double wsum = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
       for (int k = 0; k < natms; ++k) {
         const double m = sh.sm[mol][k];
@@ -581,9 +590,9 @@ __device__ __attribute__((noinline)) void propose(const DevParams &P, const Gues
       }
       choice = uni((int)floor(draw() * nmols) + 1);
       const int at = P.gbase[g] + (choice - 1) * natms + i;
-      ox = G.x[at];
-      oy = G.y[at];
-      oz = G.z[at];
+      ox = Gx[at];
+      oy = Gy[at];
+      oz = Gz[at];
     }
     // translate, utility_pack.f:3161-3218: move, then wrap the COM into the cell
     double nx = a + ox, ny = b + oy, nz = c + oz;
@@ -2598,7 +2607,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
           if (threadIdx.x == 0) mzt_advance(sh.ws);
         }
         if (take < 0) {
-          propose<NS, LF>(P, G, sh, lds, sh.ws.mz_used, sh.prop[2], false);
+          propose<NS, LF>(P, G, sh, lds, sh.ws.mz_used, 2, false);
           commit<NS>(sh, sh.prop[2]);
         }
       }
@@ -2616,7 +2625,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
       const int wv = threadIdx.x >> 6;
       if (wv == 1 || wv == 2) {
         const bool spec = P.spec_prop && uni(sh.swap_phase) == 0 && !uni(sh.ws.wl_on) && blockDim.x >= 192;
-        if (spec) propose<NS, LF>(P, G, sh, lds, sh.mz_after + (wv - 1), sh.prop[wv - 1], true);
+        if (spec) propose<NS, LF>(P, G, sh, lds, sh.mz_after + (wv - 1), wv - 1, true);
         if (threadIdx.x == 64) sh.spec_on = spec;
       }
     }
