@@ -104,6 +104,8 @@ struct Smem {
   int team, member, k0, k1;
   long long xseq;
   int team_fail;
+  // number of the last evaluation whose phase tables are written (evaluate)
+  int tab_ready;
   // proposals: [0], [1] built speculatively for the next step assuming this
   // one is rejected after 0 / 1 draws of its decision, [2] the live one;
   // spec_on: [0] and [1] were built this step; mz_after: the ring position
@@ -1122,7 +1124,7 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
 
 template <int NS, typename T, bool LF, int NSITE, bool LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
-                         KSlot<NS> *ks, bool seam) {
+                         KSlot<NS> *ks, bool seam, const int evn) {
   using A = Acc<NS>;
   // LJ columns in LDS (stage_tables), in the precision T of the mode; read
   // from global memory (L2) they are FP64 in both modes: on gfx950 an FP32
@@ -1176,6 +1178,9 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
         }
       }
     }
+    // the rows of every lane are written (this wave's LDS writes complete in
+    // order): publish them under this evaluation's number
+    if (lane == 0) __hip_atomic_store(&sh.tab_ready, evn, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   double acc[A::NACC];
 #pragma unroll
@@ -1314,7 +1319,12 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
     }
   }
   FMC_PHASE(3);
-  __syncthreads();  // phase tables written
+  // the reciprocal terms need the phase tables only: a wave waits for the
+  // top wave's number rather than for every wave's pair terms, so that the
+  // waves holding k-vectors overlap them with the pair terms of the others
+  if (wave != nwaves - 1)
+    while (__hip_atomic_load(&sh.tab_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != evn)
+      __builtin_amdgcn_s_sleep(1);
   FMC_PHASE(4);
   // ---- reciprocal terms
   const int *kl;
@@ -2253,6 +2263,7 @@ __device__ inline void stage_in(const DevParams &P, const DevWalkers &W, int w, 
     sh.k1 = (int)((long long)P.nk * (member + 1) / team);
     sh.xseq = team > 1 ? W.xcnt[w] : 0;
     sh.team_fail = 0;
+    sh.tab_ready = 0;
     sh.spec_on = 0;
     sh.accepted = 0;
   }
@@ -2580,6 +2591,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
   // A swap takes two passes of the loop: its deletion (applied provisionally)
   // and the insertion of the other kind, accepted or undone as a whole; only
   // the second pass completes a step.
+  int evn = 0;  // evaluations made (every thread counts them alike)
   for (int step = 0; step < nsteps;) {
     if (sh.ws.wl_done) break;  // converged Wang-Landau walker (uniform: written before the last barrier)
     if (sh.team_fail) break;   // a team wait gave up (the walker is flagged)
@@ -2615,7 +2627,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
     FMC_PHASE(1);
     __syncthreads();
     FMC_PHASE(2);
-    if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false);
+    if (sh.kind != 0) evaluate<NS, T, LF, NSITE, LJL>(P, W, w, sh, lds, ks, false, ++evn);
     {
       // the next step's proposal, assuming this step is rejected, on waves 1
       // and 2 (for 0 and 1 draws of the decision) while wave 0 finishes the
@@ -2755,7 +2767,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     }
   }
   __syncthreads();
-  evaluate<NS, T, false, 0, false>(P, W, w, sh, lds, ks, true);
+  evaluate<NS, T, false, 0, false>(P, W, w, sh, lds, ks, true, 1);
   if (threadIdx.x == 0) {
     decide<NS>(P, sh, 0, lds, true);
     fmc_trial *o = W.trial_out;
