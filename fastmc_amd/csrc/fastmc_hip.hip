@@ -2276,15 +2276,9 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
   hipLaunchKernelGGL(k_grad_pairs, dim3(((nfw + gmax) * GPS + 255) / 256, nconf), dim3(256),
                      sizeof(double) * ((size_t)P.mxegrd + 3), h->stream, h->dp, A);
   HIPCHK(hipGetLastError());
-  std::vector<double> out(outn);
-  HIPCHK(hipMemcpyAsync(out.data(), h->g_out, sizeof(double) * outn, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  const double *urec = out.data(), *dq = urec + nconf, *e2 = dq + (size_t)nconf * nfw, *ev = e2 + nconf;
-  const double *dpe = ev + nconf, *dps = dpe + (size_t)nconf * NTC, *dqfwp = dps + (size_t)nconf * NTC;
-  std::vector<double> dqfw(nfw, 0.0);
-  for (int z = 0; z < KSFW; ++z)
-    for (int a = 0; a < nfw; ++a) dqfw[a] += dqfwp[(size_t)z * nfw + a];
-  // ---- closed forms (FP64): self term, charged-system term, ewald3, tail
+  // ---- closed forms (FP64, host, while the kernels run): self term,
+  // charged-system term, ewald3, tail and the tail's type gradients; kept per
+  // configuration and combined with the device sums below in the same order
   const double kq = R4PIE0 / P.epsq, C = P.dc_C, alpha = P.alpha, volm = P.volm, twopi = 2.0 * PI;
   const double rc = h->rcut, rc3 = rc * rc * rc, rc9 = rc3 * rc3 * rc3;
   double q2fw = 0.0, qfw = 0.0;
@@ -2297,6 +2291,8 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     nfwt[h->fw_type[a]] += 1;
     ffwt[h->fw_type[a]] += h->fw_frz[a] ? 1 : 0;
   }
+  std::vector<double> c_sic(nconf), c_fix(nconf), c_e3(nconf), c_lrc(nconf), c_qs(nconf);
+  std::vector<double> c_te((size_t)nconf * TT, 0.0), c_ts((size_t)nconf * TT, 0.0);
   for (int c = 0; c < nconf; ++c) {
     double q2 = q2fw, qs = qfw, e3 = 0.0;
     std::vector<double> n(TT, 0.0), f(TT, 0.0);
@@ -2318,7 +2314,7 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
     const double qfix = -(0.5 * PI * kq) * ((qs / alpha) * (qs / alpha) / volm);
     // tail (lrcorrect, vdw_terms.f:581-645) with theta pairs mixed
     double elrc = 0.0;
-    std::vector<double> gte(TT, 0.0), gts(TT, 0.0);
+    double *gte = &c_te[(size_t)c * TT], *gts = &c_ts[(size_t)c * TT];
     for (int i = 1; i <= nt; ++i)
       for (int j = 1; j <= i; ++j) {
         double w = twopi * (n[i] * n[j] - f[i] * f[j]) / volm;
@@ -2349,6 +2345,24 @@ int fmc_energy_grad(fmc_handle *h, const fmc_theta *theta, int nconf, const int 
           }
         }
       }
+    c_sic[c] = engsic;
+    c_fix[c] = qfix;
+    c_e3[c] = e3;
+    c_lrc[c] = elrc;
+    c_qs[c] = qs;
+  }
+  // ---- device sums
+  std::vector<double> out(outn);
+  HIPCHK(hipMemcpyAsync(out.data(), h->g_out, sizeof(double) * outn, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double *urec = out.data(), *dq = urec + nconf, *e2 = dq + (size_t)nconf * nfw, *ev = e2 + nconf;
+  const double *dpe = ev + nconf, *dps = dpe + (size_t)nconf * NTC, *dqfwp = dps + (size_t)nconf * NTC;
+  std::vector<double> dqfw(nfw, 0.0);
+  for (int z = 0; z < KSFW; ++z)
+    for (int a = 0; a < nfw; ++a) dqfw[a] += dqfwp[(size_t)z * nfw + a];
+  for (int c = 0; c < nconf; ++c) {
+    const double engsic = c_sic[c], qfix = c_fix[c], e3 = c_e3[c], elrc = c_lrc[c], qs = c_qs[c];
+    double *gte = &c_te[(size_t)c * TT], *gts = &c_ts[(size_t)c * TT];
     // pair LJ gradients, chain rule to the type parameters
     for (int i = 1; i <= nt; ++i)
       for (int j = 1; j <= i; ++j) {

@@ -87,7 +87,7 @@ def stats_from_windows(windows, nwin: int, ntpguest: int, temperature: float):
 
 
 PHASE_NAMES = ("statistics and records at the step's end", "proposal (or commit of a speculative one)",
-               "barrier after the proposal", "pair terms (thread 0's share)", "barrier after the pair terms",
+               "barrier after the proposal", "pair terms (thread 0's share)", "wait for the phase tables",
                "reciprocal terms (thread 0's share)", "wave sums", "barrier after the wave sums", "cross-wave sums",
                "team exchange", "until the decision", "decision", "counters", "barrier after the decision",
                "accept")
