@@ -355,7 +355,7 @@ template <int NS, typename T>
 hipError_t allow_lds(int bytes) {
   const void *fns[] = {(const void *)k_run<NS, T, false, 0, false>, (const void *)k_run<NS, T, false, 0, true>,
                        (const void *)k_run<NS, T, true, 3, false>, (const void *)k_run<NS, T, true, 0, false>,
-                       (const void *)k_trial<NS, T>,
+                       (const void *)k_trial<NS, T, false>, (const void *)k_trial<NS, T, true>,
                        (const void *)k_accept<NS>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -381,9 +381,14 @@ void run_dispatch(fmc_handle *h, int nsteps, int count) {
   }
 }
 
+// The seam trial takes the LJ arithmetic of the handle's chain kernel: FP32
+// columns (mixed mode) where the chains stage them in LDS, FP64 otherwise.
 template <int NS, typename T>
 void launch_trial(fmc_handle *h, int w) {
-  hipLaunchKernelGGL((k_trial<NS, T>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w);
+  if (run_variant(h) == 1)
+    hipLaunchKernelGGL((k_trial<NS, T, true>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w);
+  else
+    hipLaunchKernelGGL((k_trial<NS, T, false>), dim3(1), dim3(h->block), h->hp.lds_bytes, h->stream, h->dp, h->dw, w);
 }
 
 void trial_dispatch(fmc_handle *h, int w) {
@@ -957,10 +962,11 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
     }
   }
   // ---- LDS plan.  Always: Smem, erfc table, phase tables, constants block.
-  // The all-in-LDS variant also needs the framework, the k list, every guest
-  // slot and, in FP32 mode, the LJ columns of the guest-site types; without
-  // room for all of them (or with FMC_NO_LDS_ALL set) the kernels read those
-  // from global memory.
+  // The all-in-LDS variant also needs the framework, the k list and every
+  // guest slot (its LJ columns are read from L2, in FP64); without room for
+  // all of them (or with FMC_NO_LDS_ALL set) the kernels read those from
+  // global memory and stage the LJ columns of the guest-site types in LDS
+  // when they fit.
   {
     const size_t budget = 160 * 1024 - 1024;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
@@ -1039,20 +1045,11 @@ int fmc_create(const fmc_system *sys, int nwalkers, const fmc_walker *walkers, i
       o = al(o + (size_t)12 * P.nk);
       int og = (int)o;
       o = al(o + (size_t)24 * P.gatoms_total);
-      int ov64 = (int)o, ov = (int)o;
-      if (precision == FMC_MIXED) {
-        const size_t nc = (size_t)std::max(ncol, 1);
-        o = al(o + (size_t)8 * nc * P.lj_n64);
-        ov = (int)o;
-        o = al(o + (size_t)4 * nc * P.vstride);
-      }
       if (o <= budget) {
         P.lds_all = 1;
         P.off_fw = ofw;
         P.off_k = ok;
         P.off_g = og;
-        P.off_vvv64 = ov64;
-        P.off_vvv = ov;
         off = o;
       }
     }

@@ -1122,16 +1122,18 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
   return t;
 }
 
-template <int NS, typename T, bool LF, int NSITE, bool LJL>
+template <int NS, typename T, bool LF, int NSITE, bool LJL, bool LJ32 = LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                          KSlot<NS> *ks, bool seam, const int evn) {
   using A = Acc<NS>;
   // LJ columns in LDS (stage_tables), in the precision T of the mode; read
   // from global memory (L2) they are FP64 in both modes: on gfx950 an FP32
   // interpolation issues at the FP64 rate and adds the conversions and the
-  // wall test, so FP32 values pay only where they halve the LDS they take
+  // wall test, so FP32 values pay only where they halve the LDS they take.
+  // LJ32: the interpolation in T from the global FP32 columns (the seam
+  // trial of a handle whose chains stage FP32 columns, same arithmetic)
   constexpr bool LV = LJL;
-  using TP = typename std::conditional<LV, T, double>::type;
+  using TP = typename std::conditional<LJ32, T, double>::type;
   constexpr int NSR = NSITE > 0 ? NSITE : KSITE;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int natms = uni(sh.natms), mol = uni(sh.mol), g = uni(sh.g);
@@ -1193,7 +1195,7 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   const int *lst = (const int *)(lds + P.off_cst + CST_LST);
   LJTab vvv;
   if constexpr (LV) vvv = lj_lds<T>(P, lds);
-  else vvv = lj_global(P, false);
+  else vvv = lj_global(P, LJ32 && sizeof(T) == 4);
   // NSITE > 0: site charges and overlap thresholds in registers; padding
   // sites (s >= natms) carry charge 0, no LJ column, no overlap or surface
   // test, so whatever their LDS coordinates hold is selected away.  The site
@@ -2736,7 +2738,7 @@ __global__ void __launch_bounds__(512) k_run(const DevParams *__restrict__ Pp, D
 // One trial of the mc_moves.f seam: the proposal comes from the pending
 // fields of walker w (written by the host); results go to W.trial_out and
 // s_new / s_old of every k-vector to the walker's scratch rows.
-template <int NS, typename T>
+template <int NS, typename T, bool LJ32>
 __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp, DevWalkers W, int w) {
   extern __shared__ __align__(16) unsigned char lds[];
   Smem<NS> &sh = *reinterpret_cast<Smem<NS> *>(lds);
@@ -2767,7 +2769,7 @@ __global__ void __launch_bounds__(512) k_trial(const DevParams *__restrict__ Pp,
     }
   }
   __syncthreads();
-  evaluate<NS, T, false, 0, false>(P, W, w, sh, lds, ks, true, 1);
+  evaluate<NS, T, false, 0, false, LJ32>(P, W, w, sh, lds, ks, true, 1);
   if (threadIdx.x == 0) {
     decide<NS>(P, sh, 0, lds, true);
     fmc_trial *o = W.trial_out;
