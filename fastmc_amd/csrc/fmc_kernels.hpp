@@ -1051,15 +1051,6 @@ __device__ void prepare(const DevParams &P, Smem<NS> &sh) {
   else sh.pref = 1.0;
 }
 
-// Trial energy terms of insertion / deletion / guest_energy (mc_moves.f:150-484)
-// with ewald2 (ewald.f:663-733), srfrce (vdw_terms.f:382-466) and ewald1_guest
-// (ewald.f:32-352).  Every thread of the workgroup takes part: atoms of the
-// pair terms from the bottom thread up, k-vectors from the top down (register
-// slots ks, then any k beyond KPT per thread through global memory); the
-// reduced sums land in sh.tot.  LF: framework, k list, guests (and in FP32
-// the LJ columns) are read from LDS.  NSITE > 0: guests have at most NSITE
-// sites (padding sites masked) and the site loops are straight-line code.
-// seam: also write s_new / s_old of the register slots to the scratch rows.
 // The team's sums of one step (lanes < NRED of wave 0, v = this member's
 // sum of row `lane`): each member publishes its row sums in its record of
 // the exchange's parity and gathers the other members' rows; the rows are
@@ -1122,6 +1113,17 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
   return t;
 }
 
+// Trial energy terms of insertion / deletion / guest_energy (mc_moves.f:150-484)
+// with ewald2 (ewald.f:663-733), srfrce (vdw_terms.f:382-466) and ewald1_guest
+// (ewald.f:32-352).  Every thread of the workgroup takes part: atoms of the
+// pair terms from the bottom thread up, k-vectors from the top down (register
+// slots ks, then any k beyond KPT per thread through global memory); the
+// reduced sums land in sh.tot.  LF: framework, k list and guests are read
+// from LDS; LJL: the LJ columns are.  NSITE > 0: guests have at most NSITE
+// sites (padding sites masked) and the site loops are straight-line code.
+// seam: also write s_new / s_old of the register slots to the scratch rows.
+// evn: the number of this evaluation in the launch (every thread counts them
+// alike); the top wave publishes the phase tables under it.
 template <int NS, typename T, bool LF, int NSITE, bool LJL, bool LJ32 = LJL>
 __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W, int w, Smem<NS> &sh, unsigned char *lds,
                          KSlot<NS> *ks, bool seam, const int evn) {
