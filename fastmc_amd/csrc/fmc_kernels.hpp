@@ -1325,10 +1325,23 @@ __device__ __forceinline__ void evaluate(const DevParams &P, const DevWalkers &W
   FMC_PHASE(3);
   // the reciprocal terms need the phase tables only: a wave waits for the
   // top wave's number rather than for every wave's pair terms, so that the
-  // waves holding k-vectors overlap them with the pair terms of the others
-  if (wave != nwaves - 1)
-    while (__hip_atomic_load(&sh.tab_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != evn)
+  // waves holding k-vectors overlap them with the pair terms of the others.
+  // The top wave publishes unconditionally; the wait is bounded all the same
+  // (XSPIN polls, then the walker is flagged as a team wait that gave up)
+  if (wave != nwaves - 1) {
+    long long spin = 0;
+    while (__hip_atomic_load(&sh.tab_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != evn) {
+      if (__builtin_expect(++spin > XSPIN, 0)) {
+        if (lane == 0) {
+          atomicOr(&sh.ws.overflow, FMC_OVF_TEAM);
+          sh.team_fail = 1;
+          if (team > 1) *W.xerr = 1;
+        }
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
+    }
+  }
   FMC_PHASE(4);
   // ---- reciprocal terms
   const int *kl;
