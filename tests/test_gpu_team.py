@@ -154,3 +154,42 @@ def test_gpu_team_wang_landau_windows_follow_oracle():
             o.close()
     finally:
         h.close()
+
+
+def test_gpu_team_size_changes_follow_oracle():
+    """One chain continued through runs of different team sizes on one handle
+    (4, 2, 4, 3, 1, 4): every change clears the exchange records, whose row
+    values carry their exchange in two mantissa bits, so no record of an
+    earlier team can pass for a current one.  The chain is the oracle's."""
+    from fastmc_amd import inputs as I
+    from fastmc_amd.lib import FMC_FP64, Handle
+    from fastmc_amd.system import read_system
+    from test_gpu_config2 import first_difference
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="fmc_teamchg_")
+    try:
+        I.config0(d, pressure_bar=10.0)
+        sysm = read_system(d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    seeds = (31, 47, 59, 71)
+    teams, n = (4, 2, 4, 3, 1, 4), 150
+    say(f"config 0 at 10 bar, FP64: {n} steps per run with teams {teams}")
+    h = Handle(sysm, precision=FMC_FP64, seeds=[seeds])
+    parts = []
+    for t in teams:
+        h.set_team(t)
+        assert h.get_team() == t
+        parts.append(h.run(n, trace=True))
+    tr = np.concatenate(parts)
+    o = O.Oracle(sysm, seed=seeds)
+    otr = o.run(n * len(teams))
+    diff = first_difference(tr, otr, sysm.temperature)
+    assert diff is None, diff
+    assert h.stats(0).overflow == 0
+    np.testing.assert_allclose(h.guests(1), o.guests(1), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(h.sk(), o.sk(), rtol=0, atol=1e-8)
+    say(f"  {n * len(teams)} steps identical, N={len(o.guests(1)) // 3}")
+    o.close()
+    h.close()
