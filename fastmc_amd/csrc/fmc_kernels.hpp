@@ -1083,21 +1083,58 @@ __device__ __forceinline__ double team_exchange(const DevWalkers &W, int w, Smem
     sh.red[member][lane] = __longlong_as_double(b & ~3LL);
   }
   int fail = 0;
-  for (int i = lane; i < team * NR; i += 64) {
-    const int q = i / NR, row = i - q * NR;
-    if (q == member) continue;
-    const long long *src = recs + (size_t)q * XREC + row;
-    long long b, spin = 0;
-    while (((b = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 3) != tag) {
+  if (team * NR <= 64) {
+    // one row per lane: lane q * NR + row polls member q's row
+    const int q = lane / NR, row = lane - q * NR;
+    if (lane < team * NR && q != member) {
+      const long long *src = recs + (size_t)q * XREC + row;
+      long long b, spin = 0;
+      while (((b = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 3) != tag) {
+        if (__builtin_expect(++spin > XSPIN, 0)) {
+          fail = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      sh.red[q][row] = __longlong_as_double(b & ~3LL);
+    }
+    fail = __any(fail);
+  } else {
+    // lane takes the rows lane, lane + 64, ... of the other members (RMAX at
+    // most); each poll reads every row it still lacks at once, so that a
+    // large team costs one round trip per poll, not one per row of a lane
+    constexpr int RMAX = (KTEAM * NR + 63) / 64;
+    static_assert(RMAX <= 31, "row mask");
+    int need = 0;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int i = lane + 64 * r, q = i / NR;
+      if (i < team * NR && q != member) need |= 1 << r;
+    }
+    long long spin = 0;
+    while (true) {
+      long long got[RMAX];
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r)
+        if ((need >> r) & 1) {
+          const int i = lane + 64 * r, q = i / NR, row = i - q * NR;
+          got[r] = __hip_atomic_load(recs + (size_t)q * XREC + row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r)
+        if (((need >> r) & 1) && (got[r] & 3) == tag) {
+          const int i = lane + 64 * r, q = i / NR, row = i - q * NR;
+          sh.red[q][row] = __longlong_as_double(got[r] & ~3LL);
+          need &= ~(1 << r);
+        }
+      if (!__any(need)) break;
       if (__builtin_expect(++spin > XSPIN, 0)) {
         fail = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    sh.red[q][row] = __longlong_as_double(b & ~3LL);
   }
-  fail = __any(fail);
   __builtin_amdgcn_wave_barrier();  // the rows other lanes staged are read below
   double t = 0.0;
   if (lane < NR)
