@@ -22,9 +22,9 @@ constexpr int KTRI = KTYPE * (KTYPE + 1) / 2;
 constexpr int KWAVES = 16;        // up to 1024 threads per workgroup
 // Teams: a walker's step split over up to KTEAM workgroups (pair atoms and
 // k-vectors by member), their partial sums exchanged once per step through
-// XREC-double records in global memory, the last slot of each carrying the
-// exchange number.  A wait that sees nothing for XSPIN polls gives up and
-// flags the walker (FMC_OVF_TEAM), so that no wait is unbounded.
+// XREC-double records in global memory, each value tagged with its exchange
+// in its low mantissa bits.  A wait that sees nothing for XSPIN polls gives
+// up and flags the walker (FMC_OVF_TEAM), so that no wait is unbounded.
 constexpr int KTEAM = 16;
 constexpr int XREC = 32;
 constexpr long long XSPIN = 1LL << 22;

@@ -478,7 +478,8 @@ int fmc_get_prob(fmc_handle *h, int w, int plot, double *out);
  * walker 0 (team member 0), the time of each phase of every GCMC step in
  * ticks of the GPU's 100 MHz real-time counter: ticks[0] statistics and
  * records at the step's end, [1] proposal (or commit of a speculative one),
- * [2] barrier, [3] pair terms (thread 0's share), [4] barrier, [5] reciprocal
+ * [2] barrier, [3] pair terms (thread 0's share), [4] wait for the phase
+ * tables, [5] reciprocal
  * terms (thread 0's share), [6] wave sums, [7] barrier, [8] cross-wave sums,
  * [9] team exchange, [10] until the decision starts, [11] decision, [12]
  * counters, [13] barrier, [14] accept; ticks[16] counts the steps and
