@@ -151,3 +151,44 @@ def test_gpu_fluctuation_estimator(setup):
     rel = np.abs(gm - g[: len(q)]).max() / max(np.abs(g[: len(q)]).max(), 1e-300)
     say("d<N>/dq: mixed vs FP64 max rel", rel, "| <N> =", N.mean())
     assert rel <= 1e-4
+
+
+def test_gpu_energy_grad_config3_batch():
+    """The config-3 batch the bench times: 256 configurations sampled every 200
+    steps after 20 000 at 1 bar, one fmc_energy_grad call.  Six of them spread
+    over the batch against the FP64 numpy oracle; the batch call equals
+    one-configuration calls (the blocked contraction and the k-range partial
+    sums do not mix configurations); mixed mode over all 256 within the
+    bounds of the 24-configuration test."""
+    from fastmc_amd import adjoint as A
+    from fastmc_amd.lib import FMC_FP64, FMC_MIXED, Handle
+    sysm = _system(1.0)
+    say("sampling 256 configurations at 1 bar on the device")
+    nmols, xyz = A.sample_configurations(sysm, 256, every=200, warmup=20000)
+    say("N range", int(nmols[:, 0].min()), int(nmols[:, 0].max()))
+    h = Handle(sysm, precision=FMC_FP64)
+    try:
+        q, eps, sig = A.theta0(sysm)
+        U, dq, de, ds = h.energy_grad(q, eps, sig, nmols, xyz, FMC_FP64)
+        model = EnergyModel(sysm)
+        for c in (0, 51, 102, 153, 204, 255):
+            Ur, dqr, der, dsr = model.energy_grad(q, eps, sig, nmols[c], xyz[c])
+            assert abs(U[c] - Ur) <= 1e-10 * max(1.0, abs(Ur)), (c, U[c], Ur)
+            assert np.abs(dq[c] - dqr).max() <= 1e-10 * np.abs(dqr).max(), c
+            np.testing.assert_allclose(de[c], der, rtol=1e-10, atol=1e-12 * np.abs(der).max())
+            np.testing.assert_allclose(ds[c], dsr, rtol=1e-10, atol=1e-12 * np.abs(dsr).max())
+        for c in (3, 130, 254):
+            U1, dq1, de1, ds1 = h.energy_grad(q, eps, sig, nmols[c: c + 1], xyz[c: c + 1], FMC_FP64)
+            assert abs(U1[0] - U[c]) <= 1e-12 * max(1.0, abs(U[c])), c
+            np.testing.assert_allclose(dq1[0], dq[c], rtol=0, atol=1e-12 * np.abs(dq[c]).max())
+            np.testing.assert_allclose(de1[0], de[c], rtol=1e-12, atol=1e-14 * np.abs(de[c]).max())
+            np.testing.assert_allclose(ds1[0], ds[c], rtol=1e-12, atol=1e-14 * np.abs(ds[c]).max())
+        Um, dqm, dem, dsm = h.energy_grad(q, eps, sig, nmols, xyz, FMC_MIXED)
+        np.testing.assert_allclose(U, Um, rtol=1e-14)
+        rel = np.abs(dqm - dq).max() / np.abs(dq).max()
+        say("256 configurations, mixed: max |dU/dq error| / max |dU/dq| =", rel)
+        assert rel <= 1e-5
+        np.testing.assert_allclose(de, dem, rtol=1e-13, atol=1e-13 * np.abs(de).max())
+        np.testing.assert_allclose(ds, dsm, rtol=1e-13, atol=1e-13 * np.abs(ds).max())
+    finally:
+        h.close()
