@@ -16,7 +16,7 @@ seeds and its printed <N> +/- block-average error (gcmc.f:1458-1557).
   what the precision does: |mean difference| <= 3 standard errors of the
   paired differences (most mixed chains never take a different decision
   from their FP64 chain, and then the difference is exactly zero).  The
-  north star's criterion, <N>_mixed within one reference block-average error
+  loading criterion for mixed precision, <N>_mixed within one reference block-average error
   of the reference mean, is checked too; z against the reference mean is
   printed for both precisions.  The reference mean of 16 runs is itself a
   sample: at 10 bar it lies 2.2 molecules (about 3.5 of its standard errors)
